@@ -1,0 +1,256 @@
+// C-ABI entry points (include/cssx.h): context, plan cache, derivative launches.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cssx.h"
+#include "cssx_internal.hpp"
+#include "cssx_kernels.hpp"
+#include "cssx_plan.hpp"
+
+namespace cssx {
+
+// ---------------------------------------------------------------------------------------------
+// Plan cache: one device buffer per (device, plan key, precision), built on first use and kept
+// until the process ends (plans are a few KB each).
+// ---------------------------------------------------------------------------------------------
+namespace {
+std::mutex g_plan_mu;
+std::map<std::tuple<int, PlanKey, int>, void *> g_plans;
+}  // namespace
+
+int get_plan(const PlanKey &key, int prec, const void **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CSSX_EHIP;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto tk = std::make_tuple(dev, key, prec);
+    auto it = g_plans.find(tk);
+    if (it != g_plans.end()) {
+        *out = it->second;
+        return CSSX_OK;
+    }
+    const int M = choose_M(key.n);
+    if (M < 0) return CSSX_EINVAL;
+    const int S = seg_count(key.n, M);
+    std::vector<double> plan;
+    build_plan(key, S, plan);
+    void *d = nullptr;
+    size_t bytes = plan.size() * (prec == CSSX_PREC_F64 ? sizeof(double) : sizeof(float));
+    if (hipMalloc(&d, bytes) != hipSuccess) return CSSX_ENOMEM;
+    hipError_t e;
+    if (prec == CSSX_PREC_F64) {
+        e = hipMemcpy(d, plan.data(), bytes, hipMemcpyHostToDevice);
+    } else {
+        std::vector<float> pf(plan.begin(), plan.end());
+        e = hipMemcpy(d, pf.data(), bytes, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return CSSX_EHIP;
+    }
+    g_plans[tk] = d;
+    *out = d;
+    return CSSX_OK;
+}
+
+// Pencil geometry of an axis on the reference layout f(nth, nphi, nr).
+void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g) {
+    const long sth = 1, sph = nth, sr = (long)nth * nphi;
+    if (axis == CSSX_AXIS_R) {          // pencils (it, ip), lanes over theta
+        g.n = nr; g.na = nth; g.nb = nphi; g.sa = sth; g.sb = sph; g.srow = sr;
+        g.g_k = (long)nth * nphi; g.g_a = 1; g.g_b = nth;
+    } else if (axis == CSSX_AXIS_THETA) {   // pencils (ip, ir), lanes over phi
+        g.n = nth; g.na = nphi; g.nb = nr; g.sa = sph; g.sb = sr; g.srow = sth;
+        g.g_k = 1; g.g_a = 4; g.g_b = 4L * nphi;
+    } else {                            // pencils (it, ir), lanes over theta
+        g.n = nphi; g.na = nth; g.nb = nr; g.sa = sth; g.sb = sr; g.srow = sph;
+        g.g_k = nth; g.g_a = 1; g.g_b = 4L * nth;
+    }
+}
+
+template <typename T>
+static void fill_launch_geom(DerivLaunch<T> &L, const AxisGeom &g) {
+    L.n = g.n;
+    L.S = seg_count(g.n, choose_M(g.n));
+    L.na = g.na; L.nb = g.nb; L.sa = g.sa; L.sb = g.sb; L.srow = g.srow;
+    L.njobs = 0;
+}
+
+template <typename T>
+static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStream_t stream) {
+    DerivLaunch<T> L;
+    fill_launch_geom(L, g);
+    for (int i = 0; i < nspec; i++) {
+        const DerivSpec &s = (&sp)[i];
+        PlanKey key{g.n, s.order, s.lo_kind == END_WALL ? END_WALL : END_HALO,
+                    s.hi_kind == END_WALL ? END_WALL : END_HALO, (s.lo_kind == END_WALL || s.hi_kind == END_WALL) ? s.bdry : 0};
+        const void *plan = nullptr;
+        int rc = get_plan(key, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
+        if (rc) return rc;
+        DerivJob<T> &J = L.jobs[L.njobs++];
+        J.in = (const T *)s.in; J.out = (T *)s.out; J.rhs_out = (T *)s.rhs_out;
+        J.glo = (const T *)s.glo; J.ghi = (const T *)s.ghi;
+        J.g_k = s.g_k >= 0 ? s.g_k : g.g_k; J.g_a = s.g_k >= 0 ? s.g_a : g.g_a; J.g_b = s.g_k >= 0 ? s.g_b : g.g_b;
+        J.plan = (const T *)plan;
+        J.pre = (const T *)s.pre; J.post = (const T *)s.post;
+        J.scale = (T)s.scale; J.neu_mul = (T)s.neu_mul; J.halo_s = (T)s.halo_s;
+        J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
+        J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind;
+        if (L.njobs == kMaxJobs || i == nspec - 1) {
+            hipError_t e = launch_deriv<T>(L, s.order, stream);
+            if (e != hipSuccess) return CSSX_EHIP;
+            L.njobs = 0;
+        }
+    }
+    return CSSX_OK;
+}
+
+// Batched derivative launch; all specs share the axis geometry and the order.
+int run_derivs(const DerivSpec *specs, int nspec, const AxisGeom &g, int prec, hipStream_t stream) {
+    if (nspec <= 0) return CSSX_OK;
+    for (int i = 1; i < nspec; i++)
+        if (specs[i].order != specs[0].order) return CSSX_EINVAL;
+    if (choose_M(g.n) < 0 || g.n < 8) return CSSX_EINVAL;
+    return prec == CSSX_PREC_F64 ? deriv_impl<double>(specs[0], g, nspec, stream)
+                                 : deriv_impl<float>(specs[0], g, nspec, stream);
+}
+
+}  // namespace cssx
+
+using namespace cssx;
+
+struct cssx_ctx {
+    cssx_grid grid;
+    int device;
+};
+
+extern "C" int cssx_version(void) { return 100; }
+
+extern "C" const char *cssx_strerror(int code) {
+    switch (code) {
+    case CSSX_OK: return "ok";
+    case CSSX_ENOMEM: return "allocation failed";
+    case CSSX_EINVAL: return "invalid argument (shape, boundary code, axis or pointer)";
+    case CSSX_ENAN: return "non-finite value detected";
+    case CSSX_EHIP: return "HIP runtime error";
+    default: return "unknown error";
+    }
+}
+
+extern "C" int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out) {
+    if (!grid || !out) return CSSX_EINVAL;
+    *out = nullptr;
+    if (grid->nr < 8 || grid->nth < 8 || grid->nphi < 8) return CSSX_EINVAL;
+    if (choose_M(grid->nr) < 0 || choose_M(grid->nth) < 0 || choose_M(grid->nphi) < 0) return CSSX_EINVAL;
+    cssx_ctx *c = new (std::nothrow) cssx_ctx;
+    if (!c) return CSSX_ENOMEM;
+    c->grid = *grid;
+    if (hipGetDevice(&c->device) != hipSuccess) {
+        delete c;
+        return CSSX_EHIP;
+    }
+    // build every plan this grid can ask for, so that later calls do not allocate
+    int ns[3] = {grid->nr, grid->nth, grid->nphi};
+    for (int ax = 0; ax < 3; ax++)
+        for (int order = 1; order <= 2; order++)
+            for (int lo = 0; lo < 2; lo++)
+                for (int hi = 0; hi < 2; hi++)
+                    for (int bd = 0; bd < 5; bd++) {
+                        if (ax == CSSX_AXIS_PHI && (lo == 0 || hi == 0)) continue;
+                        if (lo == 1 && hi == 1 && bd > 0) continue;
+                        PlanKey key{ns[ax], order, lo ? END_HALO : END_WALL, hi ? END_HALO : END_WALL, bd};
+                        const void *p;
+                        for (int prec = 0; prec < 2; prec++) {
+                            int rc = get_plan(key, prec, &p);
+                            if (rc) {
+                                delete c;
+                                return rc;
+                            }
+                        }
+                    }
+    *out = c;
+    return CSSX_OK;
+}
+
+extern "C" void cssx_ctx_destroy(cssx_ctx *ctx) { delete ctx; }
+
+extern "C" int cssx_deriv(cssx_ctx *ctx, int axis, int order, int bdry, int prec, const void *in, void *out,
+                          const void *ghost_lo, const void *ghost_hi, void *stream) {
+    if (!ctx || !in || !out || axis < 0 || axis > 2 || (order != 1 && order != 2) || bdry < 0 || bdry > 4 ||
+        (prec != CSSX_PREC_F64 && prec != CSSX_PREC_F32))
+        return CSSX_EINVAL;
+    const cssx_grid &gr = ctx->grid;
+    AxisGeom g;
+    axis_geometry(axis, gr.nth, gr.nphi, gr.nr, g);
+    DerivSpec s;
+    s.order = order;
+    s.in = in; s.out = out;
+    s.glo = ghost_lo; s.ghi = ghost_hi;
+    s.bdry = bdry;
+    double inv;
+    if (axis == CSSX_AXIS_R) {
+        s.lo_kind = gr.r_lo_wall ? END_WALL : END_HALO;
+        s.hi_kind = gr.r_hi_wall ? END_WALL : END_HALO;
+        inv = gr.dri;
+    } else if (axis == CSSX_AXIS_THETA) {
+        s.lo_kind = gr.th_lo_wall ? END_WALL : END_HALO;
+        s.hi_kind = gr.th_hi_wall ? END_WALL : END_HALO;
+        inv = gr.dti;
+    } else {
+        s.lo_kind = ghost_lo ? END_HALO : END_PERIODIC;
+        s.hi_kind = ghost_hi ? END_HALO : END_PERIODIC;
+        inv = gr.dpi;
+        s.bdry = 0;
+    }
+    if ((s.lo_kind == END_HALO && !ghost_lo) || (s.hi_kind == END_HALO && !ghost_hi)) return CSSX_EINVAL;
+    s.scale = order == 1 ? inv : inv * inv;
+    // a prescribed wall derivative enters the rows divided by dri (D2bydt2 also uses dri, T6)
+    s.neu_mul = order == 1 ? 1.0 / inv : 1.0 / gr.dri;
+    return run_derivs(&s, 1, g, prec, (hipStream_t)stream);
+}
+
+extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
+                                const double *b1, const double *b2, double *darr, double *result,
+                                void *stream) {
+    if (n < 8 || npencils < 1 || !arr || !result || ibc < 0 || ibc > 5 || dtype < 0 || dtype > 3 ||
+        choose_M(n) < 0 || npencils > (long)INT32_MAX)
+        return CSSX_EINVAL;
+    const bool lo_halo = (dtype == 1 || dtype == 3), hi_halo = (dtype == 2 || dtype == 3);
+    if ((lo_halo && !b1) || (hi_halo && !b2)) return CSSX_EINVAL;
+    const double dli = (ibc == 5) ? (double)n : (double)(n - 1);
+    // ibc 5 behaves as 3 at the walls (derivatives.py:82-115, :130-160: odd -> Dirichlet lower,
+    // not 1/2 -> Neumann upper) and only changes dli.
+    const int bd = (ibc == 5) ? 3 : ibc;
+    AxisGeom g;
+    g.n = n; g.na = (int)npencils; g.nb = 1; g.sa = 1; g.sb = 0; g.srow = npencils;
+    g.g_k = npencils; g.g_a = 1; g.g_b = 0;
+    DerivSpec s;
+    s.order = 1;
+    s.in = arr; s.out = result; s.rhs_out = darr; s.rhs_scale = dli;
+    s.lo_kind = lo_halo ? END_HALO : END_WALL;
+    s.hi_kind = hi_halo ? END_HALO : END_WALL;
+    // wall ends read the preloaded Neumann values (f'/dri) from darr rows 0 / n-1
+    s.glo = lo_halo ? (const void *)b1 : (const void *)darr;
+    s.ghi = hi_halo ? (const void *)b2 : (const void *)(darr ? darr + (long)(n - 1) * npencils : nullptr);
+    s.bdry = bd;
+    s.scale = dli * dri;
+    s.neu_mul = 1.0 / dli;
+    s.halo_s = dli / 60.0;   // derivatives.py:91,141,177,181 multiply the whole halo row by dli/60
+    return run_derivs(&s, 1, g, CSSX_PREC_F64, (hipStream_t)stream);
+}
+
+extern "C" int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, double *plan_out,
+                                int *nseg) {
+    if (!plan_out || n < 8 || choose_M(n) < 0 || (order != 1 && order != 2) || bdry < 0 || bdry > 4 ||
+        lo_kind < 0 || lo_kind > 1 || hi_kind < 0 || hi_kind > 1)
+        return CSSX_EINVAL;
+    const int S = seg_count(n, choose_M(n));
+    std::vector<double> plan;
+    build_plan(PlanKey{n, order, lo_kind, hi_kind, bdry}, S, plan);
+    for (size_t i = 0; i < plan.size(); i++) plan_out[i] = plan[i];
+    if (nseg) *nseg = S;
+    return CSSX_OK;
+}

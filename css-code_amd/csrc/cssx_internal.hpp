@@ -1,0 +1,34 @@
+// Internal host-side interfaces shared by the C-ABI sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cssx_plan.hpp"
+
+namespace cssx {
+
+// Pencil geometry of one axis on the reference layout (see axis_geometry).
+struct AxisGeom {
+    int n = 0, na = 0, nb = 0;
+    long sa = 0, sb = 0, srow = 0;
+    long g_k = 0, g_a = 0, g_b = 0;   // default ghost strides (reference halo layout)
+};
+
+// One derivative request (precision-neutral; pointers are device pointers).
+struct DerivSpec {
+    int order = 1;
+    const void *in = nullptr;
+    void *out = nullptr;
+    void *rhs_out = nullptr;
+    const void *glo = nullptr, *ghi = nullptr;
+    long g_k = -1, g_a = 0, g_b = 0;      // -1: use the axis default ghost layout
+    const void *pre = nullptr, *post = nullptr;
+    double scale = 1.0, neu_mul = 1.0, halo_s = 1.0, rhs_scale = 1.0, beta = 0.0;
+    int bdry = 0;
+    int lo_kind = END_WALL, hi_kind = END_WALL;
+};
+
+void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g);
+int get_plan(const PlanKey &key, int prec, const void **out);
+int run_derivs(const DerivSpec *specs, int nspec, const AxisGeom &g, int prec, hipStream_t stream);
+
+}  // namespace cssx

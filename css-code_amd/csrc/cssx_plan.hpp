@@ -1,0 +1,78 @@
+// Host-side plans for the segmented compact solve.
+//
+// A compact derivative along one axis solves, for every pencil, the tridiagonal system
+//   l_j f'_{j-1} + f'_j + u_j f'_{j+1} = d_j,   j = 0..n-1
+// whose coefficients depend only on the pencil length, the derivative order, the closure at
+// each end (physical wall or 4-deep halo) and the boundary code.  The reference precomputes
+// Thomas-LU tables for that system once (derivatives.f90:106-518) and sweeps every pencil
+// serially.  Here the pencil is cut into S segments, one wavefront each: every segment solves
+// its own rows with zero coupling to its neighbours, the S-1 interface rows form a small
+// tridiagonal system, and the segment solutions are corrected by the interface values.
+// All of that is linear algebra on the coefficients alone, so it is precomputed here in FP64
+// (per row: local LU factors and the two response vectors; per interface: the LU factors of
+// the reduced system) and the GPU only streams the data.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace cssx {
+
+enum EndKind : int { END_WALL = 0, END_HALO = 1, END_PERIODIC = 2 };
+
+struct PlanKey {
+    int n, order, lo, hi, bdry;
+    bool operator<(const PlanKey &o) const {
+        if (n != o.n) return n < o.n;
+        if (order != o.order) return order < o.order;
+        if (lo != o.lo) return lo < o.lo;
+        if (hi != o.hi) return hi < o.hi;
+        return bdry < o.bdry;
+    }
+};
+
+// Layout of one plan buffer (T = float or double), n rows, S segments (S <= kMaxSeg):
+//   [0n) cf  : 1/pivot of the segment-local LU
+//   [1n) cl  : l_j / pivot_j        (forward:  y_j = cf_j d_j - cl_j y_{j-1})
+//   [2n) cu  : u_j / pivot_j        (backward: x_j = y_j - cu_j x_{j+1})
+//   [3n) vL  : response of row j to the left interface value
+//   [4n) vR  : response of row j to the right interface value
+//   [5n + 0*kMaxSeg) rl  : interface row lower coefficient l_{I_k}
+//   [5n + 1*kMaxSeg) ru  : interface row upper coefficient u_{I_k}
+//   [5n + 2*kMaxSeg) rf  : reduced-system 1/pivot
+//   [5n + 3*kMaxSeg) rcl : reduced-system lower/pivot
+//   [5n + 4*kMaxSeg) rcu : reduced-system upper/pivot
+constexpr int kMaxSeg = 16;
+inline int plan_len(int n) { return 5 * n + 5 * kMaxSeg; }
+
+// Segment split used by both the plan and the kernels: S segments, the first n%S get one
+// extra row.
+__host__ __device__ inline int seg_count(int n, int M) { return (n + M - 1) / M; }
+__host__ __device__ inline int seg_start(int n, int S, int k) {
+    int base = n / S, rem = n % S;
+    return k * base + (k < rem ? k : rem);
+}
+
+// Rows-per-segment template size for a pencil of n rows (S = ceil(n/M) <= kMaxSeg).
+inline int choose_M(int n) {
+    if (n <= 16 * 8 && n <= 64) return 8;
+    if (n <= 16 * 16) return 16;
+    if (n <= 16 * 32) return 32;
+    if (n <= 16 * 64) return 64;
+    return -1;
+}
+
+// Tridiagonal coefficients l_j, u_j (diagonal 1) of the compact system.
+//   order 1: alpha = 1/3, wall rows derivatives.f90:108-125 (row 0 upper = 4 for codes 1,3
+//            else 0; row 1 = (1/8, 3/4); mirrored at the upper wall, row n-1 lower = 4 for
+//            codes 1,2 else 0).
+//   order 2: alpha = 2/11, wall rows derivatives.f90:196-216, :256-261 (row 0 upper =
+//            137/13 for Dirichlet-type ends else 707/193; row 1 = (0.1, -0.35)).
+void lhs_coefficients(int order, int n, int lo, int hi, int bdry, std::vector<double> &l,
+                      std::vector<double> &u);
+
+// Fills a plan buffer (FP64) for the given key and segment count S.
+void build_plan(const PlanKey &key, int S, std::vector<double> &plan);
+
+}  // namespace cssx

@@ -1,0 +1,95 @@
+/*
+ * cssx: MI355X-native compact sixth-order derivatives, MHD right-hand side and RK4 stages of
+ * the CSS code (orvedahl/CSS-Code), exposed as a plain C ABI.
+ *
+ * Data layout everywhere is the reference's: a field f(nth, nphi, nr) in Fortran order, i.e.
+ * theta fastest, then phi, then r (parallel_mpi.f90:41, rhs_mpi.f90:24).  A state holds nv
+ * such fields back to back in the order rho, w=u_theta, v=u_phi, u=u_r, s, A_theta, A_phi,
+ * A_r (rhs_mpi.f90:24, the magnetic slots hold the vector potential between stages).
+ *
+ * All pointers passed to compute entry points are device pointers owned by the caller;
+ * `stream` is a hipStream_t (NULL = the null stream).  Calls are stream-ordered and do not
+ * synchronize unless stated.  Every entry point returns 0 on success or a CSSX_E* code and
+ * never terminates the process (the reference's Graceful_Exit, parallel_mpi.f90:886-889,
+ * aborts the whole MPI job instead).
+ */
+#ifndef CSSX_H
+#define CSSX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CSSX_OK 0
+#define CSSX_ENOMEM 12      /* device or host allocation failed */
+#define CSSX_EINVAL 22      /* bad shape, boundary code, axis or pointer */
+#define CSSX_ENAN 33        /* non-finite value detected by a checked call */
+#define CSSX_EHIP 100       /* a HIP runtime call failed */
+
+#define CSSX_PREC_F64 0
+#define CSSX_PREC_F32 1
+
+#define CSSX_AXIS_R 0
+#define CSSX_AXIS_THETA 1
+#define CSSX_AXIS_PHI 2
+
+typedef struct cssx_ctx cssx_ctx;
+
+/* One rank's subdomain (the single-GPU build treats it as the whole domain).
+ * The *_wall flags say whether that end is a physical boundary (derivatives.f90:529-544:
+ * Determine_Dtype); phi is periodic. */
+typedef struct cssx_grid {
+    int nr, nth, nphi;
+    int r_lo_wall, r_hi_wall;
+    int th_lo_wall, th_hi_wall;
+    double dri, dti, dpi;   /* 1/dx in index space: derivatives.f90:91-97 */
+} cssx_grid;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int cssx_version(void);
+
+/* Human-readable text for an error code. */
+const char *cssx_strerror(int code);
+
+/* Replaces Initialize_Derivatives (derivatives.f90:39-520): validates the grid and prepares
+ * the Thomas/segment plans for every boundary code on the current HIP device. */
+int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out);
+void cssx_ctx_destroy(cssx_ctx *ctx);
+
+/* One directional derivative over the whole subdomain.
+ * Replaces Dbydr / Dbydt / Dbydp (order 1; derivatives.f90:547, :1010, :1480) and
+ * D2bydr2 / D2bydt2 / D2bydp2 (order 2; :799, :1265, :1594).
+ *   axis      CSSX_AXIS_R / _THETA / _PHI
+ *   bdry      boundary code 0..4 (derivatives.f90:11-21); ignored on phi
+ *   prec      CSSX_PREC_F64 (in/out double) or CSSX_PREC_F32 (float)
+ *   ghost_lo/ghost_hi  halo buffers in the reference's layout for that axis:
+ *             r     (nth, nphi, 4)   binfo(1,side)%bdata
+ *             theta (4, nphi, nr)    binfo(2,side)%bdata
+ *             phi   (nth, 4, nr)     binfo(3,side)%bdata
+ *             At a physical wall slot 0 holds the prescribed physical first derivative
+ *             (NULL = 0).  On phi, NULL ghosts select the single-subdomain periodic wrap.
+ * The result is f' (or f'') in index units times dri/dti/dpi (squared for order 2), exactly
+ * what the reference returns; the caller multiplies by dx/dr on a stretched radial grid. */
+int cssx_deriv(cssx_ctx *ctx, int axis, int order, int bdry, int prec, const void *in, void *out,
+               const void *ghost_lo, const void *ghost_hi, void *stream);
+
+/* CSS-Python compact_fd6 (CSS-Python/derivatives.py:32-185) on a batch of pencils.
+ * Pencil p's point j is arr[j*npencils + p] (pencil index fastest).  b1/b2 hold 4 ghosts
+ * per pencil at b[k*npencils + p] (used for dtype 1/3 and 2/3; may be NULL otherwise).
+ * darr (may be NULL): on entry rows 0 / n-1 carry the Neumann values f'/dri as in the Python;
+ * on exit it holds the assembled right-hand side, as the Python leaves it.
+ * result = the derivative (solution * dri), double precision. */
+int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
+                     const double *b1, const double *b2, double *darr, double *result, void *stream);
+
+/* The precomputed solve tables for one pencil system, in FP64 on the host (no device work).
+ * The counterpart of the module-level Thomas tables gam1/gam2/gam3 that Initialize_Derivatives
+ * builds (derivatives.f90:106-518); here they are the segment-local LU factors, interface
+ * response vectors and reduced-system factors of the segmented solve (cssx_plan.hpp layout).
+ * lo_kind/hi_kind: 0 wall, 1 halo.  plan_out must hold 5*n + 80 doubles; *nseg receives S. */
+int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, double *plan_out, int *nseg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CSSX_H */
