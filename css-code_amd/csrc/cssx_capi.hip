@@ -98,7 +98,7 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
         J.pre = (const T *)s.pre; J.post = (const T *)s.post;
         J.scale = (T)s.scale; J.neu_mul = (T)s.neu_mul; J.halo_s = (T)s.halo_s;
         J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
-        J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind;
+        J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind; J.lo_r0_val = s.lo_r0_val;
         if (L.njobs == kMaxJobs || i == nspec - 1) {
             hipError_t e = launch_deriv<T>(L, s.order, stream);
             if (e != hipSuccess) return CSSX_EHIP;
@@ -221,8 +221,9 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     const bool lo_halo = (dtype == 1 || dtype == 3), hi_halo = (dtype == 2 || dtype == 3);
     if ((lo_halo && !b1) || (hi_halo && !b2)) return CSSX_EINVAL;
     const double dli = (ibc == 5) ? (double)n : (double)(n - 1);
-    // ibc 5 behaves as 3 at the walls (derivatives.py:82-115, :130-160: odd -> Dirichlet lower,
-    // not 1/2 -> Neumann upper) and only changes dli.
+    // ibc 5: dli = n; at an upper wall it is Neumann with the preloaded value (not 1/2,
+    // derivatives.py:82-115); at a lower wall it keeps the Dirichlet-type tridiagonal row
+    // (5 is odd, :154-160) but its R0 is the preloaded darr[0] (no formula fills it, :134-135).
     const int bd = (ibc == 5) ? 3 : ibc;
     AxisGeom g;
     g.n = n; g.na = (int)npencils; g.nb = 1; g.sa = 1; g.sb = 0; g.srow = npencils;
@@ -236,6 +237,7 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     s.glo = lo_halo ? (const void *)b1 : (const void *)darr;
     s.ghi = hi_halo ? (const void *)b2 : (const void *)(darr ? darr + (long)(n - 1) * npencils : nullptr);
     s.bdry = bd;
+    s.lo_r0_val = (ibc == 5);
     s.scale = dli * dri;
     s.neu_mul = 1.0 / dli;
     s.halo_s = dli / 60.0;   // derivatives.py:91,141,177,181 multiply the whole halo row by dli/60

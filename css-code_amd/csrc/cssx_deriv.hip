@@ -52,9 +52,10 @@ struct Rows<T, 1> {
     }
     // wall row 0: Neumann value or the 5th-order closure preconditioned to tridiagonal form
     // (derivatives.f90:623-662): 2*R0 - 8*R1.
-    static __device__ __forceinline__ T wall0(int dirichlet, T val, T x0, T x1, T x2, T x3, T r1) {
+    static __device__ __forceinline__ T wall0(int dirichlet, T val, T x0, T x1, T x2, T x3, T r1,
+                                              bool r0_from_val = false) {
         if (dirichlet) {
-            T r0 = T(-10.0 / 3.0) * x0 + T(-3.0) * x1 + T(6.0) * x2 + T(1.0 / 3.0) * x3;
+            T r0 = r0_from_val ? val : T(-10.0 / 3.0) * x0 + T(-3.0) * x1 + T(6.0) * x2 + T(1.0 / 3.0) * x3;
             return T(2.0) * r0 + T(-8.0) * r1;
         }
         return val;
@@ -148,7 +149,8 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
             if constexpr (ORDER == 1) {
                 sp_lo1 = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
                 T val = (bd == 0) ? T(0) : lo_val * J.neu_mul;
-                sp_lo0 = Rows<T, 1>::wall0(bd == 1 || bd == 3, val, w[4], w[5], w[6], w[7], sp_lo1);
+                sp_lo0 = Rows<T, 1>::wall0(bd == 1 || bd == 3, val, w[4], w[5], w[6], w[7], sp_lo1,
+                                           J.lo_r0_val != 0);
             } else {
                 T val = (bd == 0) ? T(0) : -lo_val * J.neu_mul;
                 sp_lo0 = Rows<T, 2>::wall0(bd == 1 || bd == 3, val, w[4], w[5], w[6], w[7], w[8], w[9]);

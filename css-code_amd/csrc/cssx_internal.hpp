@@ -25,6 +25,7 @@ struct DerivSpec {
     double scale = 1.0, neu_mul = 1.0, halo_s = 1.0, rhs_scale = 1.0, beta = 0.0;
     int bdry = 0;
     int lo_kind = END_WALL, hi_kind = END_WALL;
+    int lo_r0_val = 0;
 };
 
 void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g);

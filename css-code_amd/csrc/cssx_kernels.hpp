@@ -26,6 +26,8 @@ struct DerivJob {
     T beta;             // out = f' + beta*out
     int bdry;           // boundary code 0..4
     int lo_kind, hi_kind;
+    int lo_r0_val;      // lower Dirichlet-type wall row takes R0 from the wall value (CSS-Python
+                        // compact_fd6 with ibc=5, derivatives.py:134-155)
 };
 
 template <typename T>

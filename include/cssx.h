@@ -82,6 +82,76 @@ int cssx_deriv(cssx_ctx *ctx, int axis, int order, int bdry, int prec, const voi
 int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
                      const double *b1, const double *b2, double *darr, double *result, void *stream);
 
+/* ---------------------------------------------------------------------------------------------
+ * Compressible MHD right-hand side and RK4 stepping (rhs_mpi.f90, step_rk4.f90, diffusion.f90)
+ * ------------------------------------------------------------------------------------------- */
+
+/* Physics constants, switches and boundary codes.  Field names follow the reference's
+ * namelist / module variables (constants.f90:17-72, rhs_mpi.f90:10-13, diffusion.f90:8-11).
+ * Only the stratified (not Unstratified), Laplacian=.False., do_radheat=.False. physics is
+ * supported; the flags below are the ones that path reads. */
+typedef struct cssx_mhd_params {
+    double gamma, Cp;            /* Cv=Cp/gamma, pconst=(gamma-1)*Cp (rhs_mpi.f90:59-73) */
+    double omega0;               /* frame rotation rate omega_0 */
+    double xi_cs;                /* sound-speed reduction: drho/dt *= xi_cs^2 when < 1 */
+    double damping_time, T0;     /* top relaxation toward T0 (rhs_mpi.f90:1807-1818) */
+    int magnetic;                /* 8 fields with the vector potential in slots 6-8; do_lorentz */
+    int top_relax;               /* apply the top relaxation (not Set_Stop/top_inout/...) */
+    int bottom_stress_free, top_stress_free;
+    int bottom_const_flux, top_const_flux, bottom_const_temp, top_const_temp;
+    int bottom_radial_field, top_radial_field;
+    int theta_normal_field, theta_perf_cond, theta_stress_free;
+    int ibcr[13], ibct[13];      /* boundary codes by variable index 1..12 (ibcr/ibct of
+                                    derivatives.f90:50-57: rho,w,v,u,s,T,bt,bp,br,at,ap,ar) */
+    /* SLD diffusion (diffusion.f90:8-11; Compute_Diffusion :70-474) */
+    int sld_on;                  /* SLDiffusion: Euler diffusion folded into stage 1 */
+    double sld_coef;             /* 0 gives an exactly zero diffusion tendency */
+    int mass_diffusion, magnetic_diffusion, viscous_heating, ohmic_heating;
+} cssx_mhd_params;
+
+/* Radial / latitudinal profiles (host arrays, copied at creation).  Lengths: nr or nth. */
+typedef struct cssx_mhd_profiles {
+    const double *r;             /* r_ind (nr) */
+    const double *dr;            /* dr_ind (nr): cell widths used by diffusion and CFL */
+    const double *dxdr;          /* dx/dr (nr); all ones on a uniform grid */
+    const double *theta;         /* nth colatitudes */
+    const double *gravity;       /* g(r) (nr) */
+    const double *eps;           /* heating eps(r) (nr) */
+    const double *rho_bar;       /* background density rhor (nr) */
+    const double *s_bar;         /* background entropy sr (nr) */
+    const double *dsdr1, *dsdr2; /* wall entropy gradients (nth) for constant-flux walls */
+    const double *sr1, *sr2;     /* wall entropies (nth) for constant-temperature walls */
+    const double *diffusion_speed;/* SLD speed^2 per radius, nr+2 entries (0..nr+1) */
+} cssx_mhd_profiles;
+
+typedef struct cssx_mhd cssx_mhd;
+
+/* Builds the device-side constants for one subdomain (Initialize_Physics, rhs_mpi.f90:18-73;
+ * Initialize_Step_RK4, step_rk4.f90:19-22).  Allocates its own scratch for both precisions. */
+int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *params, const cssx_mhd_profiles *prof,
+                    cssx_mhd **out);
+void cssx_mhd_destroy(cssx_mhd *m);
+
+/* RHS(call_num) (rhs_mpi.f90:279-783) on a state vars1 (nv fields, A in slots 6-8).
+ *   rhs   dvars1 (nv fields): d(rho,w,v,u,s)/dt and dA/dt
+ *   b_out optional (3 fields): B = curl A with the wall conditions of Compute_B
+ *         (derivatives.f90:1830-1892), what the reference leaves in vars1(6:8) for call 1.
+ * prec selects the arithmetic (CSSX_PREC_F64: state/rhs double; CSSX_PREC_F32: float). */
+int cssx_rhs(cssx_mhd *m, int prec, const void *state, void *rhs, void *b_out, void *stream);
+
+/* Bound (rhs_mpi.f90:2395-2491) applied in place to a state. */
+int cssx_bound(cssx_mhd *m, int prec, void *state, void *stream);
+
+/* nsteps classical RK4 steps of Step_RK4 (step_rk4.f90:339-493, SLDiffusion branch) on the
+ * FP64 state y0 (nv fields, updated in place).  prec = CSSX_PREC_F64 runs everything in FP64;
+ * CSSX_PREC_F32 evaluates stages, RHS and diffusion in FP32 while the RK4 accumulation and y0
+ * stay FP64. */
+int cssx_rk4(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream);
+
+/* Compute_Step_Size (rhs_mpi.f90:92-275): dt = cfl_safety * min over points of the sound,
+ * flow and Alfven crossing times.  Synchronizes the stream; *dt_out on the host. */
+int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream);
+
 /* The precomputed solve tables for one pencil system, in FP64 on the host (no device work).
  * The counterpart of the module-level Thomas tables gam1/gam2/gam3 that Initialize_Derivatives
  * builds (derivatives.f90:106-518); here they are the segment-local LU factors, interface

@@ -1,0 +1,603 @@
+/*
+ * CPU oracle: FP64 restatement of the CSS MHD right-hand side, Bound, Compute_B and the
+ * classical RK4 step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle_deriv.c).  The parameter structs come from the public
+ * header include/cssx.h (data layout only).
+ *
+ * Restated (reference file:line):
+ *   RHS                    rhs_mpi.f90:279-783  (stratified, Laplacian=F, do_radheat=F)
+ *   Compute_B/VectorCurlA  derivatives.f90:1830-1892, :2106-2294
+ *   Source_Theta_Terms     rhs_mpi.f90:786-1029
+ *   Phi_Terms              rhs_mpi.f90:1032-1261
+ *   Radial_Terms           rhs_mpi.f90:1264-1519
+ *   Density_Terms_Theta    rhs_mpi.f90:1522-1605
+ *   Density_Terms_Radial   rhs_mpi.f90:1710-1855
+ *   Induction_Terms        rhs_mpi.f90:2298-2366 (u x B; the div B diagnostic is separate)
+ *   wall ghost values      rhs_mpi.f90:410-455, :578-637
+ *   Bound                  rhs_mpi.f90:2395-2491
+ *   Step_RK4               step_rk4.f90:339-493 (SLDiffusion branch, T3)
+ * The subdomain has physical walls at both r ends and both theta ends; each wall is handled as
+ * the rank touching it would (the Fortran's If (myr1==1) ... ElseIf (myr2==nr) chains need
+ * two ranks per direction to visit both walls; here both are applied).
+ *
+ * Layout: field f[(ir*nphi + ip)*nth + it]; state = nv such fields, variable iv at offset
+ * (iv-1)*N.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/cssx.h"
+
+int orc_deriv_field(int axis, int order, int nth, int nphi, int nr, const double *f, double *out,
+                    int lo_wall, int hi_wall, int bdry, const double *ghost_lo, const double *ghost_hi,
+                    double scale, double hneu, const double *metric);
+
+#define PI_ 3.1415926535897932384626433832795028841972
+#define ONE_OVER_4PI (1.0 / PI_ / 4.0)
+
+typedef struct {
+    int nr, nth, nphi;
+    long N;
+    double dri, dti, dpi;
+    cssx_mhd_params p;
+    double gamm1, Cv, pconst, invpconst, invtconst, invCv;
+    const double *r, *dr, *dxdr, *theta, *gravity, *eps, *dsdr1, *dsdr2, *sr1, *sr2;
+    double *sines, *cosines;
+} orc_mhd;
+
+static void setup(orc_mhd *m, const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr)
+{
+    int it;
+    m->nr = g->nr; m->nth = g->nth; m->nphi = g->nphi;
+    m->N = (long)g->nr * g->nphi * g->nth;
+    m->dri = g->dri; m->dti = g->dti; m->dpi = g->dpi;
+    m->p = *p;
+    /* rhs_mpi.f90:59-73, stratified branch */
+    m->Cv = p->Cp / p->gamma;
+    m->invCv = 1.0 / m->Cv;
+    m->gamm1 = p->gamma - 1.0;
+    m->pconst = m->gamm1 * p->Cp;
+    m->invpconst = (1.0 / p->Cp) / m->gamm1;
+    m->invtconst = m->invCv / m->gamm1;
+    m->r = pr->r; m->dr = pr->dr; m->dxdr = pr->dxdr; m->theta = pr->theta;
+    m->gravity = pr->gravity; m->eps = pr->eps;
+    m->dsdr1 = pr->dsdr1; m->dsdr2 = pr->dsdr2; m->sr1 = pr->sr1; m->sr2 = pr->sr2;
+    m->sines = (double *)malloc(sizeof(double) * 2 * g->nth);
+    m->cosines = m->sines + g->nth;
+    for (it = 0; it < g->nth; it++) { m->sines[it] = sin(pr->theta[it]); m->cosines[it] = cos(pr->theta[it]); }
+}
+
+#define IDX(m, it, ip, ir) ((((long)(ir) * (m)->nphi + (ip)) * (m)->nth) + (it))
+
+/* derivative along theta with wall values per (ip, ir) pencil (NULL = 0) */
+static void dth(const orc_mhd *m, const double *x, double *out, int code, const double *wlo, const double *whi,
+                double *glo, double *ghi)
+{
+    long k, np = (long)m->nphi * m->nr;
+    for (k = 0; k < np; k++) {
+        glo[k * 4] = wlo ? wlo[k] : 0.0;
+        ghi[k * 4] = whi ? whi[k] : 0.0;
+    }
+    orc_deriv_field(1, 1, m->nth, m->nphi, m->nr, x, out, 1, 1, code, glo, ghi, m->dti, m->dti, NULL);
+}
+
+static void dph(const orc_mhd *m, const double *x, double *out)
+{
+    orc_deriv_field(2, 1, m->nth, m->nphi, m->nr, x, out, 0, 0, 0, NULL, NULL, m->dpi, m->dpi, NULL);
+}
+
+/* derivative along r (x-space) times dx/dr; wall values per (it, ip) in x-space units */
+static void dr_(const orc_mhd *m, const double *x, double *out, int code, const double *wlo, const double *whi,
+                double *glo, double *ghi)
+{
+    long k, np = (long)m->nphi * m->nth;
+    for (k = 0; k < np; k++) {
+        glo[k] = wlo ? wlo[k] : 0.0;
+        ghi[k] = whi ? whi[k] : 0.0;
+    }
+    orc_deriv_field(0, 1, m->nth, m->nphi, m->nr, x, out, 1, 1, code, glo, ghi, m->dri, m->dri, m->dxdr);
+}
+
+/* Compute_B: B = curl A with wall zeroing; a, b point to 3 fields (theta, phi, r). */
+static void compute_b(const orc_mhd *m, const double *a, double *b, double *work, double *work2, double *glo,
+                      double *ghi)
+{
+    const int nr = m->nr, nth = m->nth, nphi = m->nphi;
+    const long N = m->N;
+    const double *at = a, *ap = a + N, *ar = a + 2 * N;
+    double *bt = b, *bp = b + N, *br = b + 2 * N;
+    int it, ip, ir;
+    long i;
+    /* theta sweeps (derivatives.f90:2163-2178): B_r <- d(sin A_phi)/dth, B_phi <- -dA_r/dth */
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) work[IDX(m, it, ip, ir)] = m->sines[it] * ap[IDX(m, it, ip, ir)];
+    dth(m, work, work2, m->p.ibct[11], NULL, NULL, glo, ghi);
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                i = IDX(m, it, ip, ir);
+                br[i] = work2[i] * (1.0 / m->sines[it]) * (1.0 / m->r[ir]);
+            }
+    dth(m, ar, work2, m->p.ibct[12], NULL, NULL, glo, ghi);
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) bp[i] = -work2[i] * (1.0 / m->r[ir]);
+    /* phi sweeps (:2186-2207) */
+    dph(m, ar, work2);
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                i = IDX(m, it, ip, ir);
+                bt[i] = (1.0 / m->r[ir]) * work2[i] * (1.0 / m->sines[it]);
+            }
+    dph(m, at, work2);
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                i = IDX(m, it, ip, ir);
+                br[i] = br[i] - (1.0 / m->r[ir]) * work2[i] * (1.0 / m->sines[it]);
+            }
+    /* r sweeps of r*A_theta, r*A_phi (:2250-2293); wall A halos are zero, codes at/ap */
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) work[i] = m->r[ir] * at[i];
+    dr_(m, work, work2, m->p.ibcr[10], NULL, NULL, glo, ghi);
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) bp[i] = bp[i] + work2[i] * (1.0 / m->r[ir]);
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) work[i] = m->r[ir] * ap[i];
+    dr_(m, work, work2, m->p.ibcr[11], NULL, NULL, glo, ghi);
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) bt[i] = bt[i] - work2[i] * (1.0 / m->r[ir]);
+    /* wall conditions (derivatives.f90:1840-1883) */
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++) {
+            if (m->p.theta_perf_cond) {
+                bt[IDX(m, 0, ip, ir)] = 0.0;
+                bt[IDX(m, nth - 1, ip, ir)] = 0.0;
+            } else {
+                bp[IDX(m, 0, ip, ir)] = 0.0; br[IDX(m, 0, ip, ir)] = 0.0;
+                bp[IDX(m, nth - 1, ip, ir)] = 0.0; br[IDX(m, nth - 1, ip, ir)] = 0.0;
+            }
+        }
+    for (ip = 0; ip < nphi; ip++)
+        for (it = 0; it < nth; it++) {
+            if (m->p.bottom_radial_field) { bt[IDX(m, it, ip, 0)] = 0.0; bp[IDX(m, it, ip, 0)] = 0.0; }
+            if (m->p.top_radial_field) { bt[IDX(m, it, ip, nr - 1)] = 0.0; bp[IDX(m, it, ip, nr - 1)] = 0.0; }
+        }
+}
+
+/* RHS on state v (nv fields with A in 6-8) -> dv (nv fields); bout (3 fields) optional. */
+int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, const double *v,
+            double *dv, double *bout)
+{
+    orc_mhd M, *m = &M;
+    long N, i, np;
+    int nr, nth, nphi, it, ip, ir, nv;
+    double *B, *T, *ir_, *work, *work1, *work2, *wlo, *whi, *glo, *ghi;
+    const double *rho, *w, *vv, *u, *s, *bt, *bp, *br;
+    double *drho, *dw, *dvv, *du, *ds;
+    setup(m, g, p, pr);
+    N = m->N; nr = m->nr; nth = m->nth; nphi = m->nphi;
+    nv = p->magnetic ? 8 : 5;
+    np = (long)nphi * (nr > nth ? nr : nth) * 4 + 16;
+    B = (double *)calloc(3 * N, sizeof(double));
+    T = (double *)malloc(sizeof(double) * N);
+    ir_ = (double *)malloc(sizeof(double) * N);
+    work = (double *)malloc(sizeof(double) * N);
+    work1 = (double *)malloc(sizeof(double) * N);
+    work2 = (double *)malloc(sizeof(double) * N);
+    wlo = (double *)calloc((size_t)nphi * (nr > nth ? nr : nth) + 16, sizeof(double));
+    whi = (double *)calloc((size_t)nphi * (nr > nth ? nr : nth) + 16, sizeof(double));
+    glo = (double *)calloc(np * 4 > 4L * nth * nphi ? np * 4 : 4L * nth * nphi, sizeof(double));
+    ghi = (double *)calloc(np * 4 > 4L * nth * nphi ? np * 4 : 4L * nth * nphi, sizeof(double));
+    if (!B || !T || !ir_ || !work || !work1 || !work2 || !wlo || !whi || !glo || !ghi) return 12;
+    rho = v; w = v + N; vv = v + 2 * N; u = v + 3 * N; s = v + 4 * N;
+    drho = dv; dw = dv + N; dvv = dv + 2 * N; du = dv + 3 * N; ds = dv + 4 * N;
+    memset(dv, 0, sizeof(double) * nv * N);
+
+    /* Compute_B(1): B replaces A inside the RHS */
+    if (p->magnetic) {
+        compute_b(m, v + 5 * N, B, work, work2, glo, ghi);
+        if (bout) memcpy(bout, B, sizeof(double) * 3 * N);
+    }
+    bt = B; bp = B + N; br = B + 2 * N;
+
+    /* temperature, 1/rho (rhs_mpi.f90:377-387) */
+    for (i = 0; i < N; i++) {
+        double tmp = pow(rho[i], m->gamm1);
+        T[i] = m->invtconst * tmp;
+        tmp = s[i] * m->invCv;
+        tmp = exp(tmp);
+        T[i] = T[i] * tmp;
+        ir_[i] = 1.0 / rho[i];
+    }
+
+    /* ---- Source_Theta_Terms (rhs_mpi.f90:786-1029) ---- */
+    {
+        const double om2 = 2.0 * p->omega0;
+        double *tw = work1;       /* theta derivative of the current field */
+        long kk;
+        /* theta wall values of v: cot(theta)*v (rhs_mpi.f90:414, :441) */
+        for (ir = 0; ir < nr; ir++)
+            for (ip = 0; ip < nphi; ip++) {
+                kk = (long)ir * nphi + ip;
+                wlo[kk] = m->cosines[0] / m->sines[0] * vv[IDX(m, 0, ip, ir)];
+                whi[kk] = m->cosines[nth - 1] / m->sines[nth - 1] * vv[IDX(m, nth - 1, ip, ir)];
+            }
+        /* the six theta derivatives first (each is a whole-field call here) */
+        {
+            double *d_w = (double *)malloc(sizeof(double) * N), *d_v = (double *)malloc(sizeof(double) * N);
+            double *d_u = (double *)malloc(sizeof(double) * N), *d_s = (double *)malloc(sizeof(double) * N);
+            double *d_bp = NULL, *d_br = NULL;
+            dth(m, w, d_w, p->ibct[2], NULL, NULL, glo, ghi);
+            dth(m, vv, d_v, p->ibct[3], wlo, whi, glo, ghi);
+            dth(m, u, d_u, p->ibct[4], NULL, NULL, glo, ghi);
+            dth(m, s, d_s, p->ibct[5], NULL, NULL, glo, ghi);
+            if (p->magnetic) {
+                /* Theta_Normal_Field wall values: bp -> -cot*bp, br -> 0 (rhs_mpi.f90:418-422) */
+                for (ir = 0; ir < nr; ir++)
+                    for (ip = 0; ip < nphi; ip++) {
+                        kk = (long)ir * nphi + ip;
+                        wlo[kk] = p->theta_normal_field ? -m->cosines[0] / m->sines[0] * bp[IDX(m, 0, ip, ir)] : 0.0;
+                        whi[kk] = p->theta_normal_field ? -m->cosines[nth - 1] / m->sines[nth - 1] * bp[IDX(m, nth - 1, ip, ir)] : 0.0;
+                    }
+                d_bp = (double *)malloc(sizeof(double) * N);
+                d_br = (double *)malloc(sizeof(double) * N);
+                dth(m, bp, d_bp, p->ibct[8], wlo, whi, glo, ghi);
+                dth(m, br, d_br, p->ibct[9], NULL, NULL, glo, ghi);
+            }
+            for (ir = 0; ir < nr; ir++) {
+                const double ri = 1.0 / m->r[ir], rv = m->r[ir], epsr = m->eps ? m->eps[ir] : 0.0;
+                const double omsq = rv * p->omega0 * p->omega0;
+                for (ip = 0; ip < nphi; ip++)
+                    for (it = 0; it < nth; it++) {
+                        const long q = IDX(m, it, ip, ir);
+                        const double sn = m->sines[it], cs = m->cosines[it], cot = cs / sn;
+                        double Rho = rho[q], W = w[q], V = vv[q], U = u[q], t = T[q];
+                        double xdw, xdv, xdu, xdrho, xds, wk;
+                        xdw = omsq * sn * cs;
+                        xdu = -m->gravity[ir] + omsq * sn * sn;
+                        xdrho = -Rho * W * cot * ri;
+                        xdv = -om2 * cs * W;
+                        xdu = xdu + ri * W * W;
+                        wk = d_w[q] * ri;
+                        xdrho = xdrho - Rho * wk;
+                        xdw = xdw - W * wk;
+                        xdw = xdw + V * (ri * cot * V + om2 * cs);
+                        xdv = xdv - ri * V * W * cot;
+                        xdu = xdu + V * (ri * V + om2 * sn);
+                        wk = d_v[q] * ri;
+                        xdv = xdv - W * wk;
+                        drho[q] = xdrho - 2.0 * ri * Rho * U;
+                        xdv = xdv - U * (ri * V + om2 * sn);
+                        xdw = xdw - ri * U * W;
+                        if (!p->magnetic) dvv[q] = xdv;
+                        wk = d_u[q] * ri;
+                        xdu = xdu - W * wk;
+                        if (!p->magnetic) du[q] = xdu;
+                        xds = (fabs(epsr) > 0.0) ? epsr * ir_[q] / t : 0.0;
+                        wk = d_s[q] * ri;
+                        if (p->magnetic) xdw = xdw - m->gamm1 * t * wk;
+                        else dw[q] = xdw - m->gamm1 * t * wk;
+                        xds = xds - W * wk;
+                        ds[q] = xds;
+                        if (p->magnetic) {
+                            double oorho = ir_[q] * ONE_OVER_4PI * ri;
+                            double Bt = bt[q], Bp = bp[q], Br = br[q];
+                            xdu = xdu - Bt * Bt * oorho;
+                            xdw = xdw - Bp * Bp * cot * oorho;
+                            xdv = xdv + Bp * Bt * cot * oorho;
+                            xdu = xdu - Bp * Bp * oorho;
+                            wk = d_bp[q] * oorho;
+                            xdv = xdv + Bt * wk;
+                            xdw = xdw - Bp * wk;
+                            wk = d_br[q] * oorho;
+                            dw[q] = xdw + Br * (Bt * oorho - wk);
+                            dvv[q] = xdv + Br * Bp * oorho;
+                            du[q] = xdu + Bt * wk;
+                        }
+                    }
+            }
+            free(d_w); free(d_v); free(d_u); free(d_s); free(d_bp); free(d_br);
+        }
+        (void)tw;
+    }
+
+    /* ---- Phi_Terms (rhs_mpi.f90:1032-1261) ---- */
+    {
+        double *d_rho = (double *)malloc(sizeof(double) * N), *d_v = (double *)malloc(sizeof(double) * N);
+        double *d_w = (double *)malloc(sizeof(double) * N), *d_u = (double *)malloc(sizeof(double) * N);
+        double *d_s = (double *)malloc(sizeof(double) * N), *d_bt = NULL, *d_br = NULL;
+        dph(m, rho, d_rho); dph(m, vv, d_v); dph(m, w, d_w); dph(m, u, d_u); dph(m, s, d_s);
+        if (p->magnetic) {
+            d_bt = (double *)malloc(sizeof(double) * N);
+            d_br = (double *)malloc(sizeof(double) * N);
+            dph(m, bt, d_bt); dph(m, br, d_br);
+        }
+        for (ir = 0; ir < nr; ir++) {
+            const double ri = 1.0 / m->r[ir];
+            for (ip = 0; ip < nphi; ip++)
+                for (it = 0; it < nth; it++) {
+                    const long q = IDX(m, it, ip, ir);
+                    const double oors = ri * (1.0 / m->sines[it]);
+                    double Rho = rho[q], V = vv[q], t = T[q], oorho = ir_[q];
+                    double xdrho, xdv, xdw, xdu, xds, wk;
+                    wk = d_rho[q] * oors;
+                    xdrho = drho[q];
+                    xdv = dvv[q];
+                    xdrho = xdrho - V * wk;
+                    xdv = xdv - m->pconst * wk * t * oorho;
+                    xds = ds[q];
+                    wk = d_v[q] * oors;
+                    xdv = xdv - V * wk;
+                    drho[q] = xdrho - Rho * wk;
+                    xdw = dw[q];
+                    wk = d_w[q] * oors;
+                    xdw = xdw - V * wk;
+                    xdu = du[q];
+                    if (!p->magnetic) dw[q] = xdw;
+                    wk = d_u[q] * oors;
+                    xdu = xdu - V * wk;
+                    if (!p->magnetic) du[q] = xdu;
+                    wk = d_s[q] * oors;
+                    xds = xds - V * wk;
+                    xdv = xdv - m->gamm1 * t * wk;
+                    if (!p->magnetic) dvv[q] = xdv;
+                    ds[q] = xds;
+                    if (p->magnetic) {
+                        double Bt = bt[q], Bp = bp[q], Br = br[q];
+                        oorho = ONE_OVER_4PI * oorho * oors;
+                        wk = d_bt[q] * oorho;
+                        xdv = xdv - Bt * wk;
+                        dw[q] = xdw + Bp * wk;
+                        wk = d_br[q] * oorho;
+                        dvv[q] = xdv - Br * wk;
+                        du[q] = xdu + Bp * wk;
+                    }
+                }
+        }
+        free(d_rho); free(d_v); free(d_w); free(d_u); free(d_s); free(d_bt); free(d_br);
+    }
+
+    /* ---- radial wall values (rhs_mpi.f90:578-637) and Radial_Terms (:1264-1519) ---- */
+    {
+        double *d_w = (double *)malloc(sizeof(double) * N), *d_v = (double *)malloc(sizeof(double) * N);
+        double *d_u = (double *)malloc(sizeof(double) * N), *d_s = (double *)malloc(sizeof(double) * N);
+        double *d_bt = NULL, *d_bp = NULL;
+        const double r1 = m->r[0], r2 = m->r[nr - 1], x1 = m->dxdr[0], x2 = m->dxdr[nr - 1];
+        long kk;
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                kk = (long)ip * nth + it;
+                wlo[kk] = p->bottom_stress_free ? w[IDX(m, it, ip, 0)] / r1 / x1 : 0.0;
+                whi[kk] = p->top_stress_free ? w[IDX(m, it, ip, nr - 1)] / r2 / x2 : w[IDX(m, it, ip, nr - 1)];
+            }
+        dr_(m, w, d_w, p->ibcr[2], wlo, whi, glo, ghi);
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                kk = (long)ip * nth + it;
+                wlo[kk] = p->bottom_stress_free ? vv[IDX(m, it, ip, 0)] / r1 / x1 : 0.0;
+                whi[kk] = p->top_stress_free ? vv[IDX(m, it, ip, nr - 1)] / r2 / x2 : vv[IDX(m, it, ip, nr - 1)];
+            }
+        dr_(m, vv, d_v, p->ibcr[3], wlo, whi, glo, ghi);
+        dr_(m, u, d_u, p->ibcr[4], NULL, NULL, glo, ghi);
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                kk = (long)ip * nth + it;
+                wlo[kk] = p->bottom_const_flux ? m->dsdr1[it] / x1 : 0.0;
+                whi[kk] = p->top_const_flux ? m->dsdr2[it] / x2 : 0.0;
+            }
+        dr_(m, s, d_s, p->ibcr[5], wlo, whi, glo, ghi);
+        if (p->magnetic) {
+            d_bt = (double *)malloc(sizeof(double) * N);
+            d_bp = (double *)malloc(sizeof(double) * N);
+            dr_(m, bt, d_bt, p->ibcr[7], NULL, NULL, glo, ghi);
+            dr_(m, bp, d_bp, p->ibcr[8], NULL, NULL, glo, ghi);
+        }
+        for (ir = 0; ir < nr; ir++)
+            for (ip = 0; ip < nphi; ip++)
+                for (it = 0; it < nth; it++) {
+                    const long q = IDX(m, it, ip, ir);
+                    double U = u[q], t = T[q], xdw, xdv, xdu, xds, wk;
+                    wk = d_w[q];
+                    xdw = dw[q] - U * wk;
+                    if (!p->magnetic) dw[q] = xdw;
+                    wk = d_v[q];
+                    xdv = dvv[q] - U * wk;
+                    if (!p->magnetic) dvv[q] = xdv;
+                    xdu = du[q];
+                    wk = d_u[q];
+                    xdu = xdu - U * wk;
+                    drho[q] = drho[q] - rho[q] * wk;
+                    xds = ds[q];
+                    wk = d_s[q];
+                    xdu = xdu - m->gamm1 * t * wk;
+                    xds = xds - U * wk;
+                    if (!p->magnetic) du[q] = xdu;
+                    ds[q] = xds;
+                    if (p->magnetic) {
+                        double oorho = ir_[q] * ONE_OVER_4PI;
+                        wk = d_bt[q] * oorho;
+                        xdu = xdu - bt[q] * wk;
+                        dw[q] = xdw + br[q] * wk;
+                        wk = d_bp[q] * oorho;
+                        dvv[q] = xdv + br[q] * wk;
+                        du[q] = xdu - bp[q] * wk;
+                    }
+                }
+        free(d_w); free(d_v); free(d_u); free(d_s); free(d_bt); free(d_bp);
+    }
+
+    /* ---- Density_Terms_Theta (rhs_mpi.f90:1522-1605) ---- */
+    {
+        long kk;
+        for (ir = 0; ir < nr; ir++) {
+            const double rv = m->r[ir] * m->invpconst;
+            for (ip = 0; ip < nphi; ip++) {
+                long q0 = IDX(m, 0, ip, ir), q1 = IDX(m, nth - 1, ip, ir);
+                kk = (long)ir * nphi + ip;
+                wlo[kk] = rv * dw[q0] * rho[q0] / T[q0];
+                whi[kk] = rv * dw[q1] * rho[q1] / T[q1];
+            }
+        }
+        dth(m, rho, work1, p->ibct[1], wlo, whi, glo, ghi);
+        for (ir = 0; ir < nr; ir++) {
+            const double ri = 1.0 / m->r[ir];
+            for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) {
+                double wk = ri * work1[i];
+                drho[i] = drho[i] - w[i] * wk;
+                wk = wk * ir_[i];
+                dw[i] = dw[i] - m->pconst * wk * T[i];
+            }
+        }
+    }
+
+    /* ---- Density_Terms_Radial (rhs_mpi.f90:1710-1855) ---- */
+    {
+        long kk;
+        const int lb = p->ibcr[1];
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                long q0 = IDX(m, it, ip, 0), q1 = IDX(m, it, ip, nr - 1);
+                kk = (long)ip * nth + it;
+                wlo[kk] = (lb == 2 || lb == 4) ? du[q0] * rho[q0] * m->invpconst / T[q0] : rho[q0];
+                wlo[kk] = wlo[kk] / m->dxdr[0];
+                whi[kk] = (lb == 3 || lb == 4) ? du[q1] * rho[q1] * m->invpconst / T[q1] : rho[q1];
+                whi[kk] = whi[kk] / m->dxdr[nr - 1];
+            }
+        dr_(m, rho, work1, lb, wlo, whi, glo, ghi);
+        for (i = 0; i < N; i++) {
+            double wk = work1[i];
+            drho[i] = drho[i] - u[i] * wk;
+            wk = wk * ir_[i];
+            du[i] = du[i] - m->pconst * T[i] * wk;
+        }
+        if (p->top_relax) {
+            const double dt_ = p->damping_time * (p->gamma - 2.0);
+            for (ip = 0; ip < nphi; ip++)
+                for (it = 0; it < nth; it++) {
+                    long q = IDX(m, it, ip, nr - 1);
+                    double t = T[q];
+                    drho[q] = drho[q] + rho[q] * (p->T0 - t) / (dt_ * t);
+                    ds[q] = ds[q] - m->Cv * (p->T0 - t) / (dt_ * t);
+                }
+        }
+    }
+
+    /* ---- Induction_Terms: u x B (rhs_mpi.f90:2342-2346) ---- */
+    if (p->magnetic) {
+        double *dat = dv + 5 * N, *dap = dv + 6 * N, *dar = dv + 7 * N;
+        for (i = 0; i < N; i++) {
+            dat[i] = vv[i] * br[i] - u[i] * bp[i];
+            dap[i] = u[i] * bt[i] - w[i] * br[i];
+            dar[i] = w[i] * bp[i] - vv[i] * bt[i];
+        }
+    }
+    if (p->xi_cs < 1.0)
+        for (i = 0; i < N; i++) drho[i] = (p->xi_cs * p->xi_cs) * drho[i];
+
+    free(B); free(T); free(ir_); free(work); free(work1); free(work2);
+    free(wlo); free(whi); free(glo); free(ghi); free(m->sines);
+    return 0;
+}
+
+/* Bound (rhs_mpi.f90:2395-2491) on a state in place (A in slots 6-8). */
+int orc_bound(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, double *v)
+{
+    const int nr = g->nr, nth = g->nth, nphi = g->nphi;
+    const long N = (long)nr * nth * nphi;
+    double *w = v + N, *vv = v + 2 * N, *u = v + 3 * N, *s = v + 4 * N;
+    int it, ip, ir;
+#define Q(it, ip, ir) ((((long)(ir) * nphi + (ip)) * nth) + (it))
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++) {
+            w[Q(0, ip, ir)] = 0.0;
+            w[Q(nth - 1, ip, ir)] = 0.0;
+            if (!p->theta_stress_free) { u[Q(0, ip, ir)] = 0.0; u[Q(nth - 1, ip, ir)] = 0.0; }
+        }
+    for (ip = 0; ip < nphi; ip++)
+        for (it = 0; it < nth; it++) {
+            if (!p->bottom_stress_free) { w[Q(it, ip, 0)] = 0.0; vv[Q(it, ip, 0)] = 0.0; }
+            u[Q(it, ip, 0)] = 0.0;
+            if (p->bottom_const_temp) s[Q(it, ip, 0)] = pr->sr1[it];
+            if (!p->top_stress_free) { w[Q(it, ip, nr - 1)] = 0.0; vv[Q(it, ip, nr - 1)] = 0.0; }
+            u[Q(it, ip, nr - 1)] = 0.0;
+            /* Top_Constant_Temp fixes s only with Set_Stop, which this build does not model */
+        }
+    if (p->magnetic) {
+        double *at = v + 5 * N, *ap = v + 6 * N, *ar = v + 7 * N;
+        for (ir = 0; ir < nr; ir++)
+            for (ip = 0; ip < nphi; ip++) {
+                if (p->theta_perf_cond) {
+                    ap[Q(0, ip, ir)] = 0.0; ar[Q(0, ip, ir)] = 0.0;
+                    ap[Q(nth - 1, ip, ir)] = 0.0; ar[Q(nth - 1, ip, ir)] = 0.0;
+                } else {
+                    at[Q(0, ip, ir)] = 0.0;
+                    at[Q(nth - 1, ip, ir)] = 0.0;
+                }
+            }
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                if (p->bottom_radial_field) ar[Q(it, ip, 0)] = 0.0;
+                if (p->top_radial_field) ar[Q(it, ip, nr - 1)] = 0.0;
+            }
+    }
+#undef Q
+    return 0;
+}
+
+int orc_diffusion(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, const double *v,
+                  const double *b, double *dv);
+
+/* nsteps of Step_RK4 (step_rk4.f90:339-493), SLDiffusion branch: k1 with Euler diffusion. */
+int orc_rk4(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, double *y0, double dt,
+            int nsteps)
+{
+    const int nv = p->magnetic ? 8 : 5;
+    const long N = (long)g->nr * g->nth * g->nphi, NV = nv * N;
+    double *y1 = (double *)malloc(sizeof(double) * NV), *k = (double *)malloc(sizeof(double) * NV);
+    double *save = (double *)malloc(sizeof(double) * NV), *dd = (double *)malloc(sizeof(double) * NV);
+    double *B = (double *)malloc(sizeof(double) * 3 * N);
+    long i;
+    int step, rc = 0;
+    if (!y1 || !k || !save || !dd || !B) return 12;
+    for (step = 0; step < nsteps && !rc; step++) {
+        double dtcur = dt / 2.0;
+        memcpy(y1, y0, sizeof(double) * NV);
+        rc |= orc_rhs(g, p, pr, y1, k, B);
+        if (p->sld_on) {
+            /* Compute_Diffusion(1) sees vars1 with B in slots 6-8 (RHS(1) leaves it there) */
+            double *yb = dd;
+            memcpy(yb, y1, sizeof(double) * NV);
+            if (p->magnetic) memcpy(yb + 5 * N, B, sizeof(double) * 3 * N);
+            {
+                double *D = (double *)calloc(NV, sizeof(double));
+                rc |= orc_diffusion(g, p, pr, yb, y1 + 5 * N, D);
+                for (i = 0; i < NV; i++) save[i] = D[i] + k[i] / 6.0;
+                free(D);
+            }
+        } else {
+            for (i = 0; i < NV; i++) save[i] = k[i];
+        }
+        for (i = 0; i < NV; i++) y1[i] = y0[i] + dtcur * k[i];
+        orc_bound(g, p, pr, y1);
+        rc |= orc_rhs(g, p, pr, y1, k, NULL);
+        for (i = 0; i < NV; i++) save[i] = save[i] + k[i] / 3.0;
+        for (i = 0; i < NV; i++) y1[i] = y0[i] + dtcur * k[i];
+        orc_bound(g, p, pr, y1);
+        dtcur = dt;
+        rc |= orc_rhs(g, p, pr, y1, k, NULL);
+        for (i = 0; i < NV; i++) save[i] = save[i] + k[i] / 3.0;
+        for (i = 0; i < NV; i++) y1[i] = y0[i] + dtcur * k[i];
+        orc_bound(g, p, pr, y1);
+        rc |= orc_rhs(g, p, pr, y1, k, NULL);
+        for (i = 0; i < NV; i++) save[i] = save[i] + k[i] / 6.0;
+        for (i = 0; i < NV; i++) y1[i] = y0[i] + dtcur * save[i];
+        orc_bound(g, p, pr, y1);
+        memcpy(y0, y1, sizeof(double) * NV);
+    }
+    free(y1); free(k); free(save); free(dd); free(B);
+    return rc;
+}
