@@ -1,0 +1,928 @@
+// MHD right-hand side, Bound and RK4 stages on gfx950.
+//
+// RHS(call_num) of rhs_mpi.f90:279-783 (stratified, magnetic, Laplacian=F, do_radheat=F) is
+// evaluated as:
+//   1. curl A: six batched derivative sweeps + k_curl_combine (Compute_B, derivatives.f90:1830-1892)
+//   2. k_walls_state: wall values of the closures that depend on the state (rhs_mpi.f90:410-455,
+//      :578-637)
+//   3. the 19 first-derivative sweeps of Source_Theta_Terms / Phi_Terms / Radial_Terms, batched
+//      per axis (one launch per axis, one job per field)
+//   4. k_rhs_main: every pointwise term of those three routines in the reference's order
+//   5. k_walls_density: the density wall derivatives that zero the normal momentum tendency
+//      (rhs_mpi.f90:1553-1564, :1750-1773) - they need step 4's result, hence two phases
+//   6. two density sweeps + k_rhs_final: density terms, top relaxation, u x B, xi_cs
+// Temperature T = rho^(gamma-1) exp(s/Cv)/((gamma-1)Cv) and 1/rho are recomputed where used
+// instead of being stored (rhs_mpi.f90:377-387).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cssx.h"
+#include "cssx_internal.hpp"
+#include "cssx_kernels.hpp"
+#include "cssx_plan.hpp"
+
+namespace cssx {
+
+constexpr double kOneOver4Pi = 1.0 / 3.1415926535897932384626433832795028841972 / 4.0;
+
+template <typename T>
+struct Prof {
+    T *r = nullptr, *rinv = nullptr, *dxdr = nullptr, *grav = nullptr, *eps = nullptr;
+    T *sn = nullptr, *csc = nullptr, *cs = nullptr, *cot = nullptr;
+    T *dsdr1 = nullptr, *dsdr2 = nullptr, *sr1 = nullptr;
+    T *buf = nullptr;
+};
+
+// derivative scratch slots
+enum Slot {
+    D_TW = 0, D_TV, D_TU, D_TS, D_TBP, D_TBR,                 // theta: w v u s bp br
+    D_PRHO, D_PV, D_PW, D_PU, D_PS, D_PBT, D_PBR,             // phi:   rho v w u s bt br
+    D_RW, D_RV, D_RU, D_RS, D_RBT, D_RBP,                     // r:     w v u s bt bp
+    D_NSLOT
+};
+
+template <typename T>
+struct Scratch {
+    T *D = nullptr;      // D_NSLOT fields
+    T *B = nullptr;      // 3 fields: B_theta, B_phi, B_r
+    T *wall = nullptr;   // wall value buffers
+    T *y1 = nullptr;     // RK stage state (nv fields)
+    T *k = nullptr;      // RK stage tendency (nv fields)
+    bool ready = false;
+};
+
+}  // namespace cssx
+
+struct cssx_mhd {
+    cssx_grid grid;
+    cssx_mhd_params p;
+    int nr, nth, nphi, nv;
+    long N;
+    double Cv, gamm1, pconst, invpconst, invtconst, invCv;
+    double dth, dphi;
+    std::vector<double> h_r, h_dr, h_dxdr, h_theta, h_grav, h_eps, h_dsdr1, h_dsdr2, h_sr1, h_sr2, h_rho_bar,
+        h_s_bar, h_diff_speed;
+    cssx::Prof<double> p64;
+    cssx::Prof<float> p32;
+    cssx::Scratch<double> s64;
+    cssx::Scratch<float> s32;
+    double *save = nullptr;   // FP64 RK accumulator (nv fields)
+    double *red = nullptr;    // reduction partials
+    int nred = 0;
+};
+
+namespace cssx {
+
+// ---------------------------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+struct Consts {
+    int nth, nphi, nr;
+    long N;
+    T gamm1, invtconst, invCv, pconst, invpconst, Cv, omega0, om2, xi2, gamma, T0, damp;
+    int magnetic, top_relax, xi_on;
+};
+
+template <typename T>
+__device__ __forceinline__ T temperature(T rho, T s, const Consts<T> &c) {
+    T t = c.invtconst * pow(rho, c.gamm1);
+    return t * exp(s * c.invCv);
+}
+template <>
+__device__ __forceinline__ float temperature<float>(float rho, float s, const Consts<float> &c) {
+    float t = c.invtconst * powf(rho, c.gamm1);
+    return t * expf(s * c.invCv);
+}
+
+template <typename T>
+__device__ __forceinline__ void coords(long i, const Consts<T> &c, int &it, int &ip, int &ir) {
+    it = (int)(i % c.nth);
+    long q = i / c.nth;
+    ip = (int)(q % c.nphi);
+    ir = (int)(q / c.nphi);
+}
+
+// ---------------------------------------------------------------------------------------------
+// 1. curl combine + Compute_B wall conditions (derivatives.f90:2163-2293, :1840-1883)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_curl_combine(Consts<T> c, Prof<T> P, const T *__restrict__ C0,
+                                                      const T *__restrict__ C1, const T *__restrict__ C2,
+                                                      const T *__restrict__ C3, const T *__restrict__ C4,
+                                                      const T *__restrict__ C5, T *__restrict__ B, int perf_cond,
+                                                      int bot_radial, int top_radial) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    int it, ip, ir;
+    coords(i, c, it, ip, ir);
+    const T ri = P.rinv[ir], csc = P.csc[it];
+    T br = C0[i] * csc * ri;
+    T bp = -C1[i] * ri;
+    T bt = ri * C2[i] * csc;
+    br = br - ri * C3[i] * csc;
+    bp = bp + C4[i] * ri;
+    bt = bt - C5[i] * ri;
+    const bool thw = (it == 0 || it == c.nth - 1);
+    if (thw) {
+        if (perf_cond) bt = T(0);
+        else { bp = T(0); br = T(0); }
+    }
+    if ((ir == 0 && bot_radial) || (ir == c.nr - 1 && top_radial)) { bt = T(0); bp = T(0); }
+    B[i] = bt;
+    B[c.N + i] = bp;
+    B[2 * c.N + i] = br;
+}
+
+// ---------------------------------------------------------------------------------------------
+// 2. state-dependent wall values.  theta buffers are (nr, nphi) = pencils of the theta axis,
+//    r buffers (nphi, nth) = pencils of the r axis.
+// ---------------------------------------------------------------------------------------------
+struct WallPtrs {
+    long off_tv_lo, off_tv_hi, off_tbp_lo, off_tbp_hi, off_trho_lo, off_trho_hi;   // theta (nr*nphi)
+    long off_rw_lo, off_rw_hi, off_rv_lo, off_rv_hi, off_rs_lo, off_rs_hi, off_rrho_lo, off_rrho_hi;  // r
+    long total;
+};
+
+inline WallPtrs wall_layout(int nr, int nth, int nphi) {
+    WallPtrs w;
+    long nt = (long)nr * nphi, nrr = (long)nth * nphi, o = 0;
+    w.off_tv_lo = o; o += nt; w.off_tv_hi = o; o += nt;
+    w.off_tbp_lo = o; o += nt; w.off_tbp_hi = o; o += nt;
+    w.off_trho_lo = o; o += nt; w.off_trho_hi = o; o += nt;
+    w.off_rw_lo = o; o += nrr; w.off_rw_hi = o; o += nrr;
+    w.off_rv_lo = o; o += nrr; w.off_rv_hi = o; o += nrr;
+    w.off_rs_lo = o; o += nrr; w.off_rs_hi = o; o += nrr;
+    w.off_rrho_lo = o; o += nrr; w.off_rrho_hi = o; o += nrr;
+    w.total = o;
+    return w;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_walls_state(Consts<T> c, Prof<T> P, WallPtrs W, T *__restrict__ wall,
+                                                     const T *__restrict__ st, const T *__restrict__ B,
+                                                     int bot_sf, int top_sf, int bot_flux, int top_flux,
+                                                     int theta_normal) {
+    const long nt = (long)c.nr * c.nphi, nrr = (long)c.nth * c.nphi;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const T *w = st + c.N, *v = st + 2 * c.N;
+    if (i < nt) {   // theta walls (rhs_mpi.f90:410-455): pencil (ip, ir)
+        const int ip = (int)(i % c.nphi), ir = (int)(i / c.nphi);
+        const long q0 = ((long)ir * c.nphi + ip) * c.nth, q1 = q0 + c.nth - 1;
+        wall[W.off_tv_lo + i] = P.cs[0] / P.sn[0] * v[q0];
+        wall[W.off_tv_hi + i] = P.cs[c.nth - 1] / P.sn[c.nth - 1] * v[q1];
+        if (c.magnetic) {
+            const T *bp = B + c.N;
+            wall[W.off_tbp_lo + i] = theta_normal ? -P.cs[0] / P.sn[0] * bp[q0] : T(0);
+            wall[W.off_tbp_hi + i] = theta_normal ? -P.cs[c.nth - 1] / P.sn[c.nth - 1] * bp[q1] : T(0);
+        }
+    }
+    if (i < nrr) {  // radial walls (rhs_mpi.f90:578-637): pencil (it, ip)
+        const int it = (int)(i % c.nth);
+        const long q0 = i, q1 = (long)(c.nr - 1) * nrr + i;
+        const T r1 = P.r[0], r2 = P.r[c.nr - 1], x1 = P.dxdr[0], x2 = P.dxdr[c.nr - 1];
+        wall[W.off_rw_lo + i] = bot_sf ? w[q0] / r1 / x1 : T(0);
+        wall[W.off_rw_hi + i] = top_sf ? w[q1] / r2 / x2 : w[q1];
+        wall[W.off_rv_lo + i] = bot_sf ? v[q0] / r1 / x1 : T(0);
+        wall[W.off_rv_hi + i] = top_sf ? v[q1] / r2 / x2 : v[q1];
+        wall[W.off_rs_lo + i] = bot_flux ? P.dsdr1[it] / x1 : T(0);
+        wall[W.off_rs_hi + i] = top_flux ? P.dsdr2[it] / x2 : T(0);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 4. pointwise terms of Source_Theta_Terms, Phi_Terms, Radial_Terms (reference order)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const T *__restrict__ st,
+                                                  const T *__restrict__ B, const T *__restrict__ D,
+                                                  T *__restrict__ dv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    const long N = c.N;
+    int it, ip, ir;
+    coords(i, c, it, ip, ir);
+    const T ri = P.rinv[ir], rv = P.r[ir], epsr = P.eps[ir];
+    const T omsq = rv * c.omega0 * c.omega0, om2 = c.om2;
+    const T sn = P.sn[it], cs = P.cs[it], cot = P.cot[it];
+    const T Rho = st[i], W = st[N + i], V = st[2 * N + i], U = st[3 * N + i], S = st[4 * N + i];
+    const T t = temperature(Rho, S, c);
+    const T invrho = T(1) / Rho;
+    const bool mag = c.magnetic != 0;
+    T Bt = T(0), Bp = T(0), Br = T(0);
+    if (mag) { Bt = B[i]; Bp = B[N + i]; Br = B[2 * N + i]; }
+    T xdw, xdv, xdu, xdrho, xds, wk, drho_out, dw_out = T(0), dv_out = T(0), du_out = T(0);
+    // ---- Source_Theta_Terms (rhs_mpi.f90:812-1027)
+    xdw = omsq * sn * cs;
+    xdu = -P.grav[ir] + omsq * sn * sn;
+    xdrho = -Rho * W * cot * ri;
+    xdv = -om2 * cs * W;
+    xdu = xdu + ri * W * W;
+    wk = D[D_TW * N + i] * ri;
+    xdrho = xdrho - Rho * wk;
+    xdw = xdw - W * wk;
+    xdw = xdw + V * (ri * cot * V + om2 * cs);
+    xdv = xdv - ri * V * W * cot;
+    xdu = xdu + V * (ri * V + om2 * sn);
+    wk = D[D_TV * N + i] * ri;
+    xdv = xdv - W * wk;
+    drho_out = xdrho - T(2) * ri * Rho * U;
+    xdv = xdv - U * (ri * V + om2 * sn);
+    xdw = xdw - ri * U * W;
+    if (!mag) dv_out = xdv;
+    wk = D[D_TU * N + i] * ri;
+    xdu = xdu - W * wk;
+    if (!mag) du_out = xdu;
+    xds = (fabs(epsr) > T(0)) ? epsr * invrho / t : T(0);
+    wk = D[D_TS * N + i] * ri;
+    if (mag) xdw = xdw - c.gamm1 * t * wk;
+    else dw_out = xdw - c.gamm1 * t * wk;
+    xds = xds - W * wk;
+    if (mag) {
+        const T oorho = invrho * T(kOneOver4Pi) * ri;
+        xdu = xdu - Bt * Bt * oorho;
+        xdw = xdw - Bp * Bp * cot * oorho;
+        xdv = xdv + Bp * Bt * cot * oorho;
+        xdu = xdu - Bp * Bp * oorho;
+        wk = D[D_TBP * N + i] * oorho;
+        xdv = xdv + Bt * wk;
+        xdw = xdw - Bp * wk;
+        wk = D[D_TBR * N + i] * oorho;
+        dw_out = xdw + Br * (Bt * oorho - wk);
+        dv_out = xdv + Br * Bp * oorho;
+        du_out = xdu + Bt * wk;
+    }
+    // ---- Phi_Terms (rhs_mpi.f90:1064-1260)
+    {
+        const T oors = ri * P.csc[it];
+        T oorho = invrho;
+        wk = D[D_PRHO * N + i] * oors;
+        xdrho = drho_out;
+        xdv = dv_out;
+        xdrho = xdrho - V * wk;
+        xdv = xdv - c.pconst * wk * t * oorho;
+        // ds carried in xds
+        wk = D[D_PV * N + i] * oors;
+        xdv = xdv - V * wk;
+        drho_out = xdrho - Rho * wk;
+        xdw = dw_out;
+        wk = D[D_PW * N + i] * oors;
+        xdw = xdw - V * wk;
+        xdu = du_out;
+        if (!mag) dw_out = xdw;
+        wk = D[D_PU * N + i] * oors;
+        xdu = xdu - V * wk;
+        if (!mag) du_out = xdu;
+        wk = D[D_PS * N + i] * oors;
+        xds = xds - V * wk;
+        xdv = xdv - c.gamm1 * t * wk;
+        if (!mag) dv_out = xdv;
+        if (mag) {
+            oorho = T(kOneOver4Pi) * oorho * oors;
+            wk = D[D_PBT * N + i] * oorho;
+            xdv = xdv - Bt * wk;
+            dw_out = xdw + Bp * wk;
+            wk = D[D_PBR * N + i] * oorho;
+            dv_out = xdv - Br * wk;
+            du_out = xdu + Bp * wk;
+        }
+    }
+    // ---- Radial_Terms (rhs_mpi.f90:1297-1518); r derivatives already carry dx/dr
+    {
+        wk = D[D_RW * N + i];
+        xdw = dw_out - U * wk;
+        if (!mag) dw_out = xdw;
+        wk = D[D_RV * N + i];
+        xdv = dv_out - U * wk;
+        if (!mag) dv_out = xdv;
+        xdu = du_out;
+        wk = D[D_RU * N + i];
+        xdu = xdu - U * wk;
+        drho_out = drho_out - Rho * wk;
+        wk = D[D_RS * N + i];
+        xdu = xdu - c.gamm1 * t * wk;
+        xds = xds - U * wk;
+        if (!mag) du_out = xdu;
+        if (mag) {
+            const T oorho = invrho * T(kOneOver4Pi);
+            wk = D[D_RBT * N + i] * oorho;
+            xdu = xdu - Bt * wk;
+            dw_out = xdw + Br * wk;
+            wk = D[D_RBP * N + i] * oorho;
+            dv_out = xdv + Br * wk;
+            du_out = xdu - Bp * wk;
+        }
+    }
+    dv[i] = drho_out;
+    dv[N + i] = dw_out;
+    dv[2 * N + i] = dv_out;
+    dv[3 * N + i] = du_out;
+    dv[4 * N + i] = xds;
+}
+
+// ---------------------------------------------------------------------------------------------
+// 5. density wall derivatives from the momentum tendency (rhs_mpi.f90:1553-1564, :1750-1773)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, WallPtrs W, T *__restrict__ wall,
+                                                       const T *__restrict__ st, const T *__restrict__ dv,
+                                                       int lbcr_rho) {
+    const long nt = (long)c.nr * c.nphi, nrr = (long)c.nth * c.nphi, N = c.N;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nt) {
+        const int ir = (int)(i / c.nphi), ip = (int)(i % c.nphi);
+        const T rv = P.r[ir] * c.invpconst;
+        const long q0 = ((long)ir * c.nphi + ip) * c.nth, q1 = q0 + c.nth - 1;
+        T rho0 = st[q0], rho1 = st[q1];
+        T t0 = temperature(rho0, st[4 * N + q0], c), t1 = temperature(rho1, st[4 * N + q1], c);
+        wall[W.off_trho_lo + i] = rv * dv[N + q0] * rho0 / t0;
+        wall[W.off_trho_hi + i] = rv * dv[N + q1] * rho1 / t1;
+    }
+    if (i < nrr) {
+        const long q0 = i, q1 = (long)(c.nr - 1) * nrr + i;
+        T rho0 = st[q0], rho1 = st[q1];
+        T t0 = temperature(rho0, st[4 * N + q0], c), t1 = temperature(rho1, st[4 * N + q1], c);
+        T lo = (lbcr_rho == 2 || lbcr_rho == 4) ? dv[3 * N + q0] * rho0 * c.invpconst / t0 : rho0;
+        T hi = (lbcr_rho == 3 || lbcr_rho == 4) ? dv[3 * N + q1] * rho1 * c.invpconst / t1 : rho1;
+        wall[W.off_rrho_lo + i] = lo / P.dxdr[0];
+        wall[W.off_rrho_hi + i] = hi / P.dxdr[c.nr - 1];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// 6. density terms, top relaxation, induction, xi_cs (rhs_mpi.f90:1580-1596, :1781-1818,
+//    :2342-2346, :769)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const T *__restrict__ st,
+                                                   const T *__restrict__ B, const T *__restrict__ dth_rho,
+                                                   const T *__restrict__ dr_rho, T *__restrict__ dv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    const long N = c.N;
+    int it, ip, ir;
+    coords(i, c, it, ip, ir);
+    const T Rho = st[i], W = st[N + i], V = st[2 * N + i], U = st[3 * N + i], S = st[4 * N + i];
+    const T t = temperature(Rho, S, c);
+    const T invrho = T(1) / Rho;
+    T drho = dv[i], dw = dv[N + i], du = dv[3 * N + i], ds = dv[4 * N + i];
+    // Density_Terms_Theta
+    T wk = P.rinv[ir] * dth_rho[i];
+    drho = drho - W * wk;
+    wk = wk * invrho;
+    dw = dw - c.pconst * wk * t;
+    // Density_Terms_Radial
+    wk = dr_rho[i];
+    drho = drho - U * wk;
+    wk = wk * invrho;
+    du = du - c.pconst * t * wk;
+    if (c.top_relax && ir == c.nr - 1) {
+        const T dt_ = c.damp * (c.gamma - T(2));
+        drho = drho + Rho * (c.T0 - t) / (dt_ * t);
+        ds = ds - c.Cv * (c.T0 - t) / (dt_ * t);
+    }
+    if (c.xi_on) drho = c.xi2 * drho;
+    dv[i] = drho;
+    dv[N + i] = dw;
+    dv[3 * N + i] = du;
+    dv[4 * N + i] = ds;
+    if (c.magnetic) {
+        const T Bt = B[i], Bp = B[N + i], Br = B[2 * N + i];
+        dv[5 * N + i] = V * Br - U * Bp;
+        dv[6 * N + i] = U * Bt - W * Br;
+        dv[7 * N + i] = W * Bp - V * Bt;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Bound (rhs_mpi.f90:2395-2491) as a per-point predicate, fused into the stage update
+// ---------------------------------------------------------------------------------------------
+struct BoundFlags {
+    int theta_stress_free, bot_sf, top_sf, bot_ctemp, magnetic, perf_cond, bot_radial, top_radial;
+};
+
+template <typename T>
+__device__ __forceinline__ T bound_value(int iv, int it, int ip, int ir, T val, const BoundFlags &f, int nth,
+                                         int nr, const T *sr1) {
+    const bool thw = (it == 0 || it == nth - 1);
+    if (iv == 1 && thw) return T(0);                                     // w at theta walls
+    if (iv == 3 && thw && !f.theta_stress_free) return T(0);
+    if (ir == 0) {
+        if ((iv == 1 || iv == 2) && !f.bot_sf) return T(0);
+        if (iv == 3) return T(0);                                        // u_r at r1
+        if (iv == 4 && f.bot_ctemp) return sr1[it];
+    }
+    if (ir == nr - 1) {
+        if ((iv == 1 || iv == 2) && !f.top_sf) return T(0);
+        if (iv == 3) return T(0);                                        // u_r at r2
+    }
+    if (f.magnetic && iv >= 5) {
+        if (thw) {
+            if (f.perf_cond) { if (iv == 6 || iv == 7) return T(0); }
+            else if (iv == 5) return T(0);                               // A_theta at theta walls
+        }
+        if (iv == 7 && ((ir == 0 && f.bot_radial) || (ir == nr - 1 && f.top_radial))) return T(0);
+    }
+    return val;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_bound(Consts<T> c, BoundFlags f, const T *__restrict__ sr1, T *st, int nv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N * nv) return;
+    const int iv = (int)(i / c.N);
+    const long j = i - (long)iv * c.N;
+    int it, ip, ir;
+    coords(j, c, it, ip, ir);
+    st[i] = bound_value(iv, it, ip, ir, st[i], f, c.nth, c.nr, sr1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// RK4 stage update (step_rk4.f90:392, 418-419, 444-445, 470-471) + Bound.
+//   stage 1: save = Dif + k/6   y1 = y0 + (dt/2) k
+//   stage 2: save += k/3        y1 = y0 + (dt/2) k
+//   stage 3: save += k/3        y1 = y0 + dt k
+//   stage 4: save += k/6        y1 = y0 + dt save; y0 = y1
+// y0/save are FP64; k and y1 are in the stage precision TS.
+// ---------------------------------------------------------------------------------------------
+template <typename TS>
+__global__ __launch_bounds__(256) void k_rk_stage(Consts<TS> c, BoundFlags f, const TS *__restrict__ sr1,
+                                                  const double *__restrict__ sr1_64, int stage, double dt,
+                                                  double *__restrict__ y0, double *__restrict__ save,
+                                                  const TS *__restrict__ k, const double *__restrict__ dif,
+                                                  TS *__restrict__ y1, int nv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N * nv) return;
+    const int iv = (int)(i / c.N);
+    const long j = i - (long)iv * c.N;
+    int it, ip, ir;
+    coords(j, c, it, ip, ir);
+    const double kk = (double)k[i];
+    double s, y;
+    if (stage == 1) {
+        s = (dif ? dif[i] : 0.0) + kk / 6.0;
+        y = y0[i] + (dt / 2.0) * kk;
+    } else if (stage == 2) {
+        s = save[i] + kk / 3.0;
+        y = y0[i] + (dt / 2.0) * kk;
+    } else if (stage == 3) {
+        s = save[i] + kk / 3.0;
+        y = y0[i] + dt * kk;
+    } else {
+        s = save[i] + kk / 6.0;
+        y = y0[i] + dt * s;
+    }
+    if (stage < 4) {
+        save[i] = s;
+        y1[i] = bound_value<TS>(iv, it, ip, ir, (TS)y, f, c.nth, c.nr, sr1);
+    } else {
+        const double yb = bound_value<double>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_64);
+        y0[i] = yb;
+        if (sizeof(TS) != sizeof(double)) y1[i] = (TS)yb;   // next step's stage-1 input
+    }
+}
+
+// y1 = (TS) y0 (first step of an FP32 run)
+template <typename TS>
+__global__ __launch_bounds__(256) void k_cast_state(long n, const double *__restrict__ y0, TS *__restrict__ y1) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y1[i] = (TS)y0[i];
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Compute_Step_Size (rhs_mpi.f90:145-185, stratified branch): per-block minima of the sound/flow
+// crossing time and of the Alfven-included crossing time.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cfl(Consts<double> c, Prof<double> P, const double *__restrict__ dr,
+                                             double dth, double dphi, double Cp, const double *__restrict__ st,
+                                             const double *__restrict__ B, double *__restrict__ red) {
+    __shared__ double sc[256], sb[256];
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double vc = 1e4, vb = 1e4;
+    if (i < c.N) {
+        int it, ip, ir;
+        coords(i, c, it, ip, ir);
+        const long N = c.N;
+        const double radius = P.r[ir], sinth = P.sn[it];
+        const double dxmin = fmin(dr[ir], radius * fmin(dth, sinth * dphi));
+        const double rho = st[i], w = st[N + i], v = st[2 * N + i], u = st[3 * N + i], s = st[4 * N + i];
+        const double t = temperature(rho, s, c);
+        double ut = u * u + v * v + w * w + (c.gamm1 * Cp * c.xi2) * t;
+        vc = dxmin / sqrt(ut);
+        if (c.magnetic) {
+            const double b2 = B[i] * B[i] + B[N + i] * B[N + i] + B[2 * N + i] * B[2 * N + i];
+            vb = dxmin / sqrt(ut + b2 * kOneOver4Pi / rho);
+        }
+    }
+    sc[threadIdx.x] = vc;
+    sb[threadIdx.x] = vb;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sc[threadIdx.x] = fmin(sc[threadIdx.x], sc[threadIdx.x + o]);
+            sb[threadIdx.x] = fmin(sb[threadIdx.x], sb[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        red[2 * blockIdx.x] = sc[0];
+        red[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+}  // namespace cssx
+// =============================================================================================
+// Host orchestration
+// =============================================================================================
+namespace cssx {
+
+template <typename T>
+static Consts<T> make_consts(const cssx_mhd *m) {
+    Consts<T> c;
+    c.nth = m->nth; c.nphi = m->nphi; c.nr = m->nr; c.N = m->N;
+    c.gamm1 = (T)m->gamm1; c.invtconst = (T)m->invtconst; c.invCv = (T)m->invCv; c.pconst = (T)m->pconst;
+    c.invpconst = (T)m->invpconst; c.Cv = (T)m->Cv; c.omega0 = (T)m->p.omega0; c.om2 = (T)(2.0 * m->p.omega0);
+    c.xi2 = (T)(m->p.xi_cs * m->p.xi_cs); c.gamma = (T)m->p.gamma; c.T0 = (T)m->p.T0; c.damp = (T)m->p.damping_time;
+    c.magnetic = m->p.magnetic; c.top_relax = m->p.top_relax; c.xi_on = m->p.xi_cs < 1.0;
+    return c;
+}
+
+static BoundFlags make_bound_flags(const cssx_mhd *m) {
+    BoundFlags f;
+    f.theta_stress_free = m->p.theta_stress_free; f.bot_sf = m->p.bottom_stress_free; f.top_sf = m->p.top_stress_free;
+    f.bot_ctemp = m->p.bottom_const_temp; f.magnetic = m->p.magnetic; f.perf_cond = m->p.theta_perf_cond;
+    f.bot_radial = m->p.bottom_radial_field; f.top_radial = m->p.top_radial_field;
+    return f;
+}
+
+template <typename T>
+static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
+    const int nr = m->nr, nth = m->nth;
+    const size_t tot = 5 * (size_t)nr + 7 * (size_t)nth;
+    std::vector<T> h(tot);
+    size_t o = 0;
+    auto put = [&](T *&dst_off_holder, size_t &off, size_t n, auto gen) {
+        (void)dst_off_holder;
+        for (size_t i = 0; i < n; i++) h[off + i] = (T)gen(i);
+        off += n;
+    };
+    size_t o_r = o; put(P.r, o, nr, [&](size_t i) { return m->h_r[i]; });
+    size_t o_ri = o; put(P.rinv, o, nr, [&](size_t i) { return 1.0 / m->h_r[i]; });
+    size_t o_x = o; put(P.dxdr, o, nr, [&](size_t i) { return m->h_dxdr[i]; });
+    size_t o_g = o; put(P.grav, o, nr, [&](size_t i) { return m->h_grav[i]; });
+    size_t o_e = o; put(P.eps, o, nr, [&](size_t i) { return m->h_eps[i]; });
+    size_t o_sn = o; put(P.sn, o, nth, [&](size_t i) { return std::sin(m->h_theta[i]); });
+    size_t o_csc = o; put(P.csc, o, nth, [&](size_t i) { return 1.0 / std::sin(m->h_theta[i]); });
+    size_t o_cs = o; put(P.cs, o, nth, [&](size_t i) { return std::cos(m->h_theta[i]); });
+    size_t o_cot = o; put(P.cot, o, nth, [&](size_t i) { return std::cos(m->h_theta[i]) / std::sin(m->h_theta[i]); });
+    size_t o_d1 = o; put(P.dsdr1, o, nth, [&](size_t i) { return m->h_dsdr1[i]; });
+    size_t o_d2 = o; put(P.dsdr2, o, nth, [&](size_t i) { return m->h_dsdr2[i]; });
+    size_t o_s1 = o; put(P.sr1, o, nth, [&](size_t i) { return m->h_sr1[i]; });
+    T *d = nullptr;
+    if (hipMalloc(&d, tot * sizeof(T)) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMemcpy(d, h.data(), tot * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return CSSX_EHIP;
+    P.buf = d;
+    P.r = d + o_r; P.rinv = d + o_ri; P.dxdr = d + o_x; P.grav = d + o_g; P.eps = d + o_e;
+    P.sn = d + o_sn; P.csc = d + o_csc; P.cs = d + o_cs; P.cot = d + o_cot;
+    P.dsdr1 = d + o_d1; P.dsdr2 = d + o_d2; P.sr1 = d + o_s1;
+    return CSSX_OK;
+}
+
+template <typename T>
+static Scratch<T> &scratch(cssx_mhd *m);
+template <>
+Scratch<double> &scratch<double>(cssx_mhd *m) { return m->s64; }
+template <>
+Scratch<float> &scratch<float>(cssx_mhd *m) { return m->s32; }
+template <typename T>
+static Prof<T> &prof(cssx_mhd *m);
+template <>
+Prof<double> &prof<double>(cssx_mhd *m) { return m->p64; }
+template <>
+Prof<float> &prof<float>(cssx_mhd *m) { return m->p32; }
+
+template <typename T>
+static int ensure_scratch(cssx_mhd *m) {
+    Scratch<T> &S = scratch<T>(m);
+    if (S.ready) return CSSX_OK;
+    const long N = m->N;
+    WallPtrs W = wall_layout(m->nr, m->nth, m->nphi);
+    if (hipMalloc(&S.D, sizeof(T) * N * D_NSLOT) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMalloc(&S.B, sizeof(T) * N * 3) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMalloc(&S.wall, sizeof(T) * W.total) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMalloc(&S.y1, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMalloc(&S.k, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMemset(S.wall, 0, sizeof(T) * W.total) != hipSuccess) return CSSX_EHIP;
+    S.ready = true;
+    return CSSX_OK;
+}
+
+static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// one batched derivative launch along an axis
+template <typename T>
+static int derivs_axis(cssx_mhd *m, int axis, DerivSpec *specs, int ns, hipStream_t st) {
+    AxisGeom g;
+    axis_geometry(axis, m->nth, m->nphi, m->nr, g);
+    return run_derivs(specs, ns, g, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st);
+}
+
+// DerivSpec for one field on the single-subdomain walls
+template <typename T>
+static DerivSpec spec(cssx_mhd *m, int axis, const T *in, T *out, int code, const T *wlo, const T *whi,
+                      const T *pre = nullptr, const T *post = nullptr) {
+    DerivSpec s;
+    s.order = 1;
+    s.in = in; s.out = out;
+    s.glo = wlo; s.ghi = whi;
+    s.pre = pre; s.post = post;
+    s.bdry = code;
+    if (axis == CSSX_AXIS_PHI) {
+        s.lo_kind = s.hi_kind = END_PERIODIC;
+        s.scale = m->grid.dpi;
+        s.bdry = 0;
+    } else {
+        s.lo_kind = s.hi_kind = END_WALL;
+        const double inv = axis == CSSX_AXIS_R ? m->grid.dri : m->grid.dti;
+        s.scale = inv;
+        s.neu_mul = 1.0 / inv;
+        // wall-value buffers are dense per pencil: (nr, nphi) on theta, (nphi, nth) on r
+        if (axis == CSSX_AXIS_THETA) { s.g_k = 0; s.g_a = 1; s.g_b = m->nphi; }
+        else { s.g_k = 0; s.g_a = 1; s.g_b = m->nth; }
+    }
+    return s;
+}
+
+template <typename T>
+static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream) {
+    int rc = ensure_scratch<T>(m);
+    if (rc) return rc;
+    Scratch<T> &S = scratch<T>(m);
+    Prof<T> &P = prof<T>(m);
+    const Consts<T> c = make_consts<T>(m);
+    const long N = m->N;
+    const int *ibcr = m->p.ibcr, *ibct = m->p.ibct;
+    T *D = S.D;
+    const unsigned nb = nblocks(N, 256);
+    const T *at = A, *ap = A + N, *ar = A + 2 * N;
+    DerivSpec th[2] = {spec<T>(m, CSSX_AXIS_THETA, ap, D + 0 * N, ibct[11], nullptr, nullptr, P.sn),
+                       spec<T>(m, CSSX_AXIS_THETA, ar, D + 1 * N, ibct[12], nullptr, nullptr)};
+    if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, 2, stream))) return rc;
+    DerivSpec ph[2] = {spec<T>(m, CSSX_AXIS_PHI, ar, D + 2 * N, 0, nullptr, nullptr),
+                       spec<T>(m, CSSX_AXIS_PHI, at, D + 3 * N, 0, nullptr, nullptr)};
+    if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, 2, stream))) return rc;
+    DerivSpec rr[2] = {spec<T>(m, CSSX_AXIS_R, at, D + 4 * N, ibcr[10], nullptr, nullptr, P.r, P.dxdr),
+                       spec<T>(m, CSSX_AXIS_R, ap, D + 5 * N, ibcr[11], nullptr, nullptr, P.r, P.dxdr)};
+    if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, 2, stream))) return rc;
+    hipLaunchKernelGGL(k_curl_combine<T>, dim3(nb), dim3(256), 0, stream, c, P, D, D + N, D + 2 * N, D + 3 * N,
+                       D + 4 * N, D + 5 * N, B, m->p.theta_perf_cond, m->p.bottom_radial_field,
+                       m->p.top_radial_field);
+    return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+template <typename T>
+static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream) {
+    int rc = ensure_scratch<T>(m);
+    if (rc) return rc;
+    Scratch<T> &S = scratch<T>(m);
+    Prof<T> &P = prof<T>(m);
+    const Consts<T> c = make_consts<T>(m);
+    const long N = m->N;
+    const int *ibcr = m->p.ibcr, *ibct = m->p.ibct;
+    T *D = S.D;
+    T *B = Bout ? Bout : S.B;
+    const WallPtrs W = wall_layout(m->nr, m->nth, m->nphi);
+    T *wall = S.wall;
+    const T *rho = st, *w = st + N, *v = st + 2 * N, *u = st + 3 * N, *s = st + 4 * N;
+    const unsigned nb = nblocks(N, 256);
+    // 1. B = curl A
+    if (m->p.magnetic && (rc = curl_impl<T>(m, st + 5 * N, B, stream))) return rc;
+    const T *bt = B, *bp = B + N, *br = B + 2 * N;
+    // 2. wall values
+    {
+        long nw = std::max((long)m->nr * m->nphi, (long)m->nth * m->nphi);
+        hipLaunchKernelGGL(k_walls_state<T>, dim3(nblocks(nw, 256)), dim3(256), 0, stream, c, P, W, wall, st, B,
+                           m->p.bottom_stress_free, m->p.top_stress_free, m->p.bottom_const_flux,
+                           m->p.top_const_flux, m->p.theta_normal_field);
+    }
+    // 3. derivative sweeps
+    {
+        const int mag = m->p.magnetic;
+        DerivSpec th[6] = {spec<T>(m, CSSX_AXIS_THETA, w, D + D_TW * N, ibct[2], nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_THETA, v, D + D_TV * N, ibct[3], wall + W.off_tv_lo, wall + W.off_tv_hi),
+                           spec<T>(m, CSSX_AXIS_THETA, u, D + D_TU * N, ibct[4], nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_THETA, s, D + D_TS * N, ibct[5], nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_THETA, bp, D + D_TBP * N, ibct[8], wall + W.off_tbp_lo, wall + W.off_tbp_hi),
+                           spec<T>(m, CSSX_AXIS_THETA, br, D + D_TBR * N, ibct[9], nullptr, nullptr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, mag ? 6 : 4, stream))) return rc;
+        DerivSpec ph[7] = {spec<T>(m, CSSX_AXIS_PHI, rho, D + D_PRHO * N, 0, nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_PHI, v, D + D_PV * N, 0, nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_PHI, w, D + D_PW * N, 0, nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_PHI, u, D + D_PU * N, 0, nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_PHI, s, D + D_PS * N, 0, nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_PHI, bt, D + D_PBT * N, 0, nullptr, nullptr),
+                           spec<T>(m, CSSX_AXIS_PHI, br, D + D_PBR * N, 0, nullptr, nullptr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, mag ? 7 : 5, stream))) return rc;
+        DerivSpec rr[6] = {spec<T>(m, CSSX_AXIS_R, w, D + D_RW * N, ibcr[2], wall + W.off_rw_lo, wall + W.off_rw_hi, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, v, D + D_RV * N, ibcr[3], wall + W.off_rv_lo, wall + W.off_rv_hi, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, u, D + D_RU * N, ibcr[4], nullptr, nullptr, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, s, D + D_RS * N, ibcr[5], wall + W.off_rs_lo, wall + W.off_rs_hi, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, bt, D + D_RBT * N, ibcr[7], nullptr, nullptr, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, bp, D + D_RBP * N, ibcr[8], nullptr, nullptr, nullptr, P.dxdr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, mag ? 6 : 4, stream))) return rc;
+    }
+    // 4. pointwise momentum / entropy / continuity terms
+    hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv);
+    // 5. density wall derivatives from the momentum tendency
+    {
+        long nw = std::max((long)m->nr * m->nphi, (long)m->nth * m->nphi);
+        hipLaunchKernelGGL(k_walls_density<T>, dim3(nblocks(nw, 256)), dim3(256), 0, stream, c, P, W, wall, st, dv,
+                           ibcr[1]);
+    }
+    // 6. density sweeps + final terms (reuse theta slots 0/1)
+    {
+        DerivSpec th = spec<T>(m, CSSX_AXIS_THETA, rho, D + 0 * N, ibct[1], wall + W.off_trho_lo, wall + W.off_trho_hi);
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, &th, 1, stream))) return rc;
+        DerivSpec rr = spec<T>(m, CSSX_AXIS_R, rho, D + 1 * N, ibcr[1], wall + W.off_rrho_lo, wall + W.off_rrho_hi,
+                               nullptr, P.dxdr);
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, &rr, 1, stream))) return rc;
+        hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N, D + 1 * N, dv);
+    }
+    return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+template <typename T>
+static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t stream) {
+    int rc = ensure_scratch<T>(m);
+    if (rc) return rc;
+    Scratch<T> &S = scratch<T>(m);
+    Prof<T> &P = prof<T>(m);
+    const Consts<T> c = make_consts<T>(m);
+    const BoundFlags f = make_bound_flags(m);
+    const long NV = m->N * m->nv;
+    const unsigned nb = nblocks(NV, 256);
+    if (m->p.sld_on && m->p.sld_coef != 0.0) return CSSX_EINVAL;   // SLD fluxes: not in this build yet
+    const bool f32 = sizeof(T) != sizeof(double);
+    if (f32) hipLaunchKernelGGL(k_cast_state<T>, dim3(nb), dim3(256), 0, stream, NV, (const double *)y0, S.y1);
+    for (int step = 0; step < nsteps; step++) {
+        // stage 1 reads vars1 = vars0: the FP64 run reads y0 directly, the FP32 run its cast
+        const T *in1 = f32 ? S.y1 : (const T *)y0;
+        if ((rc = rhs_impl<T>(m, in1, S.k, nullptr, stream))) return rc;
+        hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1, 1, dt, y0, m->save,
+                           (const T *)S.k, (const double *)nullptr, S.y1, m->nv);
+        for (int stage = 2; stage <= 4; stage++) {
+            if ((rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
+            hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1, stage, dt, y0,
+                               m->save, (const T *)S.k, (const double *)nullptr, S.y1, m->nv);
+        }
+    }
+    return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+}  // namespace cssx
+
+using namespace cssx;
+
+extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *params, const cssx_mhd_profiles *pr,
+                               cssx_mhd **out) {
+    if (!grid || !params || !pr || !out) return CSSX_EINVAL;
+    *out = nullptr;
+    if (!pr->r || !pr->dr || !pr->dxdr || !pr->theta || !pr->gravity || !pr->eps || !pr->dsdr1 || !pr->dsdr2 ||
+        !pr->sr1 || !pr->sr2)
+        return CSSX_EINVAL;
+    if (grid->nr < 8 || grid->nth < 8 || grid->nphi < 8 || choose_M(grid->nr) < 0 || choose_M(grid->nth) < 0 ||
+        choose_M(grid->nphi) < 0)
+        return CSSX_EINVAL;
+    if (!grid->r_lo_wall || !grid->r_hi_wall || !grid->th_lo_wall || !grid->th_hi_wall) return CSSX_EINVAL;
+    if (params->gamma <= 1.0 || params->Cp <= 0.0) return CSSX_EINVAL;
+    for (int i = 1; i <= 12; i++)
+        if (params->ibcr[i] < 0 || params->ibcr[i] > 4 || params->ibct[i] < 0 || params->ibct[i] > 4)
+            return CSSX_EINVAL;
+    cssx_mhd *m = new (std::nothrow) cssx_mhd;
+    if (!m) return CSSX_ENOMEM;
+    m->grid = *grid;
+    m->p = *params;
+    m->nr = grid->nr; m->nth = grid->nth; m->nphi = grid->nphi;
+    m->nv = params->magnetic ? 8 : 5;
+    m->N = (long)grid->nr * grid->nth * grid->nphi;
+    m->Cv = params->Cp / params->gamma;
+    m->invCv = 1.0 / m->Cv;
+    m->gamm1 = params->gamma - 1.0;
+    m->pconst = m->gamm1 * params->Cp;
+    m->invpconst = (1.0 / params->Cp) / m->gamm1;
+    m->invtconst = m->invCv / m->gamm1;
+    const int nr = grid->nr, nth = grid->nth;
+    m->h_r.assign(pr->r, pr->r + nr);
+    m->h_dr.assign(pr->dr, pr->dr + nr);
+    m->h_dxdr.assign(pr->dxdr, pr->dxdr + nr);
+    m->h_theta.assign(pr->theta, pr->theta + nth);
+    m->h_grav.assign(pr->gravity, pr->gravity + nr);
+    m->h_eps.assign(pr->eps, pr->eps + nr);
+    m->h_dsdr1.assign(pr->dsdr1, pr->dsdr1 + nth);
+    m->h_dsdr2.assign(pr->dsdr2, pr->dsdr2 + nth);
+    m->h_sr1.assign(pr->sr1, pr->sr1 + nth);
+    m->h_sr2.assign(pr->sr2, pr->sr2 + nth);
+    if (pr->rho_bar) m->h_rho_bar.assign(pr->rho_bar, pr->rho_bar + nr);
+    if (pr->s_bar) m->h_s_bar.assign(pr->s_bar, pr->s_bar + nr);
+    if (pr->diffusion_speed) m->h_diff_speed.assign(pr->diffusion_speed, pr->diffusion_speed + nr + 2);
+    m->dth = (m->h_theta[nth - 1] - m->h_theta[0]) / (nth - 1);
+    m->dphi = 1.0 / grid->dpi;
+    int rc = upload_profiles<double>(m, m->p64);
+    if (!rc) rc = upload_profiles<float>(m, m->p32);
+    if (!rc && hipMalloc(&m->save, sizeof(double) * m->N * m->nv) != hipSuccess) rc = CSSX_ENOMEM;
+    // plans for every code this grid can use
+    if (!rc) {
+        cssx_ctx *ctx = nullptr;
+        rc = cssx_ctx_create(grid, &ctx);
+        cssx_ctx_destroy(ctx);
+    }
+    if (rc) {
+        cssx_mhd_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return CSSX_OK;
+}
+
+extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
+    if (!m) return;
+    for (void *p : {(void *)m->p64.buf, (void *)m->p32.buf, (void *)m->s64.D, (void *)m->s64.B, (void *)m->s64.wall,
+                    (void *)m->s64.y1, (void *)m->s64.k, (void *)m->s32.D, (void *)m->s32.B, (void *)m->s32.wall,
+                    (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red})
+        if (p) (void)hipFree(p);
+    delete m;
+}
+
+extern "C" int cssx_rhs(cssx_mhd *m, int prec, const void *state, void *rhs, void *b_out, void *stream) {
+    if (!m || !state || !rhs) return CSSX_EINVAL;
+    if (prec == CSSX_PREC_F64)
+        return rhs_impl<double>(m, (const double *)state, (double *)rhs, (double *)b_out, (hipStream_t)stream);
+    if (prec == CSSX_PREC_F32)
+        return rhs_impl<float>(m, (const float *)state, (float *)rhs, (float *)b_out, (hipStream_t)stream);
+    return CSSX_EINVAL;
+}
+
+extern "C" int cssx_bound(cssx_mhd *m, int prec, void *state, void *stream) {
+    if (!m || !state) return CSSX_EINVAL;
+    const long NV = m->N * m->nv;
+    const BoundFlags f = make_bound_flags(m);
+    if (prec == CSSX_PREC_F64)
+        hipLaunchKernelGGL(k_bound<double>, dim3(nblocks(NV, 256)), dim3(256), 0, (hipStream_t)stream,
+                           make_consts<double>(m), f, (const double *)m->p64.sr1, (double *)state, m->nv);
+    else if (prec == CSSX_PREC_F32)
+        hipLaunchKernelGGL(k_bound<float>, dim3(nblocks(NV, 256)), dim3(256), 0, (hipStream_t)stream,
+                           make_consts<float>(m), f, (const float *)m->p32.sr1, (float *)state, m->nv);
+    else
+        return CSSX_EINVAL;
+    return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+extern "C" int cssx_rk4(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream) {
+    if (!m || !y0 || nsteps < 0 || !(dt > 0.0)) return CSSX_EINVAL;
+    if (prec == CSSX_PREC_F64) return rk4_impl<double>(m, (double *)y0, dt, nsteps, (hipStream_t)stream);
+    if (prec == CSSX_PREC_F32) return rk4_impl<float>(m, (double *)y0, dt, nsteps, (hipStream_t)stream);
+    return CSSX_EINVAL;
+}
+
+extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream) {
+    if (!m || !y0_f64 || !dt_out || !(cfl_safety > 0.0)) return CSSX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    int rc = ensure_scratch<double>(m);
+    if (rc) return rc;
+    const double *y = (const double *)y0_f64;
+    const long N = m->N;
+    if (m->p.magnetic && (rc = curl_impl<double>(m, y + 5 * N, m->s64.B, st))) return rc;
+    const unsigned nb = nblocks(N, 256);
+    if (m->nred < (int)nb) {
+        if (m->red) (void)hipFree(m->red);
+        m->red = nullptr;
+        if (hipMalloc(&m->red, sizeof(double) * 2 * nb) != hipSuccess) return CSSX_ENOMEM;
+        m->nred = (int)nb;
+    }
+    double *dr = nullptr;
+    if (hipMalloc(&dr, sizeof(double) * m->nr) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMemcpyAsync(dr, m->h_dr.data(), sizeof(double) * m->nr, hipMemcpyHostToDevice, st) != hipSuccess) {
+        (void)hipFree(dr);
+        return CSSX_EHIP;
+    }
+    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, make_consts<double>(m), m->p64, (const double *)dr, m->dth,
+                       m->dphi, m->p.Cp, y, (const double *)m->s64.B, m->red);
+    std::vector<double> h(2 * (size_t)nb);
+    hipError_t e = hipMemcpyAsync(h.data(), m->red, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipFree(dr);
+    if (e != hipSuccess) return CSSX_EHIP;
+    double dtc = 1e4, dtb = 1e4;
+    for (unsigned b = 0; b < nb; b++) {
+        dtc = std::min(dtc, h[2 * b]);
+        dtb = std::min(dtb, h[2 * b + 1]);
+    }
+    const double dtd = 1e10;   // no explicit diffusion limit without Laplacian (rhs_mpi.f90:187-191)
+    *dt_out = std::min(std::min(dtb, dtc), dtd) * cfl_safety;
+    return std::isfinite(*dt_out) ? CSSX_OK : CSSX_ENAN;
+}

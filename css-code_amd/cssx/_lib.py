@@ -50,6 +50,12 @@ SIGNATURES = {
                    ctypes.c_int),
     "cssx_compact_fd6": ([ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                           _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cssx_mhd_create": ([_vp, _vp, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
+    "cssx_mhd_destroy": ([_vp], None),
+    "cssx_rhs": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cssx_bound": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "cssx_rk4": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
+    "cssx_cfl": ([_vp, _vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _vp], ctypes.c_int),
     "cssx_plan_tables": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
 }

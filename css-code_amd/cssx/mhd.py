@@ -1,0 +1,97 @@
+"""MHD right-hand side and RK4 stepping on the GPU: the counterpart of Module Physics
+(rhs_mpi.f90: RHS, Bound, Compute_Step_Size) and Module Time_Step_RK4 (step_rk4.f90:
+Step_RK4) for one subdomain.
+
+States are torch CUDA tensors of shape (nv, nr, nphi, nth) in C order (the reference's
+vars(nth, nphi, nr, nv)), variables rho, w=u_theta, v=u_phi, u=u_r, s, A_theta, A_phi, A_r.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .model import Config, Model
+
+
+def _stream():
+    return ctypes.c_void_p(int(torch.cuda.current_stream().cuda_stream))
+
+
+class Solver:
+    """GPU solver for one subdomain (walls at both r and theta ends, periodic phi)."""
+
+    def __init__(self, model: Model | Config):
+        if isinstance(model, Config):
+            model = Model(model)
+        if not torch.cuda.is_available():
+            raise RuntimeError("cssx.mhd needs a ROCm GPU")
+        self.model = model
+        c = model.c
+        self.shape = (c.nv, c.nr, c.nphi, c.nth)
+        self._g = model.grid_struct()
+        self._p = model.params_struct()
+        self._pr = model.profiles_struct()
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().cssx_mhd_create(ctypes.addressof(self._g), ctypes.addressof(self._p),
+                                              ctypes.addressof(self._pr), ctypes.byref(h)), "cssx_mhd_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().cssx_mhd_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check_state(self, x, dtype=None):
+        if tuple(x.shape) != self.shape or not x.is_cuda or not x.is_contiguous():
+            raise ValueError(f"state must be a contiguous CUDA tensor of shape {self.shape}")
+        if dtype is not None and x.dtype != dtype:
+            raise TypeError(f"state dtype must be {dtype}")
+
+    @staticmethod
+    def _prec(x):
+        if x.dtype == torch.float64:
+            return _lib.PREC_F64
+        if x.dtype == torch.float32:
+            return _lib.PREC_F32
+        raise TypeError("float32 or float64 state expected")
+
+    def rhs(self, state, out=None, want_b=False):
+        """RHS(call_num): d(state)/dt; with want_b also B = curl A (Compute_B)."""
+        self._check_state(state)
+        if out is None:
+            out = torch.empty_like(state)
+        b = torch.empty((3,) + self.shape[1:], dtype=state.dtype, device=state.device) if want_b else None
+        rc = _lib.lib().cssx_rhs(self._h, self._prec(state), state.data_ptr(), out.data_ptr(),
+                                 b.data_ptr() if b is not None else None, _stream())
+        _lib.check(rc, "cssx_rhs")
+        return (out, b) if want_b else out
+
+    def bound(self, state):
+        """Bound(): wall conditions applied in place."""
+        self._check_state(state)
+        _lib.check(_lib.lib().cssx_bound(self._h, self._prec(state), state.data_ptr(), _stream()), "cssx_bound")
+        return state
+
+    def rk4(self, y0, dt, nsteps=1, precision="f64"):
+        """nsteps of Step_RK4 on the FP64 state y0 (in place).  precision='f32' runs the stages,
+        RHS and diffusion in FP32 with FP64 accumulation."""
+        self._check_state(y0, torch.float64)
+        prec = _lib.PREC_F64 if precision == "f64" else _lib.PREC_F32
+        _lib.check(_lib.lib().cssx_rk4(self._h, prec, y0.data_ptr(), float(dt), int(nsteps), _stream()), "cssx_rk4")
+        return y0
+
+    def cfl(self, y0, cfl_safety=None):
+        """Compute_Step_Size: the CFL time step (synchronizes)."""
+        self._check_state(y0, torch.float64)
+        dt = ctypes.c_double()
+        cs = self.model.c.cfl_safety if cfl_safety is None else cfl_safety
+        _lib.check(_lib.lib().cssx_cfl(self._h, y0.data_ptr(), float(cs), ctypes.byref(dt), _stream()), "cssx_cfl")
+        return dt.value

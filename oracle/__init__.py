@@ -44,6 +44,10 @@ def lib():
                                          ctypes.c_int, _dp, _dp]
         L.orc_deriv_field.argtypes = [ctypes.c_int] * 5 + [_dp, _dp] + [ctypes.c_int] * 3 + [_dp, _dp] + \
             [ctypes.c_double, ctypes.c_double, _dp]
+        _vp = ctypes.c_void_p
+        L.orc_rhs.argtypes = [_vp, _vp, _vp, _dp, _dp, _dp]
+        L.orc_bound.argtypes = [_vp, _vp, _vp, _dp]
+        L.orc_rk4.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_int]
     return _lib
 
 
@@ -125,3 +129,42 @@ def deriv_field(axis, order, f, lo_wall=True, hi_wall=True, bdry=0, ghost_lo=Non
     if rc:
         raise ValueError(f"orc_deriv_field rc={rc}")
     return out
+
+
+def _structs(model):
+    g = model.grid_struct()
+    p = model.params_struct()
+    pr = model.profiles_struct()
+    return g, p, pr
+
+
+def rhs(model, state, want_b=False):
+    """RHS (rhs_mpi.f90:279-783) of a state (nv, nr, nphi, nth); returns dvars (and B)."""
+    g, p, pr = _structs(model)
+    st = _f64(state)
+    dv = np.empty_like(st)
+    B = np.empty((3,) + st.shape[1:]) if want_b else None
+    rc = lib().orc_rhs(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st), _ptr(dv),
+                       _ptr(B))
+    if rc:
+        raise ValueError(f"orc_rhs rc={rc}")
+    return (dv, B) if want_b else dv
+
+
+def bound(model, state):
+    """Bound (rhs_mpi.f90:2395-2491) on a copy of the state."""
+    g, p, pr = _structs(model)
+    st = _f64(state).copy()
+    lib().orc_bound(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st))
+    return st
+
+
+def rk4(model, state, dt, nsteps):
+    """nsteps of Step_RK4 (step_rk4.f90:339-493) on a copy of the state."""
+    g, p, pr = _structs(model)
+    st = _f64(state).copy()
+    rc = lib().orc_rk4(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st), float(dt),
+                       int(nsteps))
+    if rc:
+        raise ValueError(f"orc_rk4 rc={rc}")
+    return st

@@ -1,0 +1,141 @@
+"""GPU parity of the MHD right-hand side, Bound, RK4 and CFL against the CPU oracle.
+
+Small shapes (the oracle finishes in seconds) with deliberately uneven sizes; the full
+benchmark shape is covered by size-independent properties (finite state, wall invariants,
+FP32-vs-FP64 agreement).  Tolerances: FP64 1e-11 of each field's max norm; FP32 2e-4.
+"""
+import numpy as np
+import pytest
+import torch
+
+import mms
+import oracle
+from cssx.model import Config, Model
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mhd():
+    from cssx.mhd import Solver
+    return Solver
+
+
+def _rel_fields(a, b):
+    out = []
+    for iv in range(a.shape[0]):
+        sc = max(np.abs(b[iv]).max(), 1e-300)
+        out.append(np.abs(a[iv] - b[iv]).max() / sc)
+    return np.array(out)
+
+
+SHAPES = [(24, 40, 32), (40, 36, 72)]      # (nr, nth, nphi)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_rhs_matches_oracle(mhd, shape):
+    nr, nth, nphi = shape
+    c = Config(nr=nr, nth=nth, nphi=nphi, omega0=0.3, damping_time=50.0)
+    m = Model(c)
+    st, ex, terms, Bex = mms.evaluate(m)
+    ref, Bref = oracle.rhs(m, st, want_b=True)
+    sol = mhd(m)
+    x = torch.as_tensor(st, device="cuda")
+    got, B = sol.rhs(x, want_b=True)
+    e = _rel_fields(got.cpu().numpy(), ref)
+    assert e.max() < 1e-11, e
+    eb = _rel_fields(B.cpu().numpy(), Bref)
+    assert eb.max() < 1e-11, eb
+    got32 = sol.rhs(x.float()).double().cpu().numpy()
+    e32 = _rel_fields(got32, ref)
+    assert e32.max() < 2e-4, e32
+    sol.close()
+
+
+def test_rhs_hydro_and_rest_state(mhd):
+    c = Config(nr=32, nth=24, nphi=16, magnetic=False, damping_time=1e9)
+    m = Model(c)
+    st = np.zeros((5, c.nr, c.nphi, c.nth))
+    st[0] = m.rho_bar[:, None, None]
+    st[4] = m.s_bar[:, None, None] * (1 + 1e-3 * np.random.default_rng(3).standard_normal(st[4].shape))
+    ref = oracle.rhs(m, st)
+    sol = mhd(m)
+    got = sol.rhs(torch.as_tensor(st, device="cuda")).cpu().numpy()
+    assert _rel_fields(got, ref).max() < 1e-11
+    sol.close()
+
+
+def test_bound_matches_oracle(mhd):
+    c = Config(nr=16, nth=12, nphi=8)
+    m = Model(c)
+    st = np.random.default_rng(5).standard_normal((8, 16, 8, 12))
+    ref = oracle.bound(m, st)
+    sol = mhd(m)
+    x = torch.as_tensor(st, device="cuda")
+    sol.bound(x)
+    assert np.array_equal(x.cpu().numpy(), ref)
+    sol.close()
+
+
+@pytest.mark.parametrize("shape", SHAPES[:1])
+def test_rk4_matches_oracle(mhd, shape):
+    nr, nth, nphi = shape
+    c = Config(nr=nr, nth=nth, nphi=nphi, omega0=0.3, damping_time=50.0)
+    m = Model(c)
+    st, ex, terms, Bex = mms.evaluate(m, amp=0.2)
+    st = oracle.bound(m, st)
+    dt, n = 2e-4, 3
+    ref = oracle.rk4(m, st, dt, n)
+    sol = mhd(m)
+    y = torch.as_tensor(st, device="cuda").contiguous()
+    sol.rk4(y, dt, n, "f64")
+    d0 = np.abs(ref - st).max()
+    e = _rel_fields(y.cpu().numpy(), ref)
+    assert e.max() < 1e-12, e
+    # the step really moved the state: compare the change, not only the state
+    chg_gpu = y.cpu().numpy() - st
+    assert np.abs(chg_gpu - (ref - st)).max() < 1e-9 * d0
+    y32 = torch.as_tensor(st, device="cuda").contiguous()
+    sol.rk4(y32, dt, n, "f32")
+    chg32 = y32.cpu().numpy() - st
+    assert np.abs(chg32 - (ref - st)).max() < 1e-3 * d0
+    sol.close()
+
+
+def test_cfl_step(mhd):
+    c = Config(nr=24, nth=20, nphi=16)
+    m = Model(c)
+    st = m.initial_state()
+    sol = mhd(m)
+    dt = sol.cfl(torch.as_tensor(st, device="cuda"))
+    # sound crossing time of the smallest cell bounds the step (rhs_mpi.f90:145-185)
+    _, B = oracle.rhs(m, st, want_b=True)
+    T = m.invtconst * st[0] ** m.gamm1 * np.exp(st[4] / m.Cv)
+    dxmin = np.minimum(m.dr[:, None, None], m.r[:, None, None] *
+                       np.minimum(m.dth, np.sin(m.theta)[None, None, :] * m.dphi))
+    ut = st[1] ** 2 + st[2] ** 2 + st[3] ** 2 + m.gamm1 * c.Cp * T
+    vb = dxmin / np.sqrt(ut + (B ** 2).sum(0) / (4 * np.pi) / st[0])
+    ref = min(min((dxmin / np.sqrt(ut)).min(), vb.min()), 1e4) * c.cfl_safety
+    assert abs(dt - ref) < 1e-12 * ref
+    sol.close()
+
+
+def test_c2_shape_step_finite_and_fp32_close(mhd):
+    """Benchmark shape 128x256x512: two RK4 steps in FP64 and FP32 from the same state."""
+    c = Config(**{"nr": 128, "nth": 256, "nphi": 512})
+    m = Model(c)
+    st = torch.as_tensor(m.initial_state(), device="cuda")
+    sol = mhd(m)
+    dt = sol.cfl(st)
+    y64 = st.clone()
+    sol.rk4(y64, dt, 2, "f64")
+    y32 = st.clone()
+    sol.rk4(y32, dt, 2, "f32")
+    torch.cuda.synchronize()
+    assert torch.isfinite(y64).all()
+    for iv in range(8):
+        ch = (y64[iv] - st[iv]).abs().max().item()
+        err = (y32[iv] - y64[iv]).abs().max().item()
+        sc = y64[iv].abs().max().item()
+        assert err <= 1e-5 * sc + 1e-3 * ch, (iv, err, sc, ch)
+    sol.close()
