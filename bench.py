@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: RK4 steps of the 8-field compressible MHD system on one MI355X.
+
+Workload: one 128 x 256 x 512 (r x theta x phi) subdomain, isentropic polytrope (n=1.5,
+density contrast 20), seeded entropy perturbation (1e-3) and random vector potential (1e-4),
+u = 0, dt from the CFL condition at 0.4, stress-free impenetrable radial walls, periodic phi.
+A "step" is one full Step_RK4 (four RHS evaluations + stage updates + Bound) on the whole
+subdomain with the state resident in HBM.
+
+`value` is the FP32-bulk path (FP32 stages/RHS, FP64 accumulation); the FP64 path, the
+FP32-vs-FP64 relative error, per-phase kernel times, the roofline of the dominant kernel and
+a CPU baseline (the FP64 oracle on a bounded sample) are reported beside it.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "css-code_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "RK4 Mpt-steps/s + HBM TB/s, FP32 vs FP64 at ≤1e-5 rel err; closure speedup"
+HBM_PEAK_GBS = 8000.0
+# compulsory stage traffic of one RK4 step, bytes per point (8 fields): FP64 17 field
+# accesses x 8 B; mixed (FP32 stage state, FP64 y0/accumulator) 8 x (20 + 32 + 32 + 28)
+STEP_BYTES_F64 = 1088.0
+STEP_BYTES_MIXED = 896.0
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def run_path(sol, y0, dt, steps, warmup, precision):
+    import torch
+    y = y0.clone()
+    if warmup:
+        w = y0.clone()
+        sol.rk4(w, dt, warmup, precision)
+        del w
+    _sync()
+    t0 = time.perf_counter()
+    sol.rk4(y, dt, steps, precision)
+    _sync()
+    t1 = time.perf_counter()
+    return y, (t1 - t0) * 1e3 / max(steps, 1)
+
+
+def profile_phases(sol, y0, dt, steps, precision):
+    from cssx import _lib
+    y = y0.clone()
+    _lib.timing_read()
+    _lib.timing_enable(True)
+    sol.rk4(y, dt, steps, precision)
+    tab = _lib.timing_read()
+    _lib.timing_enable(False)
+    return tab
+
+
+def cpu_baseline(seconds_budget=20.0):
+    """The FP64 oracle RK4 step on a bounded sample of the same recipe (OpenMP threads)."""
+    import numpy as np
+
+    import oracle
+    from cssx.model import Config, Model
+    c = Config(nr=32, nth=64, nphi=128)
+    m = Model(c)
+    st = m.initial_state()
+    g, p, pr = oracle._structs(m)
+    t0 = time.perf_counter()
+    steps = 0
+    y = st.copy()
+    while True:
+        y = oracle.rk4(m, y, 1e-4, 1)
+        steps += 1
+        if time.perf_counter() - t0 > seconds_budget or steps >= 50:
+            break
+    el = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": c.npts * steps / el / 1e6, "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (oracle/*.c) Step_RK4 on a {c.nr}x{c.nth}x{c.nphi} subdomain of the same "
+                      f"recipe, {steps} step(s) in {el:.1f} s, OpenMP threads={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nr", type=int, default=128)
+    ap.add_argument("--nth", type=int, default=256)
+    ap.add_argument("--nphi", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "decomposition is a design, not a code path", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from cssx.mhd import Solver
+    from cssx.model import Config, Model
+
+    torch.cuda.set_device(0)
+    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi)
+    m = Model(c)
+    y0 = torch.as_tensor(m.initial_state(), device="cuda").contiguous()
+    sol = Solver(m)
+    dt = sol.cfl(y0)
+    npts = c.npts
+
+    y64, ms64 = run_path(sol, y0, dt, args.steps, args.warmup, "f64")
+    y32, ms32 = run_path(sol, y0, dt, args.steps, args.warmup, "f32")
+
+    # FP32-bulk vs FP64 after the same K steps from the same state
+    rel_state, rel_change = [], []
+    for iv in range(c.nv):
+        d = (y32[iv] - y64[iv]).abs().max().item()
+        rel_state.append(d / max(y64[iv].abs().max().item(), 1e-300))
+        ch = (y64[iv] - y0[iv]).abs().max().item()
+        rel_change.append(d / max(ch, 1e-300))
+    finite = bool(torch.isfinite(y64).all().item() and torch.isfinite(y32).all().item())
+
+    prof_steps = max(2, min(args.steps, 5))
+    tab32 = profile_phases(sol, y0, dt, prof_steps, "f32")
+    tab64 = profile_phases(sol, y0, dt, prof_steps, "f64")
+
+    def roofline(tab):
+        # dominant kernel = the phase with the largest summed time; k_deriv covers the three axes
+        dphases = ("deriv_r", "deriv_theta", "deriv_phi")
+        dom = max(tab, key=lambda k: tab[k][0])
+        ms_d = sum(tab[k][0] for k in dphases)
+        n_d = sum(tab[k][1] for k in dphases)
+        b_d = sum(tab[k][2] for k in dphases)
+        name = "k_deriv" if dom in dphases or ms_d >= tab[dom][0] else dom
+        if name == "k_deriv":
+            ms, n, by = ms_d, n_d, b_d
+        else:
+            ms, n, by = tab[dom]
+        avg_ms = ms / max(n, 1)
+        achieved = (by / max(n, 1)) / (avg_ms * 1e-3) / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 5), "launches": n,
+                "algorithmic_bytes_per_launch": by / max(n, 1)}
+
+    val32 = npts / (ms32 * 1e-3) / 1e6
+    val64 = npts / (ms64 * 1e-3) / 1e6
+    out = {
+        "metric": METRIC,
+        "value": round(val32, 2),
+        "unit": "Mpt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field", "nr": c.nr, "nth": c.nth,
+                   "nphi": c.nphi, "fields": c.nv, "accumulate": "f64", "dt": dt, "parallelism": "single-gpu"},
+        "fp64": {"value": round(val64, 2), "unit": "Mpt-steps/s", "ms_per_step": round(ms64, 4)},
+        "fp32_speedup_vs_fp64": round(ms64 / ms32, 3),
+        "fp32_vs_fp64": {"steps": args.steps, "max_rel_err_state": max(rel_state),
+                         "max_rel_err_of_change": max(rel_change), "finite": finite},
+        "hbm_tbs_algorithmic": {"f32": round(val32 * 1e6 * STEP_BYTES_MIXED / 1e12, 3),
+                                "f64": round(val64 * 1e6 * STEP_BYTES_F64 / 1e12, 3)},
+        "phases_ms_per_step": {"f32": {k: round(v[0] / prof_steps, 4) for k, v in tab32.items()},
+                               "f64": {k: round(v[0] / prof_steps, 4) for k, v in tab64.items()}},
+        "roofline": roofline(tab32),
+        "roofline_f64": roofline(tab64),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    sol.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -59,6 +59,8 @@ int get_plan(const PlanKey &key, int prec, const void **out) {
 // Pencil geometry of an axis on the reference layout f(nth, nphi, nr).
 void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g) {
     const long sth = 1, sph = nth, sr = (long)nth * nphi;
+    g.phase = axis == CSSX_AXIS_R ? CSSX_PHASE_DERIV_R : axis == CSSX_AXIS_THETA ? CSSX_PHASE_DERIV_THETA
+                                                                                : CSSX_PHASE_DERIV_PHI;
     if (axis == CSSX_AXIS_R) {          // pencils (it, ip), lanes over theta
         g.n = nr; g.na = nth; g.nb = nphi; g.sa = sth; g.sb = sph; g.srow = sr;
         g.g_k = (long)nth * nphi; g.g_a = 1; g.g_b = nth;
@@ -100,7 +102,10 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
         J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
         J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind; J.lo_r0_val = s.lo_r0_val;
         if (L.njobs == kMaxJobs || i == nspec - 1) {
+            const int phase = g.phase;
+            const int tok = timing_begin(phase, stream);
             hipError_t e = launch_deriv<T>(L, s.order, stream);
+            timing_end(tok, stream, (double)L.njobs * g.n * (double)g.na * g.nb * 2.0 * sizeof(T));
             if (e != hipSuccess) return CSSX_EHIP;
             L.njobs = 0;
         }
@@ -226,6 +231,7 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     // (5 is odd, :154-160) but its R0 is the preloaded darr[0] (no formula fills it, :134-135).
     const int bd = (ibc == 5) ? 3 : ibc;
     AxisGeom g;
+    g.phase = CSSX_PHASE_OTHER;
     g.n = n; g.na = (int)npencils; g.nb = 1; g.sa = 1; g.sb = 0; g.srow = npencils;
     g.g_k = npencils; g.g_a = 1; g.g_b = 0;
     DerivSpec s;

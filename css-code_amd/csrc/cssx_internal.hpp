@@ -11,6 +11,7 @@ struct AxisGeom {
     int n = 0, na = 0, nb = 0;
     long sa = 0, sb = 0, srow = 0;
     long g_k = 0, g_a = 0, g_b = 0;   // default ghost strides (reference halo layout)
+    int phase = 0;                    // timing bucket of launches along this axis
 };
 
 // One derivative request (precision-neutral; pointers are device pointers).
@@ -27,6 +28,10 @@ struct DerivSpec {
     int lo_kind = END_WALL, hi_kind = END_WALL;
     int lo_r0_val = 0;
 };
+
+// per-phase event timing (cssx_timing.hip); token < 0 = disabled
+int timing_begin(int phase, hipStream_t stream);
+void timing_end(int token, hipStream_t stream, double bytes);
 
 void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g);
 int get_plan(const PlanKey &key, int prec, const void **out);

@@ -25,6 +25,14 @@
 #include "cssx_kernels.hpp"
 #include "cssx_plan.hpp"
 
+// bracket one launch with the phase timer (no-op unless cssx_timing_enable(1))
+#define CSSX_TIMED(phase, stream, bytes, ...)            \
+    do {                                                 \
+        const int tok_ = timing_begin((phase), (stream)); \
+        __VA_ARGS__;                                     \
+        timing_end(tok_, (stream), (double)(bytes));     \
+    } while (0)
+
 namespace cssx {
 
 constexpr double kOneOver4Pi = 1.0 / 3.1415926535897932384626433832795028841972 / 4.0;
@@ -679,9 +687,10 @@ static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream) {
     DerivSpec rr[2] = {spec<T>(m, CSSX_AXIS_R, at, D + 4 * N, ibcr[10], nullptr, nullptr, P.r, P.dxdr),
                        spec<T>(m, CSSX_AXIS_R, ap, D + 5 * N, ibcr[11], nullptr, nullptr, P.r, P.dxdr)};
     if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, 2, stream))) return rc;
-    hipLaunchKernelGGL(k_curl_combine<T>, dim3(nb), dim3(256), 0, stream, c, P, D, D + N, D + 2 * N, D + 3 * N,
-                       D + 4 * N, D + 5 * N, B, m->p.theta_perf_cond, m->p.bottom_radial_field,
-                       m->p.top_radial_field);
+    CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, 9.0 * N * sizeof(T),
+               hipLaunchKernelGGL(k_curl_combine<T>, dim3(nb), dim3(256), 0, stream, c, P, D, D + N, D + 2 * N,
+                                  D + 3 * N, D + 4 * N, D + 5 * N, B, m->p.theta_perf_cond,
+                                  m->p.bottom_radial_field, m->p.top_radial_field));
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
 
@@ -737,7 +746,8 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, mag ? 6 : 4, stream))) return rc;
     }
     // 4. pointwise momentum / entropy / continuity terms
-    hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv);
+    CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, (m->p.magnetic ? 32.0 : 24.0) * N * sizeof(T),
+               hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv));
     // 5. density wall derivatives from the momentum tendency
     {
         long nw = std::max((long)m->nr * m->nphi, (long)m->nth * m->nphi);
@@ -751,7 +761,9 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         DerivSpec rr = spec<T>(m, CSSX_AXIS_R, rho, D + 1 * N, ibcr[1], wall + W.off_rrho_lo, wall + W.off_rrho_hi,
                                nullptr, P.dxdr);
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, &rr, 1, stream))) return rc;
-        hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N, D + 1 * N, dv);
+        CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T),
+                   hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N,
+                                      D + 1 * N, dv));
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
@@ -768,17 +780,26 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
     const unsigned nb = nblocks(NV, 256);
     if (m->p.sld_on && m->p.sld_coef != 0.0) return CSSX_EINVAL;   // SLD fluxes: not in this build yet
     const bool f32 = sizeof(T) != sizeof(double);
+    // algorithmic bytes of one stage update: read k (+ save after stage 1) and y0, write save
+    // (stages 1-3) and the new stage state (y1, or y0 and its FP32 copy at stage 4)
+    auto rk_bytes = [&](int stage) {
+        double b = sizeof(T) + 8.0 + (stage > 1 ? 8.0 : 0.0) + (stage < 4 ? 8.0 + sizeof(T) : 8.0 + (f32 ? 4.0 : 0.0));
+        return b * (double)NV;
+    };
     if (f32) hipLaunchKernelGGL(k_cast_state<T>, dim3(nb), dim3(256), 0, stream, NV, (const double *)y0, S.y1);
     for (int step = 0; step < nsteps; step++) {
         // stage 1 reads vars1 = vars0: the FP64 run reads y0 directly, the FP32 run its cast
         const T *in1 = f32 ? S.y1 : (const T *)y0;
         if ((rc = rhs_impl<T>(m, in1, S.k, nullptr, stream))) return rc;
-        hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1, 1, dt, y0, m->save,
-                           (const T *)S.k, (const double *)nullptr, S.y1, m->nv);
+        CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(1),
+                   hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1, 1, dt,
+                                      y0, m->save, (const T *)S.k, (const double *)nullptr, S.y1, m->nv));
         for (int stage = 2; stage <= 4; stage++) {
             if ((rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
-            hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1, stage, dt, y0,
-                               m->save, (const T *)S.k, (const double *)nullptr, S.y1, m->nv);
+            CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
+                       hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1,
+                                          stage, dt, y0, m->save, (const T *)S.k, (const double *)nullptr, S.y1,
+                                          m->nv));
         }
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;

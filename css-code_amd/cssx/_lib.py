@@ -19,6 +19,7 @@ CSSX_EHIP = 100
 PREC_F64 = 0
 PREC_F32 = 1
 AXIS_R, AXIS_THETA, AXIS_PHI = 0, 1, 2
+PHASES = ("deriv_r", "deriv_theta", "deriv_phi", "pointwise", "rk_update", "other")
 
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -56,6 +57,9 @@ SIGNATURES = {
     "cssx_bound": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "cssx_rk4": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
     "cssx_cfl": ([_vp, _vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _vp], ctypes.c_int),
+    "cssx_timing_enable": ([ctypes.c_int], ctypes.c_int),
+    "cssx_timing_read": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
+                          ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cssx_plan_tables": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
 }
@@ -87,3 +91,18 @@ def strerror(code: int) -> str:
 def check(code: int, where: str) -> None:
     if code != CSSX_OK:
         raise CssxError(code, where)
+
+
+def timing_enable(on: bool = True) -> None:
+    """Bracket every library kernel launch with hipEvents (cssx_timing_enable)."""
+    check(lib().cssx_timing_enable(1 if on else 0), "cssx_timing_enable")
+
+
+def timing_read() -> dict:
+    """{phase: (ms, launches, algorithmic bytes)} since the last read; synchronizes."""
+    n = len(PHASES)
+    ms = (ctypes.c_double * n)()
+    cnt = (ctypes.c_long * n)()
+    by = (ctypes.c_double * n)()
+    check(lib().cssx_timing_read(ms, cnt, by), "cssx_timing_read")
+    return {PHASES[i]: (ms[i], cnt[i], by[i]) for i in range(n)}

@@ -152,6 +152,23 @@ int cssx_rk4(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *strea
  * flow and Alfven crossing times.  Synchronizes the stream; *dt_out on the host. */
 int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream);
 
+/* Per-phase GPU timing, the counterpart of the reference's Do_Timings buckets (constants.f90:82,
+ * timings(1:16) around each phase of rhs_mpi.f90:324-766 and step_rk4.f90:343-491).  While
+ * enabled, every kernel launch of the library is bracketed by hipEvents on its stream.
+ * cssx_timing_read synchronizes the device and returns, per phase, the summed kernel time
+ * (ms), the number of launches and the algorithmic bytes those launches move (one read of
+ * every input value and one write of every output value), then resets the table.  Any of the
+ * three arrays may be NULL; each has CSSX_NPHASE entries. */
+#define CSSX_PHASE_DERIV_R 0
+#define CSSX_PHASE_DERIV_THETA 1
+#define CSSX_PHASE_DERIV_PHI 2
+#define CSSX_PHASE_POINTWISE 3     /* curl assembly, wall values, RHS terms */
+#define CSSX_PHASE_RK_UPDATE 4     /* stage update + Bound */
+#define CSSX_PHASE_OTHER 5
+#define CSSX_NPHASE 6
+int cssx_timing_enable(int on);
+int cssx_timing_read(double *ms, long *count, double *bytes);
+
 /* The precomputed solve tables for one pencil system, in FP64 on the host (no device work).
  * The counterpart of the module-level Thomas tables gam1/gam2/gam3 that Initialize_Derivatives
  * builds (derivatives.f90:106-518); here they are the segment-local LU factors, interface
