@@ -32,11 +32,16 @@ int get_plan(const PlanKey &key, int prec, const void **out) {
         *out = it->second;
         return CSSX_OK;
     }
-    const int M = choose_M(key.n);
-    if (M < 0) return CSSX_EINVAL;
-    const int S = seg_count(key.n, M);
     std::vector<double> plan;
-    build_plan(key, S, plan);
+    if (key.variant == 1) {
+        const int MR = choose_MR(key.n);
+        if (MR < 0) return CSSX_EINVAL;
+        build_plan_contig(key, MR, plan);
+    } else {
+        const int M = choose_M(key.n);
+        if (M < 0) return CSSX_EINVAL;
+        build_plan(key, seg_count(key.n, M), plan);
+    }
     void *d = nullptr;
     size_t bytes = plan.size() * (prec == CSSX_PREC_F64 ? sizeof(double) : sizeof(float));
     if (hipMalloc(&d, bytes) != hipSuccess) return CSSX_ENOMEM;
@@ -73,6 +78,22 @@ void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g) {
     }
 }
 
+// The contiguous kernel serves pencils with unit row stride; its D2 upper-wall closure needs
+// five rows below row n-1 inside one lane's window, i.e. the last lane must own >= 2 rows.
+static bool use_contig(const AxisGeom &g, int order, int hi_kind) {
+    const int MR = choose_MR(g.n);
+    if (g.srow != 1 || MR < 0) return false;
+    if (order == 2 && hi_kind == END_WALL && g.n % MR == 1) return false;
+    return true;
+}
+
+static int lanes_per_pencil(int n) {
+    const int nl = (n + choose_MR(n) - 1) / choose_MR(n);
+    int Lw = 1;
+    while (Lw < nl) Lw *= 2;
+    return Lw;
+}
+
 template <typename T>
 static void fill_launch_geom(DerivLaunch<T> &L, const AxisGeom &g) {
     L.n = g.n;
@@ -85,10 +106,14 @@ template <typename T>
 static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStream_t stream) {
     DerivLaunch<T> L;
     fill_launch_geom(L, g);
+    bool contig = true;
+    for (int i = 0; i < nspec; i++) contig = contig && use_contig(g, (&sp)[i].order, (&sp)[i].hi_kind);
+    if (contig) L.S = lanes_per_pencil(g.n);
     for (int i = 0; i < nspec; i++) {
         const DerivSpec &s = (&sp)[i];
         PlanKey key{g.n, s.order, s.lo_kind == END_WALL ? END_WALL : END_HALO,
                     s.hi_kind == END_WALL ? END_WALL : END_HALO, (s.lo_kind == END_WALL || s.hi_kind == END_WALL) ? s.bdry : 0};
+        key.variant = contig ? 1 : 0;
         const void *plan = nullptr;
         int rc = get_plan(key, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
         if (rc) return rc;
@@ -104,7 +129,7 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
         if (L.njobs == kMaxJobs || i == nspec - 1) {
             const int phase = g.phase;
             const int tok = timing_begin(phase, stream);
-            hipError_t e = launch_deriv<T>(L, s.order, stream);
+            hipError_t e = contig ? launch_deriv_contig<T>(L, s.order, stream) : launch_deriv<T>(L, s.order, stream);
             timing_end(tok, stream, (double)L.njobs * g.n * (double)g.na * g.nb * 2.0 * sizeof(T));
             if (e != hipSuccess) return CSSX_EHIP;
             L.njobs = 0;
@@ -168,11 +193,15 @@ extern "C" int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out) {
                         if (lo == 1 && hi == 1 && bd > 0) continue;
                         PlanKey key{ns[ax], order, lo ? END_HALO : END_WALL, hi ? END_HALO : END_WALL, bd};
                         const void *p;
-                        for (int prec = 0; prec < 2; prec++) {
-                            int rc = get_plan(key, prec, &p);
-                            if (rc) {
-                                delete c;
-                                return rc;
+                        for (int var = 0; var < 2; var++) {
+                            if (var == 1 && (ax != CSSX_AXIS_THETA || choose_MR(ns[ax]) < 0)) continue;
+                            key.variant = var;
+                            for (int prec = 0; prec < 2; prec++) {
+                                int rc = get_plan(key, prec, &p);
+                                if (rc) {
+                                    delete c;
+                                    return rc;
+                                }
                             }
                         }
                     }
@@ -250,14 +279,23 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     return run_derivs(&s, 1, g, CSSX_PREC_F64, (hipStream_t)stream);
 }
 
-extern "C" int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, double *plan_out,
-                                int *nseg) {
-    if (!plan_out || n < 8 || choose_M(n) < 0 || (order != 1 && order != 2) || bdry < 0 || bdry > 4 ||
-        lo_kind < 0 || lo_kind > 1 || hi_kind < 0 || hi_kind > 1)
+extern "C" int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, int variant,
+                                double *plan_out, int *nseg) {
+    if (!plan_out || n < 8 || (order != 1 && order != 2) || bdry < 0 || bdry > 4 || lo_kind < 0 || lo_kind > 1 ||
+        hi_kind < 0 || hi_kind > 1 || variant < 0 || variant > 1)
         return CSSX_EINVAL;
-    const int S = seg_count(n, choose_M(n));
     std::vector<double> plan;
-    build_plan(PlanKey{n, order, lo_kind, hi_kind, bdry}, S, plan);
+    PlanKey key{n, order, lo_kind, hi_kind, bdry};
+    int S;
+    if (variant == 0) {
+        if (choose_M(n) < 0) return CSSX_EINVAL;
+        S = seg_count(n, choose_M(n));
+        build_plan(key, S, plan);
+    } else {
+        if (choose_MR(n) < 0) return CSSX_EINVAL;
+        S = (n + choose_MR(n) - 1) / choose_MR(n);
+        build_plan_contig(key, choose_MR(n), plan);
+    }
     for (size_t i = 0; i < plan.size(); i++) plan_out[i] = plan[i];
     if (nseg) *nseg = S;
     return CSSX_OK;

@@ -1,0 +1,246 @@
+// Compact derivative along a contiguous axis (theta, stride 1) for gfx950.
+//
+// Same operator as k_deriv (derivatives.f90 Dbydt :1010-1262, D2bydt2 :1265-1477), but the
+// lanes walk the pencil instead of across pencils: lane j of a pencil group owns rows
+// [j*MR, j*MR+MR), so a wavefront reads and writes whole pencils as contiguous runs (fully
+// coalesced).  The 4-row halos of a lane's stencil come from its neighbours by wavefront
+// shuffles, the nl-1 interface rows are solved by parallel cyclic reduction over shuffles with
+// precomputed multipliers, and no LDS traffic or barrier touches the data path (the plan
+// tables are staged in LDS once per workgroup).
+#include <hip/hip_runtime.h>
+
+#include "cssx_kernels.hpp"
+#include "cssx_plan.hpp"
+#include "cssx_rows.hpp"
+
+namespace cssx {
+
+template <typename T>
+__device__ __forceinline__ T shfl(T v, int src) {
+    return __shfl(v, src, 64);
+}
+
+template <typename T, int ORDER, int MR>
+__global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
+    extern __shared__ unsigned char smem_raw[];
+    T *sp = reinterpret_cast<T *>(smem_raw);
+    const DerivJob<T> &J = L.jobs[blockIdx.z];
+    const int n = L.n;
+    const int plen = plan_contig_len(n);
+    for (int i = threadIdx.x; i < plen; i += blockDim.x) sp[i] = J.plan[i];
+    __syncthreads();
+    const T *cf = sp, *cl = cf + n, *cu = cl + n, *vL = cu + n, *vR = vL + n;
+    const T *rl = sp + 5 * n, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
+            *invb = k2 + kPcrSteps * kMaxLanes;
+
+    constexpr int W = MR + 8;
+    const int Lw = L.S;                       // lanes per pencil group (power of two)
+    const int nl = (n + MR - 1) / MR;         // lanes holding rows
+    const int nI = nl - 1;                    // interface equations
+    const int lane = threadIdx.x & 63;
+    const int j = lane & (Lw - 1);
+    const int ppw = 64 / Lw;
+    const long a = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * ppw + lane / Lw;
+    const int b = blockIdx.y;
+    const bool pen_ok = a < L.na;
+    const bool active = pen_ok && j < nl;
+    const int st = j * MR;
+    const int m = active ? min(MR, n - st) : 0;
+    const long pbase = a * L.sa + (long)b * L.sb;
+    const long gbase = a * J.g_a + (long)b * J.g_b;
+
+    // ---- own rows, then the halo rows from the neighbouring lanes ----------------------------
+    const T *__restrict__ in = J.in;
+    const long pb_ld = (pen_ok ? a : 0) * L.sa + (long)b * L.sb;
+    T v[MR];
+#pragma unroll
+    for (int k = 0; k < MR; k++) {
+        const int q = min(st + k, n - 1);
+        v[k] = in[pb_ld + (long)q * L.srow];
+    }
+    if (J.pre) {
+#pragma unroll
+        for (int k = 0; k < MR; k++) v[k] *= J.pre[min(st + k, n - 1)];
+    }
+#pragma unroll
+    for (int k = 0; k < MR; k++)
+        if (k >= m) v[k] = T(0);
+    T w[W];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        w[k] = shfl(v[MR - 4 + k], lane - 1);      // rows st-4 .. st-1
+        w[MR + 4 + k] = shfl(v[k], lane + 1);       // rows st+MR .. st+MR+3
+    }
+#pragma unroll
+    for (int k = 0; k < MR; k++) w[4 + k] = v[k];
+    // rows outside the pencil: ghosts (halo end), or zero (wall end; not read by wall rows)
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const int q = st - 4 + i;
+        if (q < 0 || q >= n) {
+            T g = T(0);
+            if (pen_ok) {
+                if (q < 0 && q >= -4 && J.lo_kind == END_HALO) g = J.glo[gbase + (long)(q + 4) * J.g_k];
+                if (q >= n && q < n + 4 && J.hi_kind == END_HALO) g = J.ghi[gbase + (long)(q - n) * J.g_k];
+            }
+            w[i] = g;
+        }
+    }
+    T lo_val = T(0), hi_val = T(0);
+    if (active && j == 0 && J.lo_kind == END_WALL && J.glo) lo_val = J.glo[gbase];
+    if (active && j == nl - 1 && J.hi_kind == END_WALL && J.ghi) hi_val = J.ghi[gbase];
+
+    // ---- right-hand side rows ---------------------------------------------------------------
+    const int bd = J.bdry;
+    T d[MR];
+#pragma unroll
+    for (int jj = 0; jj < MR; jj++) {
+        T val;
+        if constexpr (ORDER == 1)
+            val = Rows<T, 1>::interior(w[jj + 2], w[jj + 3], w[jj + 5], w[jj + 6]);
+        else
+            val = Rows<T, 2>::interior(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6]);
+        const int row = st + jj;
+        if (row == 0) {
+            if (J.lo_kind == END_WALL) {
+                if constexpr (ORDER == 1) {
+                    T r1 = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
+                    T vv = (bd == 0) ? T(0) : lo_val * J.neu_mul;
+                    val = Rows<T, 1>::wall0(bd == 1 || bd == 3, vv, w[4], w[5], w[6], w[7], r1, J.lo_r0_val != 0);
+                } else {
+                    T vv = (bd == 0) ? T(0) : -lo_val * J.neu_mul;
+                    val = Rows<T, 2>::wall0(bd == 1 || bd == 3, vv, w[4], w[5], w[6], w[7], w[8], w[9]);
+                }
+            } else {
+                if constexpr (ORDER == 1)
+                    val = Rows<T, 1>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6], J.halo_s);
+                else
+                    val = Rows<T, 2>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+            }
+        } else if (row == 1 && J.lo_kind == END_WALL) {
+            if constexpr (ORDER == 1) val = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
+            else val = Rows<T, 2>::wall1(w[4], w[5], w[6], w[7], w[8]);
+        }
+        if (row == n - 1) {
+            if (J.hi_kind == END_WALL) {
+                if constexpr (ORDER == 1) {
+                    T rn2 = -Rows<T, 1>::wall1(w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj]);
+                    T vv = (bd == 3 || bd == 4) ? hi_val * J.neu_mul : T(0);
+                    T rn1 = Rows<T, 1>::wall0(bd == 1 || bd == 2, vv, w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], -rn2);
+                    val = (bd == 1 || bd == 2) ? -rn1 : rn1;
+                } else {
+                    T vv = (bd == 0) ? T(0) : hi_val * J.neu_mul;
+                    val = Rows<T, 2>::wall0(bd == 1 || bd == 2, vv, w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1],
+                                            w[jj], w[jj - 1 < 0 ? 0 : jj - 1]);
+                }
+            } else {
+                if constexpr (ORDER == 1)
+                    val = Rows<T, 1>::halo_hi0(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7],
+                                               w[jj + 8], J.halo_s);
+                else
+                    val = Rows<T, 2>::halo_hi0(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7],
+                                               w[jj + 8]);
+            }
+        } else if (row == n - 2 && J.hi_kind == END_WALL) {
+            // row n-2's window sits one row lower: f[n-1] = w[jj+5]
+            if constexpr (ORDER == 1) val = -Rows<T, 1>::wall1(w[jj + 5], w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1]);
+            else val = Rows<T, 2>::wall1(w[jj + 5], w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1]);
+        }
+        d[jj] = val;
+    }
+    if (J.rhs_out && active) {
+#pragma unroll
+        for (int jj = 0; jj < MR; jj++)
+            if (jj < m) J.rhs_out[pbase + (long)(st + jj) * L.srow] = d[jj] * J.rhs_scale;
+    }
+
+    // ---- segment-local solve --------------------------------------------------------------------
+    const bool has_iface = active && j < nl - 1;
+    const int mi = has_iface ? MR - 1 : m;
+    const T dI = d[MR - 1];
+    T y = T(0);
+#pragma unroll
+    for (int jj = 0; jj < MR; jj++) {
+        if (jj < mi) {
+            y = cf[st + jj] * d[jj] - cl[st + jj] * y;
+            d[jj] = y;
+        }
+    }
+    T xn = T(0);
+#pragma unroll
+    for (int jj = MR - 1; jj >= 0; jj--) {
+        if (jj < mi) {
+            xn = d[jj] - cu[st + jj] * xn;
+            d[jj] = xn;
+        }
+    }
+    const T xfirst = d[0];
+    const T xlast = d[MR - 2];   // last interior row of a full segment
+
+    // ---- interface system by parallel cyclic reduction -----------------------------------------
+    const T xf_next = shfl(xfirst, lane + 1);
+    T R = T(0);
+    if (has_iface) R = dI - rl[j] * xlast - ru[j] * xf_next;
+    int steps = 0;
+    while ((1 << steps) < nI) steps++;
+    for (int s = 0; s < steps; s++) {
+        const int dd = 1 << s;
+        const T Rm = shfl(R, lane - dd);
+        const T Rp = shfl(R, lane + dd);
+        if (has_iface) R = R - k1[s * kMaxLanes + j] * Rm - k2[s * kMaxLanes + j] * Rp;
+    }
+    const T xI = has_iface ? R * invb[j] : T(0);
+    T xl = shfl(xI, lane - 1);
+    if (j == 0) xl = T(0);
+    const T xr = xI;
+
+    // ---- correct and store ------------------------------------------------------------------------
+    if (!active) return;
+    T *__restrict__ out = J.out;
+    const T *__restrict__ post = J.post;
+    const T scale = J.scale, beta = J.beta;
+    T o[MR];
+#pragma unroll
+    for (int jj = 0; jj < MR; jj++) {
+        const int row = min(st + jj, n - 1);
+        T x = (jj < mi) ? d[jj] + vL[row] * xl + vR[row] * xr : xr;
+        o[jj] = scale * x;
+        if (post) o[jj] *= post[row];
+    }
+    if (beta != T(0)) {
+#pragma unroll
+        for (int jj = 0; jj < MR; jj++)
+            if (jj < m) o[jj] += beta * out[pbase + (long)(st + jj) * L.srow];
+    }
+#pragma unroll
+    for (int jj = 0; jj < MR; jj++)
+        if (jj < m) out[pbase + (long)(st + jj) * L.srow] = o[jj];
+}
+
+template <typename T>
+hipError_t launch_deriv_contig(const DerivLaunch<T> &L, int order, hipStream_t stream) {
+    const int MR = choose_MR(L.n);
+    if (MR < 0 || L.njobs < 1 || L.njobs > kMaxJobs) return hipErrorInvalidValue;
+    const int Lw = L.S;
+    const int ppw = 64 / Lw;
+    const int wpb = 4;
+    dim3 grid((unsigned)((L.na + (long)ppw * wpb - 1) / ((long)ppw * wpb)), L.nb, L.njobs), block(64 * wpb);
+    const size_t shm = (size_t)plan_contig_len(L.n) * sizeof(T);
+#define CSSX_LAUNCH_C(MM)                                                                                \
+    do {                                                                                                 \
+        if (order == 1) hipLaunchKernelGGL((k_deriv_contig<T, 1, MM>), grid, block, shm, stream, L);     \
+        else hipLaunchKernelGGL((k_deriv_contig<T, 2, MM>), grid, block, shm, stream, L);                \
+    } while (0)
+    switch (MR) {
+    case 4: CSSX_LAUNCH_C(4); break;
+    case 8: CSSX_LAUNCH_C(8); break;
+    default: CSSX_LAUNCH_C(16); break;
+    }
+#undef CSSX_LAUNCH_C
+    return hipGetLastError();
+}
+
+template hipError_t launch_deriv_contig<float>(const DerivLaunch<float> &, int, hipStream_t);
+template hipError_t launch_deriv_contig<double>(const DerivLaunch<double> &, int, hipStream_t);
+
+}  // namespace cssx

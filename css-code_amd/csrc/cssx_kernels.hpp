@@ -32,7 +32,7 @@ struct DerivJob {
 
 template <typename T>
 struct DerivLaunch {
-    int n, S;           // pencil length, segments per pencil
+    int n, S;           // pencil length, segments per pencil (contiguous kernel: lanes per pencil)
     int na, nb;         // pencils: fast index a (lanes) x slow index b (grid.y)
     long sa, sb, srow;  // element strides of a, b and along the pencil
     int njobs;
@@ -41,5 +41,7 @@ struct DerivLaunch {
 
 template <typename T>
 hipError_t launch_deriv(const DerivLaunch<T> &L, int order, hipStream_t stream);
+template <typename T>
+hipError_t launch_deriv_contig(const DerivLaunch<T> &L, int order, hipStream_t stream);
 
 }  // namespace cssx

@@ -23,7 +23,9 @@ enum EndKind : int { END_WALL = 0, END_HALO = 1, END_PERIODIC = 2 };
 
 struct PlanKey {
     int n, order, lo, hi, bdry;
+    int variant = 0;   // 0: strided (waves = segments), 1: contiguous (lanes = segments)
     bool operator<(const PlanKey &o) const {
+        if (variant != o.variant) return variant < o.variant;
         if (n != o.n) return n < o.n;
         if (order != o.order) return order < o.order;
         if (lo != o.lo) return lo < o.lo;
@@ -44,7 +46,7 @@ struct PlanKey {
 //   [5n + 3*kMaxSeg) rcl : reduced-system lower/pivot
 //   [5n + 4*kMaxSeg) rcu : reduced-system upper/pivot
 constexpr int kMaxSeg = 16;
-inline int plan_len(int n) { return 5 * n + 5 * kMaxSeg; }
+__host__ __device__ inline int plan_len(int n) { return 5 * n + 5 * kMaxSeg; }
 
 // Segment split used by both the plan and the kernels: S segments, the first n%S get one
 // extra row.
@@ -74,5 +76,26 @@ void lhs_coefficients(int order, int n, int lo, int hi, int bdry, std::vector<do
 
 // Fills a plan buffer (FP64) for the given key and segment count S.
 void build_plan(const PlanKey &key, int S, std::vector<double> &plan);
+
+// ---- contiguous-axis variant: lanes walk the pencil --------------------------------------------
+// Segment j of a pencil is rows [j*MR, min((j+1)*MR, n)), one lane each, nl = ceil(n/MR) <= 64
+// lanes.  The nl-1 interface rows form a tridiagonal system that the lanes solve together by
+// parallel cyclic reduction over wavefront shuffles; its multipliers are data-independent.
+// Layout (n rows, K = kMaxLanes):
+//   [0n..5n)       cf, cl, cu, vL, vR            (as in the strided plan)
+//   [5n)           rl[K], ru[K]                  interface row coefficients
+//   [5n+2K)        k1[kPcrSteps][K], k2[kPcrSteps][K]   PCR multipliers
+//   [5n+2K+2SK)    inv_b[K]                      1/diagonal after the last PCR step
+constexpr int kMaxLanes = 64;
+constexpr int kPcrSteps = 6;
+__host__ __device__ inline int plan_contig_len(int n) { return 5 * n + 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes; }
+// rows per lane for a pencil of n rows (4, 8 or 16), -1 if n > 1024
+inline int choose_MR(int n) {
+    if (n <= 64 * 4) return 4;
+    if (n <= 64 * 8) return 8;
+    if (n <= 64 * 16) return 16;
+    return -1;
+}
+void build_plan_contig(const PlanKey &key, int MR, std::vector<double> &plan);
 
 }  // namespace cssx

@@ -42,6 +42,9 @@ struct Prof {
     T *r = nullptr, *rinv = nullptr, *dxdr = nullptr, *grav = nullptr, *eps = nullptr;
     T *sn = nullptr, *csc = nullptr, *cs = nullptr, *cot = nullptr;
     T *dsdr1 = nullptr, *dsdr2 = nullptr, *sr1 = nullptr;
+    // hydrostatic background (nr): rho_bar, s_bar, T_bar and the physical radial derivatives
+    // of rho_bar and s_bar by the compact operator with zero wall values
+    T *rbar = nullptr, *sbar = nullptr, *tbar = nullptr, *drbar = nullptr, *dsbar = nullptr;
     T *buf = nullptr;
 };
 
@@ -94,6 +97,7 @@ struct Consts {
     long N;
     T gamm1, invtconst, invCv, pconst, invpconst, Cv, omega0, om2, xi2, gamma, T0, damp;
     int magnetic, top_relax, xi_on;
+    int fluct;   // state slots 1 and 5 hold rho - rho_bar(r) and s - s_bar(r) (FP32 path)
 };
 
 template <typename T>
@@ -105,6 +109,29 @@ template <>
 __device__ __forceinline__ float temperature<float>(float rho, float s, const Consts<float> &c) {
     float t = c.invtconst * powf(rho, c.gamm1);
     return t * expf(s * c.invCv);
+}
+
+// full density and temperature of point (ir) from the stored slots.  In the fluctuation form
+// T = T_bar (1 + rho'/rho_bar)^(gamma-1) exp(s'/Cv), which equals
+// invtconst rho^(gamma-1) exp(s/Cv) exactly but keeps FP32 relative precision.
+template <typename T>
+struct Thermo {
+    T rho, t;
+};
+template <typename T>
+__device__ __forceinline__ Thermo<T> thermo(T rho_st, T s_st, int ir, const Consts<T> &c, const Prof<T> &P) {
+    Thermo<T> o;
+    if (c.fluct) {
+        const T rb = P.rbar[ir];
+        o.rho = rb + rho_st;
+        T q = T(1) + rho_st / rb;
+        if constexpr (sizeof(T) == 4) o.t = P.tbar[ir] * powf(q, c.gamm1) * expf(s_st * c.invCv);
+        else o.t = P.tbar[ir] * pow(q, c.gamm1) * exp(s_st * c.invCv);
+    } else {
+        o.rho = rho_st;
+        o.t = temperature(rho_st, s_st, c);
+    }
+    return o;
 }
 
 template <typename T>
@@ -217,8 +244,9 @@ __global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const 
     const T ri = P.rinv[ir], rv = P.r[ir], epsr = P.eps[ir];
     const T omsq = rv * c.omega0 * c.omega0, om2 = c.om2;
     const T sn = P.sn[it], cs = P.cs[it], cot = P.cot[it];
-    const T Rho = st[i], W = st[N + i], V = st[2 * N + i], U = st[3 * N + i], S = st[4 * N + i];
-    const T t = temperature(Rho, S, c);
+    const T W = st[N + i], V = st[2 * N + i], U = st[3 * N + i];
+    const Thermo<T> th = thermo(st[i], st[4 * N + i], ir, c, P);
+    const T Rho = th.rho, t = th.t;
     const T invrho = T(1) / Rho;
     const bool mag = c.magnetic != 0;
     T Bt = T(0), Bp = T(0), Br = T(0);
@@ -312,6 +340,7 @@ __global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const 
         xdu = xdu - U * wk;
         drho_out = drho_out - Rho * wk;
         wk = D[D_RS * N + i];
+        if (c.fluct) wk = wk + P.dsbar[ir];
         xdu = xdu - c.gamm1 * t * wk;
         xds = xds - U * wk;
         if (!mag) du_out = xdu;
@@ -345,15 +374,16 @@ __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, W
         const int ir = (int)(i / c.nphi), ip = (int)(i % c.nphi);
         const T rv = P.r[ir] * c.invpconst;
         const long q0 = ((long)ir * c.nphi + ip) * c.nth, q1 = q0 + c.nth - 1;
-        T rho0 = st[q0], rho1 = st[q1];
-        T t0 = temperature(rho0, st[4 * N + q0], c), t1 = temperature(rho1, st[4 * N + q1], c);
+        const Thermo<T> a0 = thermo(st[q0], st[4 * N + q0], ir, c, P), a1 = thermo(st[q1], st[4 * N + q1], ir, c, P);
+        const T rho0 = a0.rho, rho1 = a1.rho, t0 = a0.t, t1 = a1.t;
         wall[W.off_trho_lo + i] = rv * dv[N + q0] * rho0 / t0;
         wall[W.off_trho_hi + i] = rv * dv[N + q1] * rho1 / t1;
     }
     if (i < nrr) {
         const long q0 = i, q1 = (long)(c.nr - 1) * nrr + i;
-        T rho0 = st[q0], rho1 = st[q1];
-        T t0 = temperature(rho0, st[4 * N + q0], c), t1 = temperature(rho1, st[4 * N + q1], c);
+        const Thermo<T> a0 = thermo(st[q0], st[4 * N + q0], 0, c, P);
+        const Thermo<T> a1 = thermo(st[q1], st[4 * N + q1], c.nr - 1, c, P);
+        const T rho0 = a0.rho, rho1 = a1.rho, t0 = a0.t, t1 = a1.t;
         T lo = (lbcr_rho == 2 || lbcr_rho == 4) ? dv[3 * N + q0] * rho0 * c.invpconst / t0 : rho0;
         T hi = (lbcr_rho == 3 || lbcr_rho == 4) ? dv[3 * N + q1] * rho1 * c.invpconst / t1 : rho1;
         wall[W.off_rrho_lo + i] = lo / P.dxdr[0];
@@ -374,8 +404,9 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
     const long N = c.N;
     int it, ip, ir;
     coords(i, c, it, ip, ir);
-    const T Rho = st[i], W = st[N + i], V = st[2 * N + i], U = st[3 * N + i], S = st[4 * N + i];
-    const T t = temperature(Rho, S, c);
+    const T W = st[N + i], V = st[2 * N + i], U = st[3 * N + i];
+    const Thermo<T> th = thermo(st[i], st[4 * N + i], ir, c, P);
+    const T Rho = th.rho, t = th.t;
     const T invrho = T(1) / Rho;
     T drho = dv[i], dw = dv[N + i], du = dv[3 * N + i], ds = dv[4 * N + i];
     // Density_Terms_Theta
@@ -385,6 +416,7 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
     dw = dw - c.pconst * wk * t;
     // Density_Terms_Radial
     wk = dr_rho[i];
+    if (c.fluct) wk = wk + P.drbar[ir];
     drho = drho - U * wk;
     wk = wk * invrho;
     du = du - c.pconst * t * wk;
@@ -458,8 +490,9 @@ __global__ __launch_bounds__(256) void k_bound(Consts<T> c, BoundFlags f, const 
 // y0/save are FP64; k and y1 are in the stage precision TS.
 // ---------------------------------------------------------------------------------------------
 template <typename TS>
-__global__ __launch_bounds__(256) void k_rk_stage(Consts<TS> c, BoundFlags f, const TS *__restrict__ sr1,
-                                                  const double *__restrict__ sr1_64, int stage, double dt,
+__global__ __launch_bounds__(256) void k_rk_stage(Consts<TS> c, BoundFlags f, const double *__restrict__ sr1_64,
+                                                  const double *__restrict__ rbar64,
+                                                  const double *__restrict__ sbar64, int stage, double dt,
                                                   double *__restrict__ y0, double *__restrict__ save,
                                                   const TS *__restrict__ k, const double *__restrict__ dif,
                                                   TS *__restrict__ y1, int nv) {
@@ -484,21 +517,29 @@ __global__ __launch_bounds__(256) void k_rk_stage(Consts<TS> c, BoundFlags f, co
         s = save[i] + kk / 6.0;
         y = y0[i] + dt * s;
     }
+    const double yb = bound_value<double>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_64);
+    // FP32 stage states carry rho and s as fluctuations about the FP64 background
+    const double bg = (sizeof(TS) == sizeof(double)) ? 0.0 : (iv == 0 ? rbar64[ir] : iv == 4 ? sbar64[ir] : 0.0);
     if (stage < 4) {
         save[i] = s;
-        y1[i] = bound_value<TS>(iv, it, ip, ir, (TS)y, f, c.nth, c.nr, sr1);
+        y1[i] = (TS)(yb - bg);
     } else {
-        const double yb = bound_value<double>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_64);
         y0[i] = yb;
-        if (sizeof(TS) != sizeof(double)) y1[i] = (TS)yb;   // next step's stage-1 input
+        if (sizeof(TS) != sizeof(double)) y1[i] = (TS)(yb - bg);   // next step's stage-1 input
     }
 }
 
-// y1 = (TS) y0 (first step of an FP32 run)
+// y1 = (TS)(y0 - background) (first step of an FP32 run; rho and s as fluctuations)
 template <typename TS>
-__global__ __launch_bounds__(256) void k_cast_state(long n, const double *__restrict__ y0, TS *__restrict__ y1) {
+__global__ __launch_bounds__(256) void k_cast_state(Consts<TS> c, int nv, const double *__restrict__ rbar64,
+                                                    const double *__restrict__ sbar64, const double *__restrict__ y0,
+                                                    TS *__restrict__ y1) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y1[i] = (TS)y0[i];
+    if (i >= c.N * nv) return;
+    const int iv = (int)(i / c.N);
+    const int ir = (int)((i - (long)iv * c.N) / ((long)c.nth * c.nphi));
+    const double bg = iv == 0 ? rbar64[ir] : iv == 4 ? sbar64[ir] : 0.0;
+    y1[i] = (TS)(y0[i] - bg);
 }
 
 
@@ -557,6 +598,7 @@ static Consts<T> make_consts(const cssx_mhd *m) {
     c.invpconst = (T)m->invpconst; c.Cv = (T)m->Cv; c.omega0 = (T)m->p.omega0; c.om2 = (T)(2.0 * m->p.omega0);
     c.xi2 = (T)(m->p.xi_cs * m->p.xi_cs); c.gamma = (T)m->p.gamma; c.T0 = (T)m->p.T0; c.damp = (T)m->p.damping_time;
     c.magnetic = m->p.magnetic; c.top_relax = m->p.top_relax; c.xi_on = m->p.xi_cs < 1.0;
+    c.fluct = sizeof(T) == 4;
     return c;
 }
 
@@ -569,36 +611,77 @@ static BoundFlags make_bound_flags(const cssx_mhd *m) {
 }
 
 template <typename T>
-static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
+static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &drbar,
+                           const std::vector<double> &dsbar) {
     const int nr = m->nr, nth = m->nth;
-    const size_t tot = 5 * (size_t)nr + 7 * (size_t)nth;
+    const size_t tot = 10 * (size_t)nr + 7 * (size_t)nth;
     std::vector<T> h(tot);
     size_t o = 0;
-    auto put = [&](T *&dst_off_holder, size_t &off, size_t n, auto gen) {
-        (void)dst_off_holder;
-        for (size_t i = 0; i < n; i++) h[off + i] = (T)gen(i);
-        off += n;
+    auto put = [&](size_t n, auto gen) {
+        const size_t at = o;
+        for (size_t i = 0; i < n; i++) h[o + i] = (T)gen(i);
+        o += n;
+        return at;
     };
-    size_t o_r = o; put(P.r, o, nr, [&](size_t i) { return m->h_r[i]; });
-    size_t o_ri = o; put(P.rinv, o, nr, [&](size_t i) { return 1.0 / m->h_r[i]; });
-    size_t o_x = o; put(P.dxdr, o, nr, [&](size_t i) { return m->h_dxdr[i]; });
-    size_t o_g = o; put(P.grav, o, nr, [&](size_t i) { return m->h_grav[i]; });
-    size_t o_e = o; put(P.eps, o, nr, [&](size_t i) { return m->h_eps[i]; });
-    size_t o_sn = o; put(P.sn, o, nth, [&](size_t i) { return std::sin(m->h_theta[i]); });
-    size_t o_csc = o; put(P.csc, o, nth, [&](size_t i) { return 1.0 / std::sin(m->h_theta[i]); });
-    size_t o_cs = o; put(P.cs, o, nth, [&](size_t i) { return std::cos(m->h_theta[i]); });
-    size_t o_cot = o; put(P.cot, o, nth, [&](size_t i) { return std::cos(m->h_theta[i]) / std::sin(m->h_theta[i]); });
-    size_t o_d1 = o; put(P.dsdr1, o, nth, [&](size_t i) { return m->h_dsdr1[i]; });
-    size_t o_d2 = o; put(P.dsdr2, o, nth, [&](size_t i) { return m->h_dsdr2[i]; });
-    size_t o_s1 = o; put(P.sr1, o, nth, [&](size_t i) { return m->h_sr1[i]; });
+    const bool f32 = sizeof(T) == 4;
+    const size_t o_r = put(nr, [&](size_t i) { return m->h_r[i]; });
+    const size_t o_ri = put(nr, [&](size_t i) { return 1.0 / m->h_r[i]; });
+    const size_t o_x = put(nr, [&](size_t i) { return m->h_dxdr[i]; });
+    const size_t o_g = put(nr, [&](size_t i) { return m->h_grav[i]; });
+    const size_t o_e = put(nr, [&](size_t i) { return m->h_eps[i]; });
+    const size_t o_rb = put(nr, [&](size_t i) { return m->h_rho_bar[i]; });
+    const size_t o_sb = put(nr, [&](size_t i) { return m->h_s_bar[i]; });
+    const size_t o_tb = put(nr, [&](size_t i) {
+        return m->invtconst * std::pow(m->h_rho_bar[i], m->gamm1) * std::exp(m->h_s_bar[i] * m->invCv);
+    });
+    const size_t o_drb = put(nr, [&](size_t i) { return drbar.empty() ? 0.0 : drbar[i]; });
+    const size_t o_dsb = put(nr, [&](size_t i) { return dsbar.empty() ? 0.0 : dsbar[i]; });
+    const size_t o_sn = put(nth, [&](size_t i) { return std::sin(m->h_theta[i]); });
+    const size_t o_csc = put(nth, [&](size_t i) { return 1.0 / std::sin(m->h_theta[i]); });
+    const size_t o_cs = put(nth, [&](size_t i) { return std::cos(m->h_theta[i]); });
+    const size_t o_cot = put(nth, [&](size_t i) { return std::cos(m->h_theta[i]) / std::sin(m->h_theta[i]); });
+    const size_t o_d1 = put(nth, [&](size_t i) { return m->h_dsdr1[i]; });
+    const size_t o_d2 = put(nth, [&](size_t i) { return m->h_dsdr2[i]; });
+    // FP32 states hold s - s_bar: the constant-temperature wall value is stored the same way
+    const size_t o_s1 = put(nth, [&](size_t i) { return m->h_sr1[i] - (f32 ? m->h_s_bar[0] : 0.0); });
     T *d = nullptr;
-    if (hipMalloc(&d, tot * sizeof(T)) != hipSuccess) return CSSX_ENOMEM;
+    if (P.buf) {
+        d = P.buf;
+    } else if (hipMalloc(&d, tot * sizeof(T)) != hipSuccess) {
+        return CSSX_ENOMEM;
+    }
     if (hipMemcpy(d, h.data(), tot * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return CSSX_EHIP;
     P.buf = d;
     P.r = d + o_r; P.rinv = d + o_ri; P.dxdr = d + o_x; P.grav = d + o_g; P.eps = d + o_e;
+    P.rbar = d + o_rb; P.sbar = d + o_sb; P.tbar = d + o_tb; P.drbar = d + o_drb; P.dsbar = d + o_dsb;
     P.sn = d + o_sn; P.csc = d + o_csc; P.cs = d + o_cs; P.cot = d + o_cot;
     P.dsdr1 = d + o_d1; P.dsdr2 = d + o_d2; P.sr1 = d + o_s1;
     return CSSX_OK;
+}
+
+// Physical radial derivative (times dx/dr) of a radial profile by the FP64 compact operator
+// with zero wall values: the background part of d/dr in the FP32 fluctuation form.
+static int background_derivative(cssx_mhd *m, const double *prof_dev, int code, std::vector<double> &out) {
+    double *d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * m->nr) != hipSuccess) return CSSX_ENOMEM;
+    AxisGeom g;
+    g.n = m->nr; g.na = 1; g.nb = 1; g.sa = 1; g.sb = 0; g.srow = 1;
+    g.g_k = 0; g.g_a = 0; g.g_b = 0; g.phase = CSSX_PHASE_OTHER;
+    DerivSpec s;
+    s.order = 1;
+    s.in = prof_dev; s.out = d;
+    s.lo_kind = s.hi_kind = END_WALL;
+    s.bdry = code;
+    s.scale = m->grid.dri;
+    s.neu_mul = 1.0 / m->grid.dri;
+    s.post = m->p64.dxdr;
+    int rc = run_derivs(&s, 1, g, CSSX_PREC_F64, nullptr);
+    out.assign(m->nr, 0.0);
+    if (!rc && (hipDeviceSynchronize() != hipSuccess ||
+                hipMemcpy(out.data(), d, sizeof(double) * m->nr, hipMemcpyDeviceToHost) != hipSuccess))
+        rc = CSSX_EHIP;
+    (void)hipFree(d);
+    return rc;
 }
 
 template <typename T>
@@ -786,20 +869,24 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
         double b = sizeof(T) + 8.0 + (stage > 1 ? 8.0 : 0.0) + (stage < 4 ? 8.0 + sizeof(T) : 8.0 + (f32 ? 4.0 : 0.0));
         return b * (double)NV;
     };
-    if (f32) hipLaunchKernelGGL(k_cast_state<T>, dim3(nb), dim3(256), 0, stream, NV, (const double *)y0, S.y1);
+    if (f32)
+        hipLaunchKernelGGL(k_cast_state<T>, dim3(nb), dim3(256), 0, stream, c, m->nv, (const double *)m->p64.rbar,
+                           (const double *)m->p64.sbar, (const double *)y0, S.y1);
     for (int step = 0; step < nsteps; step++) {
         // stage 1 reads vars1 = vars0: the FP64 run reads y0 directly, the FP32 run its cast
         const T *in1 = f32 ? S.y1 : (const T *)y0;
         if ((rc = rhs_impl<T>(m, in1, S.k, nullptr, stream))) return rc;
         CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(1),
-                   hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1, 1, dt,
-                                      y0, m->save, (const T *)S.k, (const double *)nullptr, S.y1, m->nv));
+                   hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, (const double *)m->p64.sr1,
+                                      (const double *)m->p64.rbar, (const double *)m->p64.sbar, 1, dt, y0, m->save,
+                                      (const T *)S.k, (const double *)nullptr, S.y1, m->nv));
         for (int stage = 2; stage <= 4; stage++) {
             if ((rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
             CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
-                       hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, P.sr1, m->p64.sr1,
-                                          stage, dt, y0, m->save, (const T *)S.k, (const double *)nullptr, S.y1,
-                                          m->nv));
+                       hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f,
+                                          (const double *)m->p64.sr1, (const double *)m->p64.rbar,
+                                          (const double *)m->p64.sbar, stage, dt, y0, m->save, (const T *)S.k,
+                                          (const double *)nullptr, S.y1, m->nv));
         }
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
@@ -814,7 +901,7 @@ extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *par
     if (!grid || !params || !pr || !out) return CSSX_EINVAL;
     *out = nullptr;
     if (!pr->r || !pr->dr || !pr->dxdr || !pr->theta || !pr->gravity || !pr->eps || !pr->dsdr1 || !pr->dsdr2 ||
-        !pr->sr1 || !pr->sr2)
+        !pr->sr1 || !pr->sr2 || !pr->rho_bar || !pr->s_bar)
         return CSSX_EINVAL;
     if (grid->nr < 8 || grid->nth < 8 || grid->nphi < 8 || choose_M(grid->nr) < 0 || choose_M(grid->nth) < 0 ||
         choose_M(grid->nphi) < 0)
@@ -848,20 +935,24 @@ extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *par
     m->h_dsdr2.assign(pr->dsdr2, pr->dsdr2 + nth);
     m->h_sr1.assign(pr->sr1, pr->sr1 + nth);
     m->h_sr2.assign(pr->sr2, pr->sr2 + nth);
-    if (pr->rho_bar) m->h_rho_bar.assign(pr->rho_bar, pr->rho_bar + nr);
-    if (pr->s_bar) m->h_s_bar.assign(pr->s_bar, pr->s_bar + nr);
+    m->h_rho_bar.assign(pr->rho_bar, pr->rho_bar + nr);
+    m->h_s_bar.assign(pr->s_bar, pr->s_bar + nr);
     if (pr->diffusion_speed) m->h_diff_speed.assign(pr->diffusion_speed, pr->diffusion_speed + nr + 2);
     m->dth = (m->h_theta[nth - 1] - m->h_theta[0]) / (nth - 1);
     m->dphi = 1.0 / grid->dpi;
-    int rc = upload_profiles<double>(m, m->p64);
-    if (!rc) rc = upload_profiles<float>(m, m->p32);
-    if (!rc && hipMalloc(&m->save, sizeof(double) * m->N * m->nv) != hipSuccess) rc = CSSX_ENOMEM;
-    // plans for every code this grid can use
+    // background derivatives need the FP64 profiles and the plans on the device first
+    std::vector<double> none, drbar, dsbar;
+    int rc = upload_profiles<double>(m, m->p64, none, none);
     if (!rc) {
         cssx_ctx *ctx = nullptr;
         rc = cssx_ctx_create(grid, &ctx);
         cssx_ctx_destroy(ctx);
     }
+    if (!rc) rc = background_derivative(m, m->p64.rbar, params->ibcr[1], drbar);
+    if (!rc) rc = background_derivative(m, m->p64.sbar, params->ibcr[5], dsbar);
+    if (!rc) rc = upload_profiles<double>(m, m->p64, drbar, dsbar);
+    if (!rc) rc = upload_profiles<float>(m, m->p32, drbar, dsbar);
+    if (!rc && hipMalloc(&m->save, sizeof(double) * m->N * m->nv) != hipSuccess) rc = CSSX_ENOMEM;
     if (rc) {
         cssx_mhd_destroy(m);
         return rc;

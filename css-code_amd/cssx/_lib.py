@@ -60,8 +60,8 @@ SIGNATURES = {
     "cssx_timing_enable": ([ctypes.c_int], ctypes.c_int),
     "cssx_timing_read": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
-    "cssx_plan_tables": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp,
-                          ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    "cssx_plan_tables": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                          _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
 }
 
 

@@ -64,7 +64,8 @@ class Solver:
         raise TypeError("float32 or float64 state expected")
 
     def rhs(self, state, out=None, want_b=False):
-        """RHS(call_num): d(state)/dt; with want_b also B = curl A (Compute_B)."""
+        """RHS(call_num): d(state)/dt; with want_b also B = curl A (Compute_B).
+        float32 states are in fluctuation form (see to_f32)."""
         self._check_state(state)
         if out is None:
             out = torch.empty_like(state)
@@ -73,6 +74,22 @@ class Solver:
                                  b.data_ptr() if b is not None else None, _stream())
         _lib.check(rc, "cssx_rhs")
         return (out, b) if want_b else out
+
+    def background(self, device="cuda"):
+        """(nv, nr, 1, 1) FP64 tensor with rho_bar and s_bar in slots 0 and 4, zeros elsewhere."""
+        m = self.model
+        bg = torch.zeros((self.shape[0], self.shape[1], 1, 1), dtype=torch.float64, device=device)
+        bg[0, :, 0, 0] = torch.as_tensor(m.rho_bar, device=device)
+        bg[4, :, 0, 0] = torch.as_tensor(m.s_bar, device=device)
+        return bg
+
+    def to_f32(self, y):
+        """FP64 full state -> FP32 state in fluctuation form (rho - rho_bar, s - s_bar)."""
+        return (y - self.background(y.device)).float().contiguous()
+
+    def from_f32(self, y32):
+        """FP32 fluctuation-form state -> FP64 full state."""
+        return (y32.double() + self.background(y32.device)).contiguous()
 
     def bound(self, state):
         """Bound(): wall conditions applied in place."""

@@ -136,10 +136,13 @@ void cssx_mhd_destroy(cssx_mhd *m);
  *   rhs   dvars1 (nv fields): d(rho,w,v,u,s)/dt and dA/dt
  *   b_out optional (3 fields): B = curl A with the wall conditions of Compute_B
  *         (derivatives.f90:1830-1892), what the reference leaves in vars1(6:8) for call 1.
- * prec selects the arithmetic (CSSX_PREC_F64: state/rhs double; CSSX_PREC_F32: float). */
+ * prec selects the arithmetic (CSSX_PREC_F64: state/rhs double; CSSX_PREC_F32: float).
+ * FP32 states are stored in fluctuation form: slot 1 holds rho - rho_bar(r) and slot 5
+ * s - s_bar(r) (the profiles given at creation); every other slot, and every rhs slot, is
+ * the plain value.  This keeps FP32 gradients of the strongly stratified background exact. */
 int cssx_rhs(cssx_mhd *m, int prec, const void *state, void *rhs, void *b_out, void *stream);
 
-/* Bound (rhs_mpi.f90:2395-2491) applied in place to a state. */
+/* Bound (rhs_mpi.f90:2395-2491) applied in place to a state (FP32: fluctuation form). */
 int cssx_bound(cssx_mhd *m, int prec, void *state, void *stream);
 
 /* nsteps classical RK4 steps of Step_RK4 (step_rk4.f90:339-493, SLDiffusion branch) on the
@@ -173,8 +176,11 @@ int cssx_timing_read(double *ms, long *count, double *bytes);
  * The counterpart of the module-level Thomas tables gam1/gam2/gam3 that Initialize_Derivatives
  * builds (derivatives.f90:106-518); here they are the segment-local LU factors, interface
  * response vectors and reduced-system factors of the segmented solve (cssx_plan.hpp layout).
- * lo_kind/hi_kind: 0 wall, 1 halo.  plan_out must hold 5*n + 80 doubles; *nseg receives S. */
-int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, double *plan_out, int *nseg);
+ * lo_kind/hi_kind: 0 wall, 1 halo.  variant 0: strided-axis plan (waves = segments, plan_out
+ * holds 5*n + 80 doubles); variant 1: contiguous-axis plan (lanes = segments, 5*n + 960
+ * doubles).  *nseg receives the number of segments. */
+int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, int variant, double *plan_out,
+                     int *nseg);
 
 #ifdef __cplusplus
 }

@@ -39,17 +39,18 @@ def test_version_and_errors():
 def test_invalid_arguments_rejected_without_device():
     L = cssx.lib()
     buf = (ctypes.c_double * 100)()
-    assert L.cssx_plan_tables(4, 1, 0, 0, 1, buf, None) == _lib.CSSX_EINVAL      # too short
-    assert L.cssx_plan_tables(40, 3, 0, 0, 1, buf, None) == _lib.CSSX_EINVAL     # bad order
-    assert L.cssx_plan_tables(40, 1, 0, 0, 7, buf, None) == _lib.CSSX_EINVAL     # bad code
+    assert L.cssx_plan_tables(4, 1, 0, 0, 1, 0, buf, None) == _lib.CSSX_EINVAL      # too short
+    assert L.cssx_plan_tables(40, 3, 0, 0, 1, 0, buf, None) == _lib.CSSX_EINVAL     # bad order
+    assert L.cssx_plan_tables(40, 1, 0, 0, 7, 0, buf, None) == _lib.CSSX_EINVAL     # bad code
+    assert L.cssx_plan_tables(40, 1, 0, 0, 1, 2, buf, None) == _lib.CSSX_EINVAL     # bad variant
     assert L.cssx_ctx_create(None, None) == _lib.CSSX_EINVAL
 
 
-def _plan(n, order, lo, hi, bdry):
+def _plan(n, order, lo, hi, bdry, variant=0):
     L = cssx.lib()
-    buf = (ctypes.c_double * (5 * n + 80))()
+    buf = (ctypes.c_double * (5 * n + 960))()
     S = ctypes.c_int()
-    assert L.cssx_plan_tables(n, order, lo, hi, bdry, buf, ctypes.byref(S)) == 0
+    assert L.cssx_plan_tables(n, order, lo, hi, bdry, variant, buf, ctypes.byref(S)) == 0
     return np.frombuffer(buf, dtype=np.float64).copy(), S.value
 
 
@@ -129,5 +130,69 @@ def test_segmented_solve_equals_dense(n, order, lo, hi, bdry):
     rng = np.random.default_rng(n * 31 + order)
     d = rng.standard_normal(n)
     x = _segmented_solve(plan, S, n, d)
+    ref = np.linalg.solve(_dense_lhs(order, n, lo, hi, bdry), d)
+    assert np.abs(x - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def _contig_solve(plan, nl, n, MR, d):
+    """Numpy restatement of k_deriv_contig's solve: per-lane segments of MR rows, PCR on the
+    interface rows with the precomputed multipliers (cssx_deriv_contig.hip)."""
+    K, P = 64, 6
+    cf, cl, cu, vL, vR = (plan[i * n:(i + 1) * n] for i in range(5))
+    base = 5 * n
+    rl, ru = plan[base:base + K], plan[base + K:base + 2 * K]
+    k1 = plan[base + 2 * K:base + 2 * K + P * K].reshape(P, K)
+    k2 = plan[base + 2 * K + P * K:base + 2 * K + 2 * P * K].reshape(P, K)
+    invb = plan[base + 2 * K + 2 * P * K:base + 3 * K + 2 * P * K]
+    xh = np.zeros(n)
+    first, last, dI = np.zeros(K), np.zeros(K), np.zeros(K)
+    for j in range(nl):
+        st, en = j * MR, min((j + 1) * MR, n)
+        mi = (en - st - 1) if j < nl - 1 else (en - st)
+        y = 0.0
+        for r in range(st, st + mi):
+            y = cf[r] * d[r] - cl[r] * y
+            xh[r] = y
+        x = 0.0
+        for r in range(st + mi - 1, st - 1, -1):
+            x = xh[r] - cu[r] * x
+            xh[r] = x
+        first[j], last[j], dI[j] = xh[st], xh[st + mi - 1], d[en - 1]
+    nI = nl - 1
+    R = np.zeros(K)
+    for j in range(nI):
+        R[j] = dI[j] - rl[j] * last[j] - ru[j] * first[j + 1]
+    steps = 0
+    while (1 << steps) < nI:
+        steps += 1
+    for s in range(steps):
+        dd = 1 << s
+        Rm = np.array([R[j - dd] if j - dd >= 0 else 0.0 for j in range(K)])
+        Rp = np.array([R[j + dd] if j + dd < K else 0.0 for j in range(K)])
+        R = np.where(np.arange(K) < nI, R - k1[s] * Rm - k2[s] * Rp, 0.0)
+    xI = R * invb
+    x = np.zeros(n)
+    for j in range(nl):
+        st, en = j * MR, min((j + 1) * MR, n)
+        mi = (en - st - 1) if j < nl - 1 else (en - st)
+        xl = xI[j - 1] if j > 0 else 0.0
+        xr = xI[j] if j < nI else 0.0
+        for r in range(st, st + mi):
+            x[r] = xh[r] + vL[r] * xl + vR[r] * xr
+        if j < nI:
+            x[en - 1] = xr
+    return x
+
+
+@pytest.mark.parametrize("n", [8, 9, 13, 40, 64, 65, 128, 256, 257, 300, 512, 777, 1024])
+@pytest.mark.parametrize("order,lo,hi,bdry", [(1, 0, 0, 1), (1, 0, 0, 4), (1, 1, 1, 0), (1, 0, 1, 2),
+                                              (2, 0, 0, 1), (2, 0, 0, 0), (2, 1, 1, 0), (2, 1, 0, 3)])
+def test_contig_pcr_solve_equals_dense(n, order, lo, hi, bdry):
+    MR = 4 if n <= 256 else (8 if n <= 512 else 16)
+    plan, nl = _plan(n, order, lo, hi, bdry, variant=1)
+    assert nl == (n + MR - 1) // MR <= 64
+    rng = np.random.default_rng(n * 17 + order)
+    d = rng.standard_normal(n)
+    x = _contig_solve(plan, nl, n, MR, d)
     ref = np.linalg.solve(_dense_lhs(order, n, lo, hi, bdry), d)
     assert np.abs(x - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
