@@ -87,6 +87,13 @@ class Config:
     sld_speedh: float = 1.0
     sld_speedr: float = 0.0
     sld_exp: float = 0.0
+    dstype: int = 1                 # SLD speed includes |u|^2 and the Alfven speed^2
+    sld_pr: float = 1.0
+    sld_prm: float = 1.0
+    Mass_Diffusion: bool = True     # diffusion.f90:11 defaults
+    Magnetic_Diffusion: bool = True
+    Do_Viscous_Heating: bool = True
+    Do_Ohmic_Heating: bool = True
     # boundary flags (constants.f90:41-62), the benchmark's walls
     Top_Stress_Free: bool = True
     Bottom_Stress_Free: bool = True
@@ -98,6 +105,8 @@ class Config:
     Bottom_Radial_Field: bool = True
     Theta_Normal_Field: bool = True
     Theta_Perf_Conductor: bool = False
+    Top_Perfect_Cond: bool = False
+    Bottom_Perfect_Cond: bool = False
     Theta_Stress_Free: bool = True
 
     @property
@@ -146,6 +155,8 @@ def boundary_codes(c: Config):
         br = bt = bp = ar = at = ap = 1
         if c.Top_Radial_Field and c.Bottom_Radial_Field:
             br, bt, bp, ar, at, ap = 4, 1, 1, 1, 0, 0
+        if c.Top_Perfect_Cond and c.Bottom_Perfect_Cond:
+            br = bt = bp = ar = at = ap = 1
         tbr = tbt = tbp = tar = tat = tap = 1
         if c.Theta_Normal_Field:
             tbr, tbt, tbp, tar, tat, tap = 1, 4, 1, 0, 1, 0
@@ -255,7 +266,11 @@ class Model:
             p.ibct[i] = self.ibct[i]
         p.sld_on = 1
         p.sld_coef = c.sld_coef
-        p.mass_diffusion = p.magnetic_diffusion = p.viscous_heating = p.ohmic_heating = 1
+        p.mass_diffusion, p.magnetic_diffusion = int(c.Mass_Diffusion), int(c.Magnetic_Diffusion)
+        p.viscous_heating, p.ohmic_heating = int(c.Do_Viscous_Heating), int(c.Do_Ohmic_Heating)
+        p.dstype = c.dstype
+        p.sld_pr, p.sld_prm = c.sld_pr, c.sld_prm
+        p.bottom_perf_cond, p.top_perf_cond = int(c.Bottom_Perfect_Cond), int(c.Top_Perfect_Cond)
         return p
 
     def profiles_struct(self):
@@ -289,7 +304,9 @@ class MhdParams(ctypes.Structure):
                 ("ibcr", ctypes.c_int * 13), ("ibct", ctypes.c_int * 13),
                 ("sld_on", ctypes.c_int), ("sld_coef", ctypes.c_double),
                 ("mass_diffusion", ctypes.c_int), ("magnetic_diffusion", ctypes.c_int),
-                ("viscous_heating", ctypes.c_int), ("ohmic_heating", ctypes.c_int)]
+                ("viscous_heating", ctypes.c_int), ("ohmic_heating", ctypes.c_int),
+                ("dstype", ctypes.c_int), ("sld_pr", ctypes.c_double), ("sld_prm", ctypes.c_double),
+                ("bottom_perf_cond", ctypes.c_int), ("top_perf_cond", ctypes.c_int)]
 
 
 class MhdProfiles(ctypes.Structure):

@@ -107,6 +107,10 @@ typedef struct cssx_mhd_params {
     int sld_on;                  /* SLDiffusion: Euler diffusion folded into stage 1 */
     double sld_coef;             /* 0 gives an exactly zero diffusion tendency */
     int mass_diffusion, magnetic_diffusion, viscous_heating, ohmic_heating;
+    int dstype;                  /* 1: add |u|^2 and B^2/(4 pi rho) to the SLD speed^2 */
+    double sld_pr, sld_prm;      /* flux multipliers (1 = off): diffusion.f90:846-854 */
+    int bottom_perf_cond, top_perf_cond;   /* A_theta, A_phi ghosts 0 at that r wall in the SLD
+                                              fluxes (diffusion.f90:628-643, 691-706) */
 } cssx_mhd_params;
 
 /* Radial / latitudinal profiles (host arrays, copied at creation).  Lengths: nr or nth. */

@@ -48,6 +48,7 @@ def lib():
         L.orc_rhs.argtypes = [_vp, _vp, _vp, _dp, _dp, _dp]
         L.orc_bound.argtypes = [_vp, _vp, _vp, _dp]
         L.orc_rk4.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_int]
+        L.orc_diffusion.argtypes = [_vp, _vp, _vp, _dp, _dp, _dp]
     return _lib
 
 
@@ -157,6 +158,26 @@ def bound(model, state):
     st = _f64(state).copy()
     lib().orc_bound(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st))
     return st
+
+
+def diffusion(model, state, B=None):
+    """Compute_Diffusion(1) (diffusion.f90:70-474) of a state (A in slots 6-8): the SLD
+    tendency.  B = curl A as RHS(1) leaves it in vars1(6:8); computed when not given."""
+    g, p, pr = _structs(model)
+    st = _f64(state)
+    nv = st.shape[0]
+    v = st.copy()
+    a = np.ascontiguousarray(st[5:8]) if nv == 8 else np.zeros((3,) + st.shape[1:])
+    if nv == 8:
+        if B is None:
+            B = rhs(model, st, want_b=True)[1]
+        v[5:8] = B
+    dv = np.empty_like(st)
+    rc = lib().orc_diffusion(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(v), _ptr(a),
+                             _ptr(dv))
+    if rc:
+        raise ValueError(f"orc_diffusion rc={rc}")
+    return dv
 
 
 def rk4(model, state, dt, nsteps):

@@ -102,7 +102,7 @@ static void dr_(const orc_mhd *m, const double *x, double *out, int code, const 
 
 /* Compute_B: B = curl A with wall zeroing; a, b point to 3 fields (theta, phi, r). */
 static void compute_b(const orc_mhd *m, const double *a, double *b, double *work, double *work2, double *glo,
-                      double *ghi)
+                      double *ghi, int zero_walls)
 {
     const int nr = m->nr, nth = m->nth, nphi = m->nphi;
     const long N = m->N;
@@ -150,7 +150,9 @@ static void compute_b(const orc_mhd *m, const double *a, double *b, double *work
     dr_(m, work, work2, m->p.ibcr[11], NULL, NULL, glo, ghi);
     for (ir = 0; ir < nr; ir++)
         for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) bt[i] = bt[i] - work2[i] * (1.0 / m->r[ir]);
-    /* wall conditions (derivatives.f90:1840-1883) */
+    /* wall conditions (derivatives.f90:1840-1883); VectorCurlA alone (Ohmic heating,
+     * diffusion.f90:437) leaves them out */
+    if (!zero_walls) return;
     for (ir = 0; ir < nr; ir++)
         for (ip = 0; ip < nphi; ip++) {
             if (m->p.theta_perf_cond) {
@@ -199,7 +201,7 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
 
     /* Compute_B(1): B replaces A inside the RHS */
     if (p->magnetic) {
-        compute_b(m, v + 5 * N, B, work, work2, glo, ghi);
+        compute_b(m, v + 5 * N, B, work, work2, glo, ghi, 1);
         if (bout) memcpy(bout, B, sizeof(double) * 3 * N);
     }
     bt = B; bp = B + N; br = B + 2 * N;
@@ -548,8 +550,268 @@ int orc_bound(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profi
     return 0;
 }
 
+/* ---------------------------------------------------------------------------------------------
+ * Compute_Diffusion (diffusion.f90:70-474) with Diffusion_R (:476-891), Diffusion_Th
+ * (:893-1301), Diffusion_Phi (:1303-1495): slope-limited (Van Albada) second-order fluxes,
+ * speed sld_coef*sqrt(vcc), zero flux through the r and theta walls, then viscous heating,
+ * mass diffusion and Ohmic heating corrections.  v = vars1 as the RHS(1) call leaves it
+ * (B in slots 6-8), a = the vector potential A (maga), dv = the diffusion tendency.
+ * The background meanV is the hydrostatic profile (update_background = .False.) and the
+ * optional flux limiters are off (their defaults, diffusion.f90:8).
+ * ------------------------------------------------------------------------------------------- */
+static double van_albada(double q, double qph, double tiny)
+{
+    double rt = fabs(q);
+    rt = 1.0 / fmax(rt, tiny);
+    rt = q * rt * rt;
+    rt = rt * qph;
+    rt = fmin(1.0, rt);
+    return q * fmax(0.0, rt);
+}
+
+/* face fluxes of one pencil: tmp[0..n-1], ghosts gm (the value before tmp[0], "b1(:,4)") and
+ * gp (after tmp[n-1], "b2(:,1)"), gm2/gp2 the next ones; delue[0..n] = qph - (phiL+phiR)/2 */
+static void sld_faces(int n, const double *tmp, double gm2, double gm, double gp, double gp2, double tiny,
+                      double *delue)
+{
+    int k;
+    for (k = 0; k <= n; k++) {
+        /* face k sits between tmp[k-1] and tmp[k] (Fortran face k+1) */
+        double fm2 = (k - 2 >= 0) ? tmp[k - 2] : (k - 2 == -1 ? gm : gm2);
+        double fm1 = (k - 1 >= 0) ? tmp[k - 1] : gm;
+        double f0 = (k < n) ? tmp[k] : gp;
+        double fp1 = (k + 1 < n) ? tmp[k + 1] : (k + 1 == n ? gp : gp2);
+        double qmh = fm1 - fm2, qph = f0 - fm1, qp3h = fp1 - f0;
+        double phiL = van_albada(qmh, qph, tiny);
+        double phiR = van_albada(qp3h, qph, tiny);
+        delue[k] = qph - 0.5 * (phiR + phiL);
+    }
+}
+
 int orc_diffusion(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, const double *v,
-                  const double *b, double *dv);
+                  const double *a, double *dv)
+{
+    orc_mhd M, *m = &M;
+    const int nr = g->nr, nth = g->nth, nphi = g->nphi, nv = p->magnetic ? 8 : 5;
+    const long N = (long)nr * nth * nphi;
+    const int vth = nth + 2, vph = nphi + 2;
+    double *vcc, *f, *invrho, *itemp, *tmp, *delue, *work, *work2, *glo, *ghi, *curl;
+    double tinyvar[9], dth, dphi;
+    int varlist[8], nsld = 0, isl, iv, it, ip, ir, k;
+    long i;
+    setup(m, g, p, pr);
+    memset(dv, 0, sizeof(double) * nv * N);
+    if (!p->sld_on) { free(m->sines); return 0; }
+    /* tinyvar (diffusion.f90:76-80), 1-based by variable */
+    tinyvar[1] = 1e-8 * pr->rho_bar[nr - 1];
+    tinyvar[2] = tinyvar[3] = tinyvar[4] = 1e-8;
+    tinyvar[5] = 1e-3;
+    tinyvar[6] = tinyvar[7] = tinyvar[8] = 1.0;
+    /* varlist (:86-112) */
+    for (iv = (p->mass_diffusion ? 1 : 2); iv <= ((p->magnetic && p->magnetic_diffusion) ? 8 : 5); iv++)
+        varlist[nsld++] = iv;
+    dth = (pr->theta[nth - 1] - pr->theta[0]) / (nth - 1);
+    dphi = 1.0 / g->dpi;
+    vcc = (double *)calloc((size_t)vth * vph * (nr + 2), sizeof(double));
+    f = (double *)malloc(sizeof(double) * 8 * N);
+    invrho = (double *)malloc(sizeof(double) * N);
+    itemp = (double *)malloc(sizeof(double) * N);
+    tmp = (double *)malloc(sizeof(double) * (nr + nth + nphi + 8));
+    delue = (double *)malloc(sizeof(double) * (nr + nth + nphi + 8));
+    work = (double *)malloc(sizeof(double) * N);
+    work2 = (double *)malloc(sizeof(double) * N);
+    curl = (double *)malloc(sizeof(double) * 3 * N);
+    glo = (double *)calloc(4 * (size_t)N / (nr < nth ? nr : nth) + 64, sizeof(double));
+    ghi = (double *)calloc(4 * (size_t)N / (nr < nth ? nr : nth) + 64, sizeof(double));
+    if (!vcc || !f || !invrho || !itemp || !tmp || !delue || !work || !work2 || !curl || !glo || !ghi) return 12;
+#define VC(it, ip, ir) vcc[((long)(ir) * vph + (ip)) * vth + (it)]
+    /* 1/rho and 1/T (:114-121), fields with the background removed (:168-175) */
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                i = IDX(m, it, ip, ir);
+                invrho[i] = 1.0 / v[i];
+                itemp[i] = pow(v[i], -m->gamm1) * exp(-v[4 * N + i] * m->invCv) / m->invtconst;
+                f[i] = v[i] - pr->rho_bar[ir];
+                f[N + i] = v[N + i]; f[2 * N + i] = v[2 * N + i]; f[3 * N + i] = v[3 * N + i];
+                f[4 * N + i] = v[4 * N + i] - pr->s_bar[ir];
+                if (p->magnetic) { f[5 * N + i] = a[i]; f[6 * N + i] = a[N + i]; f[7 * N + i] = a[2 * N + i]; }
+            }
+    /* vcc: speed^2 with theta/phi/r ghosts (:177-287); phi ghosts are the periodic wrap */
+    for (ir = 0; ir <= nr + 1; ir++)
+        for (ip = 0; ip < vph; ip++)
+            for (it = 0; it < vth; it++) VC(it, ip, ir) = pr->diffusion_speed[ir];
+    if (p->dstype == 1) {
+        for (ir = 0; ir < nr; ir++)
+            for (ip = -1; ip <= nphi; ip++) {
+                int ipw = (ip + nphi) % nphi;
+                for (it = 0; it < nth; it++) {
+                    i = IDX(m, it, ipw, ir);
+                    for (iv = 2; iv <= 4; iv++) VC(it + 1, ip + 1, ir + 1) += v[(iv - 1) * N + i] * v[(iv - 1) * N + i];
+                }
+            }
+        if (p->magnetic)
+            for (ir = 0; ir < nr; ir++)
+                for (ip = -1; ip <= nphi; ip++) {
+                    int ipw = (ip + nphi) % nphi;
+                    for (it = 0; it < nth; it++) {
+                        i = IDX(m, it, ipw, ir);
+                        for (iv = 6; iv <= 8; iv++)
+                            VC(it + 1, ip + 1, ir + 1) += ONE_OVER_4PI * invrho[i] * v[(iv - 1) * N + i] * v[(iv - 1) * N + i];
+                    }
+                }
+    }
+    for (ip = 0; ip < vph; ip++)
+        for (it = 0; it < vth; it++) { VC(it, ip, 0) = VC(it, ip, 1); VC(it, ip, nr + 1) = VC(it, ip, nr); }
+    for (ir = 0; ir <= nr + 1; ir++)
+        for (ip = 0; ip < vph; ip++) { VC(0, ip, ir) = VC(1, ip, ir); VC(nth + 1, ip, ir) = VC(nth, ip, ir); }
+
+    for (isl = 0; isl < nsld; isl++) {
+        const double tiny = tinyvar[varlist[isl]];
+        const double *fv;
+        double *dvv;
+        iv = varlist[isl];
+        fv = f + (long)(iv - 1) * N;
+        dvv = dv + (long)(iv - 1) * N;
+        /* ---- Diffusion_R ---- */
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                double gm, gp;
+                for (ir = 0; ir < nr; ir++) tmp[ir] = fv[IDX(m, it, ip, ir)];
+                switch (iv) {           /* b1(:,4) and b2(:,1) at the walls (:586-711) */
+                case 1: case 5: gm = 0.0; gp = 0.0; break;
+                case 2: case 3:
+                    gm = -10.0 / 3.0 * tmp[0] / pr->r[0] + 6.0 * tmp[1] / pr->r[1] - 2.0 * tmp[2] / pr->r[2] +
+                         1.0 / 3.0 * tmp[3] / pr->r[3];
+                    gp = -10.0 / 3.0 * tmp[nr - 1] / pr->r[nr - 1] + 6.0 * tmp[nr - 2] / pr->r[nr - 2] -
+                         2.0 * tmp[nr - 3] / pr->r[nr - 3] + 1.0 / 3.0 * tmp[nr - 4] / pr->r[nr - 4];
+                    break;
+                case 6: case 7:
+                    gm = p->bottom_perf_cond ? 0.0 : tmp[0];
+                    gp = p->top_perf_cond ? 0.0 : tmp[nr - 1];
+                    break;
+                default: gm = tmp[0]; gp = tmp[nr - 1]; break;   /* 4 and 8 */
+                }
+                sld_faces(nr, tmp, gm, gm, gp, gp, tiny, delue);
+                delue[0] = 0.0;
+                delue[nr] = 0.0;
+                for (k = 1; k < nr; k++) {
+                    double rph = 0.5 * (pr->r[k] + pr->r[k - 1]);
+                    delue[k] = delue[k] * rph * rph;
+                }
+                for (k = 0; k <= nr; k++) {
+                    double vt = p->sld_coef * sqrt(0.5 * (VC(it + 1, ip + 1, k + 1) + VC(it + 1, ip + 1, k)));
+                    delue[k] = vt * delue[k];
+                }
+                for (ir = 0; ir < nr; ir++) {
+                    double rl = (ir == 0) ? pr->r[0] - 0.5 * pr->dr[0] : 0.5 * (pr->r[ir] + pr->r[ir - 1]);
+                    double rh = (ir == nr - 1) ? pr->r[nr - 1] + 0.5 * pr->dr[nr - 1] : 0.5 * (pr->r[ir + 1] + pr->r[ir]);
+                    double fl = 3.0 / (rh * rh * rh - rl * rl * rl) * (delue[ir + 1] - delue[ir]);
+                    if (p->sld_pr != 1.0 || p->sld_prm != 1.0) {
+                        if (iv <= 3) fl = p->sld_pr * fl;
+                        else if (iv >= 5) fl = p->sld_prm * fl;
+                    }
+                    dvv[IDX(m, it, ip, ir)] += fl;
+                }
+            }
+        /* ---- Diffusion_Th ---- */
+        for (ir = 0; ir < nr; ir++) {
+            double rm = (ir == 0) ? pr->r[0] - 0.5 * pr->dr[0] : 0.5 * (pr->r[ir] + pr->r[ir - 1]);
+            double rp = (ir == nr - 1) ? pr->r[nr - 1] + 0.5 * pr->dr[nr - 1] : 0.5 * (pr->r[ir + 1] + pr->r[ir]);
+            double rfac = 1.5 * (rp * rp - rm * rm) / (rp * rp * rp - rm * rm * rm);
+            for (ip = 0; ip < nphi; ip++) {
+                double gm, gp;
+                for (it = 0; it < nth; it++) tmp[it] = fv[IDX(m, it, ip, ir)];
+                switch (iv) {           /* b1(:,4), b2(:,1) at the theta walls (:1008-1120) */
+                case 2: gm = 0.0; gp = 0.0; break;
+                case 3:
+                    gm = -10.0 / 3.0 * tmp[0] / m->sines[0] + 6.0 * tmp[1] / m->sines[1] -
+                         2.0 * tmp[2] / m->sines[2] + 1.0 / 3.0 * tmp[3] / m->sines[3];
+                    gp = -10.0 / 3.0 * tmp[nth - 1] / m->sines[nth - 1] + 6.0 * tmp[nth - 2] / m->sines[nth - 2] -
+                         2.0 * tmp[nth - 3] / m->sines[nth - 3] + 1.0 / 3.0 * tmp[nth - 4] / m->sines[nth - 4];
+                    break;
+                case 6:
+                    gm = p->theta_normal_field ? 0.0 : tmp[0];
+                    gp = p->theta_normal_field ? 0.0 : tmp[nth - 1];
+                    break;
+                case 7: case 8:
+                    gm = p->theta_perf_cond ? 0.0 : tmp[0];
+                    gp = p->theta_perf_cond ? 0.0 : tmp[nth - 1];
+                    break;
+                default: gm = tmp[0]; gp = tmp[nth - 1]; break;
+                }
+                sld_faces(nth, tmp, gm, gm, gp, gp, tiny, delue);
+                delue[0] = 0.0;
+                delue[nth] = 0.0;
+                for (k = 0; k <= nth; k++) {
+                    double th = (k == 0) ? pr->theta[0] - 0.5 * dth
+                              : (k == nth) ? pr->theta[nth - 1] + 0.5 * dth : 0.5 * (pr->theta[k] + pr->theta[k - 1]);
+                    double sph = (k == 0) ? fabs(sin(th)) : sin(th);
+                    double vt = p->sld_coef * sqrt(0.5 * (VC(k + 1, ip + 1, ir + 1) + VC(k, ip + 1, ir + 1)));
+                    delue[k] = vt * sph * delue[k];
+                }
+                for (it = 0; it < nth; it++) {
+                    double thl = (it == 0) ? pr->theta[0] - 0.5 * dth : 0.5 * (pr->theta[it] + pr->theta[it - 1]);
+                    double thh = (it == nth - 1) ? pr->theta[nth - 1] + 0.5 * dth : 0.5 * (pr->theta[it + 1] + pr->theta[it]);
+                    double t = rfac / (cos(thl) - cos(thh));
+                    double fl = t * (delue[it + 1] - delue[it]);
+                    if (p->sld_pr != 1.0 || p->sld_prm != 1.0) {
+                        if (iv < 5) fl = p->sld_pr * fl;
+                        else if (iv > 5) fl = p->sld_prm * fl;
+                    }
+                    dvv[IDX(m, it, ip, ir)] += fl;
+                }
+            }
+        }
+        /* ---- Diffusion_Phi ---- */
+        for (ir = 0; ir < nr; ir++) {
+            double rm = (ir == 0) ? pr->r[0] - 0.5 * pr->dr[0] : 0.5 * (pr->r[ir] + pr->r[ir - 1]);
+            double rp = (ir == nr - 1) ? pr->r[nr - 1] + 0.5 * pr->dr[nr - 1] : 0.5 * (pr->r[ir + 1] + pr->r[ir]);
+            double rfac = 1.5 * (rp * rp - rm * rm) / (rp * rp * rp - rm * rm * rm) / dphi;
+            for (it = 0; it < nth; it++) {
+                double thl = (it == 0) ? pr->theta[0] - 0.5 * dth : 0.5 * (pr->theta[it] + pr->theta[it - 1]);
+                double thh = (it == nth - 1) ? pr->theta[nth - 1] + 0.5 * dth : 0.5 * (pr->theta[it + 1] + pr->theta[it]);
+                double geom = ((thh - thl) / (cos(thl) - cos(thh))) * rfac;
+                for (ip = 0; ip < nphi; ip++) tmp[ip] = fv[IDX(m, it, ip, ir)];
+                sld_faces(nphi, tmp, tmp[nphi - 2], tmp[nphi - 1], tmp[0], tmp[1], tiny, delue);
+                for (k = 0; k <= nphi; k++) {
+                    double vt = p->sld_coef * sqrt(0.5 * (VC(it + 1, k, ir + 1) + VC(it + 1, k + 1, ir + 1)));
+                    delue[k] = vt * delue[k];
+                }
+                for (ip = 0; ip < nphi; ip++) {
+                    double fl = delue[ip + 1] - delue[ip];
+                    if (p->sld_pr != 1.0 || p->sld_prm != 1.0) {
+                        if (iv < 5) fl = p->sld_pr * fl;
+                        else if (iv > 5) fl = p->sld_prm * fl;
+                    }
+                    dvv[IDX(m, it, ip, ir)] += geom * fl;
+                }
+            }
+        }
+    }
+    /* viscous heating (:375-393) with temperature = 1/T, then mass diffusion (:395-428) */
+    if (p->viscous_heating)
+        for (iv = 2; iv <= 4; iv++)
+            for (i = 0; i < N; i++) dv[4 * N + i] = dv[4 * N + i] - v[(iv - 1) * N + i] * dv[(iv - 1) * N + i] * itemp[i];
+    if (p->mass_diffusion) {
+        for (iv = 2; iv <= 4; iv++)
+            for (i = 0; i < N; i++) dv[(iv - 1) * N + i] = dv[(iv - 1) * N + i] - invrho[i] * v[(iv - 1) * N + i] * dv[i];
+        for (i = 0; i < N; i++) dv[4 * N + i] = dv[4 * N + i] - (m->gamm1 * m->Cv) * invrho[i] * dv[i];
+    }
+    /* Ohmic heating (:430-461): curl of the A tendency against B */
+    if (p->magnetic && p->ohmic_heating) {
+        compute_b(m, dv + 5 * N, curl, work, work2, glo, ghi, 0);
+        for (iv = 6; iv <= 8; iv++)
+            for (i = 0; i < N; i++) {
+                double t = (0.5 * ONE_OVER_4PI) * invrho[i] * itemp[i];
+                dv[4 * N + i] = dv[4 * N + i] - v[(iv - 1) * N + i] * curl[(iv - 6) * N + i] * t;
+            }
+    }
+#undef VC
+    free(vcc); free(f); free(invrho); free(itemp); free(tmp); free(delue); free(work); free(work2); free(curl);
+    free(glo); free(ghi); free(m->sines);
+    return 0;
+}
 
 /* nsteps of Step_RK4 (step_rk4.f90:339-493), SLDiffusion branch: k1 with Euler diffusion. */
 int orc_rk4(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, double *y0, double dt,
@@ -574,7 +836,7 @@ int orc_rk4(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
             if (p->magnetic) memcpy(yb + 5 * N, B, sizeof(double) * 3 * N);
             {
                 double *D = (double *)calloc(NV, sizeof(double));
-                rc |= orc_diffusion(g, p, pr, yb, y1 + 5 * N, D);
+                rc |= orc_diffusion(g, p, pr, yb, y1 + 5 * N, D);   /* A = slots 6-8 of vars0 */
                 for (i = 0; i < NV; i++) save[i] = D[i] + k[i] / 6.0;
                 free(D);
             }
