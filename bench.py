@@ -17,6 +17,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -65,13 +66,13 @@ def profile_phases(sol, y0, dt, steps, precision):
     return tab
 
 
-def cpu_baseline(seconds_budget=20.0):
+def cpu_baseline(sld_coef, seconds_budget=20.0):
     """The FP64 oracle RK4 step on a bounded sample of the same recipe (OpenMP threads)."""
     import numpy as np
 
     import oracle
     from cssx.model import Config, Model
-    c = Config(nr=32, nth=64, nphi=128)
+    c = Config(nr=32, nth=64, nphi=128, sld_coef=sld_coef)
     m = Model(c)
     st = m.initial_state()
     g, p, pr = oracle._structs(m)
@@ -90,6 +91,17 @@ def cpu_baseline(seconds_budget=20.0):
                       f"recipe, {steps} step(s) in {el:.1f} s, OpenMP threads={threads}"}
 
 
+def _clean(o):
+    """NaN/inf (a precision path that was not run) -> null, so the line stays strict JSON."""
+    if isinstance(o, dict):
+        return {k: _clean(v) for k, v in o.items()}
+    if isinstance(o, list):
+        return [_clean(v) for v in o]
+    if isinstance(o, float) and not math.isfinite(o):
+        return None
+    return o
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,6 +111,11 @@ def main():
     ap.add_argument("--nr", type=int, default=128)
     ap.add_argument("--nth", type=int, default=256)
     ap.add_argument("--nphi", type=int, default=512)
+    ap.add_argument("--sld-coef", type=float, default=0.1,
+                    help="SLD diffusion coefficient (SLDiffusion is on; 0 keeps its cost with a zero tendency)")
+    ap.add_argument("--paths", default="f64,f32", help="precisions to run: f64,f32 | f32 | f64")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -112,30 +129,53 @@ def main():
     from cssx.model import Config, Model
 
     torch.cuda.set_device(0)
-    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi)
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}))
+        return
+    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=args.sld_coef)
     m = Model(c)
     y0 = torch.as_tensor(m.initial_state(), device="cuda").contiguous()
     sol = Solver(m)
     dt = sol.cfl(y0)
     npts = c.npts
 
-    y64, ms64 = run_path(sol, y0, dt, args.steps, args.warmup, "f64")
-    y32, ms32 = run_path(sol, y0, dt, args.steps, args.warmup, "f32")
+    paths = set(args.paths.split(","))
+    y64 = y32 = None
+    ms64 = ms32 = float("nan")
+    if "f64" in paths:
+        y64, ms64 = run_path(sol, y0, dt, args.steps, args.warmup, "f64")
+    if "f32" in paths:
+        y32, ms32 = run_path(sol, y0, dt, args.steps, args.warmup, "f32")
 
     # FP32-bulk vs FP64 after the same K steps from the same state
-    rel_state, rel_change = [], []
-    for iv in range(c.nv):
-        d = (y32[iv] - y64[iv]).abs().max().item()
-        rel_state.append(d / max(y64[iv].abs().max().item(), 1e-300))
-        ch = (y64[iv] - y0[iv]).abs().max().item()
-        rel_change.append(d / max(ch, 1e-300))
-    finite = bool(torch.isfinite(y64).all().item() and torch.isfinite(y32).all().item())
+    cmp = None
+    if y64 is not None and y32 is not None:
+        rel_state, rel_change, rms_state = [], [], []
+        for iv in range(c.nv):
+            diff = y32[iv] - y64[iv]
+            d = diff.abs().max().item()
+            rel_state.append(d / max(y64[iv].abs().max().item(), 1e-300))
+            rms_state.append((diff.pow(2).mean().sqrt() / y64[iv].pow(2).mean().sqrt().clamp_min(1e-300)).item())
+            ch = (y64[iv] - y0[iv]).abs().max().item()
+            rel_change.append(d / max(ch, 1e-300))
+        finite = bool(torch.isfinite(y64).all().item() and torch.isfinite(y32).all().item())
+        cmp = {"steps": args.steps, "max_rel_err_state": max(rel_state), "max_rel_err_of_change": max(rel_change),
+               "rms_rel_err_state": max(rms_state), "per_field_max_rel_err": [float(f"{x:.3e}") for x in rel_state],
+               "finite": finite}
 
     prof_steps = max(2, min(args.steps, 5))
-    tab32 = profile_phases(sol, y0, dt, prof_steps, "f32")
-    tab64 = profile_phases(sol, y0, dt, prof_steps, "f64")
+    tab32 = profile_phases(sol, y0, dt, prof_steps, "f32") if "f32" in paths else None
+    tab64 = profile_phases(sol, y0, dt, prof_steps, "f64") if "f64" in paths else None
 
     def roofline(tab):
+        if tab is None:
+            return None
         # dominant kernel = the phase with the largest summed time; k_deriv covers the three axes
         dphases = ("deriv_r", "deriv_theta", "deriv_phi")
         dom = max(tab, key=lambda k: tab[k][0])
@@ -158,34 +198,34 @@ def main():
     val64 = npts / (ms64 * 1e-3) / 1e6
     out = {
         "metric": METRIC,
-        "value": round(val32, 2),
+        "value": round(val32, 2) if "f32" in paths else round(val64, 2),
         "unit": "Mpt-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms32, 4),
+        "ms_per_step": round(ms32 if "f32" in paths else ms64, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if "f32" in paths else "f64",
         "data": "synthetic",
         "config": {"workload": f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field", "nr": c.nr, "nth": c.nth,
-                   "nphi": c.nphi, "fields": c.nv, "accumulate": "f64", "dt": dt, "parallelism": "single-gpu"},
+                   "nphi": c.nphi, "fields": c.nv, "accumulate": "f64", "dt": dt, "sld_coef": c.sld_coef,
+                   "parallelism": "single-gpu"},
         "fp64": {"value": round(val64, 2), "unit": "Mpt-steps/s", "ms_per_step": round(ms64, 4)},
-        "fp32_speedup_vs_fp64": round(ms64 / ms32, 3),
-        "fp32_vs_fp64": {"steps": args.steps, "max_rel_err_state": max(rel_state),
-                         "max_rel_err_of_change": max(rel_change), "finite": finite},
+        "fp32_speedup_vs_fp64": round(ms64 / ms32, 3) if len(paths) == 2 else None,
+        "fp32_vs_fp64": cmp,
         "hbm_tbs_algorithmic": {"f32": round(val32 * 1e6 * STEP_BYTES_MIXED / 1e12, 3),
                                 "f64": round(val64 * 1e6 * STEP_BYTES_F64 / 1e12, 3)},
-        "phases_ms_per_step": {"f32": {k: round(v[0] / prof_steps, 4) for k, v in tab32.items()},
-                               "f64": {k: round(v[0] / prof_steps, 4) for k, v in tab64.items()}},
+        "phases_ms_per_step": {p: ({k: round(v[0] / prof_steps, 4) for k, v in t.items()} if t else None)
+                               for p, t in (("f32", tab32), ("f64", tab64))},
         "roofline": roofline(tab32),
         "roofline_f64": roofline(tab64),
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        out["cpu_baseline"] = cpu_baseline(c.sld_coef)
     sol.close()
-    print(json.dumps(out))
+    print(json.dumps(_clean(out)))
 
 
 if __name__ == "__main__":

@@ -45,6 +45,12 @@ struct Prof {
     // hydrostatic background (nr): rho_bar, s_bar, T_bar and the physical radial derivatives
     // of rho_bar and s_bar by the compact operator with zero wall values
     T *rbar = nullptr, *sbar = nullptr, *tbar = nullptr, *drbar = nullptr, *dsbar = nullptr;
+    // SLD face geometry (diffusion.f90:805-838, 1219-1250, 1316-1348): r face radii rfc (nr+1,
+    // 0 on the walls so their flux vanishes), 3/(r_h^3-r_l^3) rv (nr), sin of the theta faces
+    // sph (nth+1, 0 on the walls), 1.5(r_h^2-r_l^2)/(r_h^3-r_l^3) rfac (nr),
+    // 1/(cos th_l - cos th_h) ct (nth), (th_h-th_l)/(cos th_l - cos th_h) gt (nth), and the
+    // SLD speed^2 of each radius vsp (nr)
+    T *rfc = nullptr, *rv = nullptr, *sph = nullptr, *rfac = nullptr, *ct = nullptr, *gt = nullptr, *vsp = nullptr;
     T *buf = nullptr;
 };
 
@@ -63,6 +69,7 @@ struct Scratch {
     T *wall = nullptr;   // wall value buffers
     T *y1 = nullptr;     // RK stage state (nv fields)
     T *k = nullptr;      // RK stage tendency (nv fields)
+    T *dif = nullptr;    // SLD diffusion tendency of stage 1 (nv fields, SLDiffusion runs)
     bool ready = false;
 };
 
@@ -150,7 +157,7 @@ __global__ __launch_bounds__(256) void k_curl_combine(Consts<T> c, Prof<T> P, co
                                                       const T *__restrict__ C1, const T *__restrict__ C2,
                                                       const T *__restrict__ C3, const T *__restrict__ C4,
                                                       const T *__restrict__ C5, T *__restrict__ B, int perf_cond,
-                                                      int bot_radial, int top_radial) {
+                                                      int bot_radial, int top_radial, int zero_walls) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.N) return;
     int it, ip, ir;
@@ -162,12 +169,12 @@ __global__ __launch_bounds__(256) void k_curl_combine(Consts<T> c, Prof<T> P, co
     br = br - ri * C3[i] * csc;
     bp = bp + C4[i] * ri;
     bt = bt - C5[i] * ri;
-    const bool thw = (it == 0 || it == c.nth - 1);
+    const bool thw = zero_walls && (it == 0 || it == c.nth - 1);
     if (thw) {
         if (perf_cond) bt = T(0);
         else { bp = T(0); br = T(0); }
     }
-    if ((ir == 0 && bot_radial) || (ir == c.nr - 1 && top_radial)) { bt = T(0); bp = T(0); }
+    if (zero_walls && ((ir == 0 && bot_radial) || (ir == c.nr - 1 && top_radial))) { bt = T(0); bp = T(0); }
     B[i] = bt;
     B[c.N + i] = bp;
     B[2 * c.N + i] = br;
@@ -494,7 +501,7 @@ __global__ __launch_bounds__(256) void k_rk_stage(Consts<TS> c, BoundFlags f, co
                                                   const double *__restrict__ rbar64,
                                                   const double *__restrict__ sbar64, int stage, double dt,
                                                   double *__restrict__ y0, double *__restrict__ save,
-                                                  const TS *__restrict__ k, const double *__restrict__ dif,
+                                                  const TS *__restrict__ k, const TS *__restrict__ dif,
                                                   TS *__restrict__ y1, int nv) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.N * nv) return;
@@ -505,7 +512,7 @@ __global__ __launch_bounds__(256) void k_rk_stage(Consts<TS> c, BoundFlags f, co
     const double kk = (double)k[i];
     double s, y;
     if (stage == 1) {
-        s = (dif ? dif[i] : 0.0) + kk / 6.0;
+        s = (dif ? (double)dif[i] : 0.0) + kk / 6.0;
         y = y0[i] + (dt / 2.0) * kk;
     } else if (stage == 2) {
         s = save[i] + kk / 3.0;
@@ -542,6 +549,39 @@ __global__ __launch_bounds__(256) void k_cast_state(Consts<TS> c, int nv, const 
     y1[i] = (TS)(y0[i] - bg);
 }
 
+
+// Stage of Step_RK3_TVD / Step_RK2_TVD (step_rk4.f90:166-335, :31-162) with Bound:
+//   mode 0 (stage 1)  y = y0 + dtc k
+//   mode 1            y = a0 y0 + a1 ys + dtc k      (ys = the previous stage state, FP64)
+//   last stage        y = y + dt dif                  (SLDiffusion's Euler diffusion)
+// Intermediate stages keep y in ys (FP64) and its stage-precision copy in y1; the last writes y0.
+template <typename TS>
+__global__ __launch_bounds__(256) void k_tvd_stage(Consts<TS> c, BoundFlags f, const double *__restrict__ sr1_64,
+                                                   const double *__restrict__ rbar64,
+                                                   const double *__restrict__ sbar64, int mode, double a0,
+                                                   double a1, double dtc, double dt, int last,
+                                                   double *__restrict__ y0, double *__restrict__ ys,
+                                                   const TS *__restrict__ k, const TS *__restrict__ dif,
+                                                   TS *__restrict__ y1, int nv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N * nv) return;
+    const int iv = (int)(i / c.N);
+    const long j = i - (long)iv * c.N;
+    int it, ip, ir;
+    coords(j, c, it, ip, ir);
+    const double kk = (double)k[i];
+    double y = (mode == 0) ? y0[i] + dtc * kk : a0 * y0[i] + a1 * ys[i] + dtc * kk;
+    if (last && dif) y = y + dt * (double)dif[i];
+    const double yb = bound_value<double>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_64);
+    const double bg = (sizeof(TS) == sizeof(double)) ? 0.0 : (iv == 0 ? rbar64[ir] : iv == 4 ? sbar64[ir] : 0.0);
+    if (!last) {
+        ys[i] = yb;
+        y1[i] = (TS)(yb - bg);
+    } else {
+        y0[i] = yb;
+        if (sizeof(TS) != sizeof(double)) y1[i] = (TS)(yb - bg);
+    }
+}
 
 // ---------------------------------------------------------------------------------------------
 // Compute_Step_Size (rhs_mpi.f90:145-185, stratified branch): per-block minima of the sound/flow
@@ -585,6 +625,9 @@ __global__ __launch_bounds__(256) void k_cfl(Consts<double> c, Prof<double> P, c
 }
 
 }  // namespace cssx
+
+#include "cssx_sld.hpp"
+
 // =============================================================================================
 // Host orchestration
 // =============================================================================================
@@ -614,7 +657,7 @@ template <typename T>
 static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &drbar,
                            const std::vector<double> &dsbar) {
     const int nr = m->nr, nth = m->nth;
-    const size_t tot = 10 * (size_t)nr + 7 * (size_t)nth;
+    const size_t tot = 14 * (size_t)nr + 10 * (size_t)nth + 2;
     std::vector<T> h(tot);
     size_t o = 0;
     auto put = [&](size_t n, auto gen) {
@@ -644,6 +687,29 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &d
     const size_t o_d2 = put(nth, [&](size_t i) { return m->h_dsdr2[i]; });
     // FP32 states hold s - s_bar: the constant-temperature wall value is stored the same way
     const size_t o_s1 = put(nth, [&](size_t i) { return m->h_sr1[i] - (f32 ? m->h_s_bar[0] : 0.0); });
+    // SLD geometry: cell faces at the midpoints, the wall faces half a cell out
+    const std::vector<double> &r = m->h_r, &dr = m->h_dr, &th = m->h_theta;
+    auto r_lo = [&](size_t i) { return i == 0 ? r[0] - 0.5 * dr[0] : 0.5 * (r[i] + r[i - 1]); };
+    auto r_hi = [&](size_t i) { return i + 1 == (size_t)nr ? r[nr - 1] + 0.5 * dr[nr - 1] : 0.5 * (r[i + 1] + r[i]); };
+    auto t_lo = [&](size_t i) { return i == 0 ? th[0] - 0.5 * m->dth : 0.5 * (th[i] + th[i - 1]); };
+    auto t_hi = [&](size_t i) { return i + 1 == (size_t)nth ? th[nth - 1] + 0.5 * m->dth : 0.5 * (th[i + 1] + th[i]); };
+    const size_t o_rfc = put(nr + 1, [&](size_t k) { return (k == 0 || k == (size_t)nr) ? 0.0 : 0.5 * (r[k] + r[k - 1]); });
+    const size_t o_rv = put(nr, [&](size_t i) {
+        const double a = r_lo(i), b = r_hi(i);
+        return 3.0 / (b * b * b - a * a * a);
+    });
+    const size_t o_sph = put(nth + 1, [&](size_t k) {
+        return (k == 0 || k == (size_t)nth) ? 0.0 : std::sin(0.5 * (th[k] + th[k - 1]));
+    });
+    const size_t o_rfac = put(nr, [&](size_t i) {
+        const double a = r_lo(i), b = r_hi(i);
+        return 1.5 * (b * b - a * a) / (b * b * b - a * a * a);
+    });
+    const size_t o_ct = put(nth, [&](size_t i) { return 1.0 / (std::cos(t_lo(i)) - std::cos(t_hi(i))); });
+    const size_t o_gt = put(nth, [&](size_t i) {
+        return (t_hi(i) - t_lo(i)) / (std::cos(t_lo(i)) - std::cos(t_hi(i)));
+    });
+    const size_t o_vsp = put(nr, [&](size_t i) { return m->h_diff_speed.empty() ? 0.0 : m->h_diff_speed[i + 1]; });
     T *d = nullptr;
     if (P.buf) {
         d = P.buf;
@@ -656,6 +722,8 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &d
     P.rbar = d + o_rb; P.sbar = d + o_sb; P.tbar = d + o_tb; P.drbar = d + o_drb; P.dsbar = d + o_dsb;
     P.sn = d + o_sn; P.csc = d + o_csc; P.cs = d + o_cs; P.cot = d + o_cot;
     P.dsdr1 = d + o_d1; P.dsdr2 = d + o_d2; P.sr1 = d + o_s1;
+    P.rfc = d + o_rfc; P.rv = d + o_rv; P.sph = d + o_sph; P.rfac = d + o_rfac; P.ct = d + o_ct; P.gt = d + o_gt;
+    P.vsp = d + o_vsp;
     return CSSX_OK;
 }
 
@@ -708,6 +776,7 @@ static int ensure_scratch(cssx_mhd *m) {
     if (hipMalloc(&S.wall, sizeof(T) * W.total) != hipSuccess) return CSSX_ENOMEM;
     if (hipMalloc(&S.y1, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (hipMalloc(&S.k, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
+    if (m->p.sld_on && hipMalloc(&S.dif, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (hipMemset(S.wall, 0, sizeof(T) * W.total) != hipSuccess) return CSSX_EHIP;
     S.ready = true;
     return CSSX_OK;
@@ -750,7 +819,7 @@ static DerivSpec spec(cssx_mhd *m, int axis, const T *in, T *out, int code, cons
 }
 
 template <typename T>
-static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream) {
+static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream, bool zero_walls = true) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
     Scratch<T> &S = scratch<T>(m);
@@ -773,7 +842,7 @@ static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream) {
     CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, 9.0 * N * sizeof(T),
                hipLaunchKernelGGL(k_curl_combine<T>, dim3(nb), dim3(256), 0, stream, c, P, D, D + N, D + 2 * N,
                                   D + 3 * N, D + 4 * N, D + 5 * N, B, m->p.theta_perf_cond,
-                                  m->p.bottom_radial_field, m->p.top_radial_field));
+                                  m->p.bottom_radial_field, m->p.top_radial_field, (int)zero_walls));
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
 
@@ -852,6 +921,71 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
 }
 
 template <typename T>
+static SldParams<T> make_sld(const cssx_mhd *m) {
+    SldParams<T> sp;
+    const cssx_mhd_params &p = m->p;
+    sp.coef = (T)p.sld_coef;
+    const double tv[8] = {1e-8 * m->h_rho_bar[m->nr - 1], 1e-8, 1e-8, 1e-8, 1e-3, 1.0, 1.0, 1.0};
+    for (int f = 0; f < 8; f++) {
+        sp.tiny[f] = (T)tv[f];
+        const int iv = f + 1;
+        // Diffusion_R scales rho,w,v by sld_pr and s,A by sld_prm (:846-854); Diffusion_Th/Phi
+        // scale rho,w,v,u by sld_pr and A by sld_prm (:1252-1258, :1476-1482)
+        sp.mr[f] = (T)(iv <= 3 ? p.sld_pr : (iv >= 5 ? p.sld_prm : 1.0));
+        sp.mt[f] = (T)(iv < 5 ? p.sld_pr : (iv > 5 ? p.sld_prm : 1.0));
+    }
+    sp.invdphi = (T)(1.0 / m->dphi);
+    sp.gcv = (T)(m->gamm1 * m->Cv);
+    sp.first = p.mass_diffusion ? 0 : 1;
+    sp.last = (p.magnetic && p.magnetic_diffusion) ? 7 : 4;
+    sp.dstype = p.dstype;
+    sp.magnetic = p.magnetic;
+    sp.visc = p.viscous_heating;
+    sp.mass = p.mass_diffusion;
+    sp.ohmic = p.magnetic && p.ohmic_heating && p.magnetic_diffusion;
+    sp.bot_pc = p.bottom_perf_cond;
+    sp.top_pc = p.top_perf_cond;
+    sp.th_normal = p.theta_normal_field;
+    sp.th_pc = p.theta_perf_cond;
+    return sp;
+}
+
+// Compute_Diffusion(1): dif (nv fields) = the SLD tendency of state st, B = curl A with the
+// Compute_B wall conditions (magnetic runs).  dif must not alias the derivative scratch (the
+// Ohmic curl uses slots 0-8, the SLD speed slot 18).
+template <typename T>
+static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream) {
+    int rc = ensure_scratch<T>(m);
+    if (rc) return rc;
+    Scratch<T> &S = scratch<T>(m);
+    Prof<T> &P = prof<T>(m);
+    const Consts<T> c = make_consts<T>(m);
+    const SldParams<T> sp = make_sld<T>(m);
+    const long N = m->N;
+    T *vcc = S.D + 18 * N;
+    const unsigned nb = nblocks(N, 256);
+    const int nv = m->nv;
+    const double ts = (double)sizeof(T);
+    const double b_speed = ts * N * (1.0 + (sp.dstype == 1 ? 3.0 + (sp.magnetic ? 4.0 : 0.0) : 0.0));
+    CSSX_TIMED(CSSX_PHASE_SLD, stream, b_speed,
+               hipLaunchKernelGGL(k_sld_speed<T>, dim3(nb), dim3(256), 0, stream, c, P, sp, st, B, vcc));
+    const double nd = (double)(sp.last - sp.first + 1);
+    CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (3.0 * nd + (nv - nd)),
+               hipLaunchKernelGGL(k_sld_flux<T>, dim3(nb, nv), dim3(256), 0, stream, c, P, sp, st,
+                                  (const T *)vcc, dif));
+    T *C = nullptr;
+    if (sp.ohmic) {
+        C = S.D + 6 * N;
+        if ((rc = curl_impl<T>(m, dif + 5 * N, C, stream, false))) return rc;
+    }
+    const double b_fin = ts * N * (2.0 + 4.0 + 4.0 + (sp.mass ? 1.0 + 3.0 : 0.0) + (C ? 6.0 : 0.0) + 1.0);
+    CSSX_TIMED(CSSX_PHASE_SLD, stream, b_fin,
+               hipLaunchKernelGGL(k_sld_finish<T>, dim3(nb), dim3(256), 0, stream, c, P, sp, st, B,
+                                  (const T *)C, dif));
+    return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+template <typename T>
 static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t stream) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
@@ -861,7 +995,8 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
     const BoundFlags f = make_bound_flags(m);
     const long NV = m->N * m->nv;
     const unsigned nb = nblocks(NV, 256);
-    if (m->p.sld_on && m->p.sld_coef != 0.0) return CSSX_EINVAL;   // SLD fluxes: not in this build yet
+    const bool sld = m->p.sld_on != 0;
+    T *dif = sld ? S.dif : nullptr;
     const bool f32 = sizeof(T) != sizeof(double);
     // algorithmic bytes of one stage update: read k (+ save after stage 1) and y0, write save
     // (stages 1-3) and the new stage state (y1, or y0 and its FP32 copy at stage 4)
@@ -876,17 +1011,61 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
         // stage 1 reads vars1 = vars0: the FP64 run reads y0 directly, the FP32 run its cast
         const T *in1 = f32 ? S.y1 : (const T *)y0;
         if ((rc = rhs_impl<T>(m, in1, S.k, nullptr, stream))) return rc;
+        // Compute_Diffusion(1) on vars1 with B as RHS(1) leaves it (step_rk4.f90:366-383)
+        if (sld && (rc = sld_impl<T>(m, in1, S.B, dif, stream))) return rc;
         CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(1),
                    hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, (const double *)m->p64.sr1,
                                       (const double *)m->p64.rbar, (const double *)m->p64.sbar, 1, dt, y0, m->save,
-                                      (const T *)S.k, (const double *)nullptr, S.y1, m->nv));
+                                      (const T *)S.k, (const T *)dif, S.y1, m->nv));
         for (int stage = 2; stage <= 4; stage++) {
             if ((rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
             CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
                        hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f,
                                           (const double *)m->p64.sr1, (const double *)m->p64.rbar,
                                           (const double *)m->p64.sbar, stage, dt, y0, m->save, (const T *)S.k,
-                                          (const double *)nullptr, S.y1, m->nv));
+                                          (const T *)nullptr, S.y1, m->nv));
+        }
+    }
+    return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+// nsteps of Step_RK3_TVD (order 3) or Step_RK2_TVD (order 2)
+template <typename T>
+static int tvd_impl(cssx_mhd *m, double *y0, double dt, int nsteps, int order, hipStream_t stream) {
+    int rc = ensure_scratch<T>(m);
+    if (rc) return rc;
+    Scratch<T> &S = scratch<T>(m);
+    const Consts<T> c = make_consts<T>(m);
+    const BoundFlags f = make_bound_flags(m);
+    const long NV = m->N * m->nv;
+    const unsigned nb = nblocks(NV, 256);
+    const bool f32 = sizeof(T) != sizeof(double);
+    T *dif = m->p.sld_on ? S.dif : nullptr;
+    const double ts = (double)sizeof(T);
+    auto stage = [&](int mode, double a0, double a1, double dtc, int last) {
+        const double by = (ts + 8.0 + (mode ? 8.0 : 0.0) + (last && dif ? ts : 0.0) +
+                           (last ? 8.0 + (f32 ? ts : 0.0) : 8.0 + ts)) * (double)NV;
+        CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, by,
+                   hipLaunchKernelGGL(k_tvd_stage<T>, dim3(nb), dim3(256), 0, stream, c, f,
+                                      (const double *)m->p64.sr1, (const double *)m->p64.rbar,
+                                      (const double *)m->p64.sbar, mode, a0, a1, dtc, dt, last, y0, m->save,
+                                      (const T *)S.k, (const T *)dif, S.y1, m->nv));
+    };
+    if (f32)
+        hipLaunchKernelGGL(k_cast_state<T>, dim3(nb), dim3(256), 0, stream, c, m->nv, (const double *)m->p64.rbar,
+                           (const double *)m->p64.sbar, (const double *)y0, S.y1);
+    for (int step = 0; step < nsteps; step++) {
+        const T *in1 = f32 ? S.y1 : (const T *)y0;
+        if ((rc = rhs_impl<T>(m, in1, S.k, nullptr, stream))) return rc;
+        if (dif && (rc = sld_impl<T>(m, in1, S.B, dif, stream))) return rc;
+        stage(0, 1.0, 0.0, dt, 0);
+        if ((rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
+        if (order == 2) {
+            stage(1, 0.5, 0.5, 0.5 * dt, 1);
+        } else {
+            stage(1, 0.75, 0.25, dt / 4.0, 0);
+            if ((rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
+            stage(1, 1.0 / 3.0, 2.0 / 3.0, 2.0 * dt / 3.0, 1);
         }
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
@@ -965,7 +1144,8 @@ extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
     if (!m) return;
     for (void *p : {(void *)m->p64.buf, (void *)m->p32.buf, (void *)m->s64.D, (void *)m->s64.B, (void *)m->s64.wall,
                     (void *)m->s64.y1, (void *)m->s64.k, (void *)m->s32.D, (void *)m->s32.B, (void *)m->s32.wall,
-                    (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red})
+                    (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red, (void *)m->s64.dif,
+                    (void *)m->s32.dif})
         if (p) (void)hipFree(p);
     delete m;
 }
@@ -998,6 +1178,41 @@ extern "C" int cssx_rk4(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, 
     if (!m || !y0 || nsteps < 0 || !(dt > 0.0)) return CSSX_EINVAL;
     if (prec == CSSX_PREC_F64) return rk4_impl<double>(m, (double *)y0, dt, nsteps, (hipStream_t)stream);
     if (prec == CSSX_PREC_F32) return rk4_impl<float>(m, (double *)y0, dt, nsteps, (hipStream_t)stream);
+    return CSSX_EINVAL;
+}
+
+template <typename T>
+static int diffusion_entry(cssx_mhd *m, const T *st, const T *b, T *dif, hipStream_t stream) {
+    int rc = ensure_scratch<T>(m);
+    if (rc) return rc;
+    Scratch<T> &S = scratch<T>(m);
+    const T *B = b;
+    if (m->p.magnetic && !B) {
+        if ((rc = curl_impl<T>(m, st + 5 * m->N, S.B, stream))) return rc;
+        B = S.B;
+    }
+    return sld_impl<T>(m, st, B, dif, stream);
+}
+
+extern "C" int cssx_diffusion(cssx_mhd *m, int prec, const void *state, const void *b, void *dif, void *stream) {
+    if (!m || !state || !dif) return CSSX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (prec == CSSX_PREC_F64) return diffusion_entry<double>(m, (const double *)state, (const double *)b, (double *)dif, st);
+    if (prec == CSSX_PREC_F32) return diffusion_entry<float>(m, (const float *)state, (const float *)b, (float *)dif, st);
+    return CSSX_EINVAL;
+}
+
+extern "C" int cssx_rk3_tvd(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream) {
+    if (!m || !y0 || nsteps < 0 || !(dt > 0.0)) return CSSX_EINVAL;
+    if (prec == CSSX_PREC_F64) return tvd_impl<double>(m, (double *)y0, dt, nsteps, 3, (hipStream_t)stream);
+    if (prec == CSSX_PREC_F32) return tvd_impl<float>(m, (double *)y0, dt, nsteps, 3, (hipStream_t)stream);
+    return CSSX_EINVAL;
+}
+
+extern "C" int cssx_rk2_tvd(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream) {
+    if (!m || !y0 || nsteps < 0 || !(dt > 0.0)) return CSSX_EINVAL;
+    if (prec == CSSX_PREC_F64) return tvd_impl<double>(m, (double *)y0, dt, nsteps, 2, (hipStream_t)stream);
+    if (prec == CSSX_PREC_F32) return tvd_impl<float>(m, (double *)y0, dt, nsteps, 2, (hipStream_t)stream);
     return CSSX_EINVAL;
 }
 

@@ -19,7 +19,7 @@ CSSX_EHIP = 100
 PREC_F64 = 0
 PREC_F32 = 1
 AXIS_R, AXIS_THETA, AXIS_PHI = 0, 1, 2
-PHASES = ("deriv_r", "deriv_theta", "deriv_phi", "pointwise", "rk_update", "other")
+PHASES = ("deriv_r", "deriv_theta", "deriv_phi", "pointwise", "rk_update", "other", "sld")
 
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -57,6 +57,9 @@ SIGNATURES = {
     "cssx_bound": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "cssx_rk4": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
     "cssx_cfl": ([_vp, _vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _vp], ctypes.c_int),
+    "cssx_diffusion": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cssx_rk3_tvd": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
+    "cssx_rk2_tvd": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
     "cssx_timing_enable": ([ctypes.c_int], ctypes.c_int),
     "cssx_timing_read": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long),
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),

@@ -91,18 +91,36 @@ class Solver:
         """FP32 fluctuation-form state -> FP64 full state."""
         return (y32.double() + self.background(y32.device)).contiguous()
 
+    def diffusion(self, state, b=None, out=None):
+        """Compute_Diffusion(1): the SLD tendency of a state (A in slots 6-8; float32 states in
+        fluctuation form).  b = curl A as RHS(1) leaves it; computed when not given."""
+        self._check_state(state)
+        if out is None:
+            out = torch.empty_like(state)
+        if b is not None and (tuple(b.shape) != (3,) + self.shape[1:] or b.dtype != state.dtype
+                              or not b.is_contiguous()):
+            raise ValueError("b must be a contiguous (3, nr, nphi, nth) tensor of the state's dtype")
+        rc = _lib.lib().cssx_diffusion(self._h, self._prec(state), state.data_ptr(),
+                                       b.data_ptr() if b is not None else None, out.data_ptr(), _stream())
+        _lib.check(rc, "cssx_diffusion")
+        return out
+
     def bound(self, state):
         """Bound(): wall conditions applied in place."""
         self._check_state(state)
         _lib.check(_lib.lib().cssx_bound(self._h, self._prec(state), state.data_ptr(), _stream()), "cssx_bound")
         return state
 
-    def rk4(self, y0, dt, nsteps=1, precision="f64"):
+    def rk4(self, y0, dt, nsteps=1, precision="f64", scheme="rk4"):
         """nsteps of Step_RK4 on the FP64 state y0 (in place).  precision='f32' runs the stages,
-        RHS and diffusion in FP32 with FP64 accumulation."""
+        RHS and diffusion in FP32 with FP64 accumulation.  scheme 'rk3_tvd' / 'rk2_tvd' runs
+        Step_RK3_TVD / Step_RK2_TVD instead."""
         self._check_state(y0, torch.float64)
+        if precision not in ("f64", "f32"):
+            raise ValueError("precision must be 'f64' or 'f32'")
         prec = _lib.PREC_F64 if precision == "f64" else _lib.PREC_F32
-        _lib.check(_lib.lib().cssx_rk4(self._h, prec, y0.data_ptr(), float(dt), int(nsteps), _stream()), "cssx_rk4")
+        name = {"rk4": "cssx_rk4", "rk3_tvd": "cssx_rk3_tvd", "rk2_tvd": "cssx_rk2_tvd"}[scheme]
+        _lib.check(getattr(_lib.lib(), name)(self._h, prec, y0.data_ptr(), float(dt), int(nsteps), _stream()), name)
         return y0
 
     def cfl(self, y0, cfl_safety=None):

@@ -155,6 +155,23 @@ int cssx_bound(cssx_mhd *m, int prec, void *state, void *stream);
  * stay FP64. */
 int cssx_rk4(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream);
 
+/* nsteps of Step_RK3_TVD (step_rk4.f90:166-335: y1 = y0 + dt k1, y2 = 3/4 y0 + 1/4 y1 + dt/4 k2,
+ * y0 <- 1/3 y0 + 2/3 y2 + 2dt/3 k3 [+ dt D]) and of Step_RK2_TVD (:31-162: y1 = y0 + dt k1,
+ * y0 <- 1/2 y0 + 1/2 y1 + dt/2 k2 [+ dt D]), Bound after every stage; D = the SLD tendency of
+ * y0 when SLDiffusion.  prec as for cssx_rk4 (FP32 stages keep FP64 copies of the stage states
+ * for the combinations). */
+int cssx_rk3_tvd(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream);
+int cssx_rk2_tvd(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream);
+
+/* Compute_Diffusion(1) (diffusion.f90:70-474 with Diffusion_R/Th/Phi :476-1495): the SLD
+ * tendency dif (nv fields) of a state (A in slots 6-8; FP32 in fluctuation form): Van Albada
+ * limited second-order fluxes per variable and direction at speed sld_coef*sqrt(vcc), zero
+ * flux through the walls, then viscous heating, mass diffusion and Ohmic heating.
+ *   b  optional B = curl A (3 fields) as RHS(1) leaves it in vars1(6:8); NULL computes it.
+ * Unlike the reference, the state is not modified (it subtracts the background in place and
+ * overwrites the shared temperature array, diffusion.f90:122, :168-175). */
+int cssx_diffusion(cssx_mhd *m, int prec, const void *state, const void *b, void *dif, void *stream);
+
 /* Compute_Step_Size (rhs_mpi.f90:92-275): dt = cfl_safety * min over points of the sound,
  * flow and Alfven crossing times.  Synchronizes the stream; *dt_out on the host. */
 int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream);
@@ -172,7 +189,8 @@ int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out,
 #define CSSX_PHASE_POINTWISE 3     /* curl assembly, wall values, RHS terms */
 #define CSSX_PHASE_RK_UPDATE 4     /* stage update + Bound */
 #define CSSX_PHASE_OTHER 5
-#define CSSX_NPHASE 6
+#define CSSX_PHASE_SLD 6           /* SLD diffusion: speed, fluxes, corrections (not its curl) */
+#define CSSX_NPHASE 7
 int cssx_timing_enable(int on);
 int cssx_timing_read(double *ms, long *count, double *bytes);
 

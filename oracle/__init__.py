@@ -49,6 +49,8 @@ def lib():
         L.orc_bound.argtypes = [_vp, _vp, _vp, _dp]
         L.orc_rk4.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_int]
         L.orc_diffusion.argtypes = [_vp, _vp, _vp, _dp, _dp, _dp]
+        L.orc_rk2_tvd.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_int]
+        L.orc_rk3_tvd.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_int]
     return _lib
 
 
@@ -180,12 +182,13 @@ def diffusion(model, state, B=None):
     return dv
 
 
-def rk4(model, state, dt, nsteps):
-    """nsteps of Step_RK4 (step_rk4.f90:339-493) on a copy of the state."""
+def rk4(model, state, dt, nsteps, scheme="rk4"):
+    """nsteps of Step_RK4 (step_rk4.f90:339-493), or of Step_RK3_TVD (:166-335) / Step_RK2_TVD
+    (:31-162) for scheme 'rk3_tvd' / 'rk2_tvd', on a copy of the state."""
     g, p, pr = _structs(model)
     st = _f64(state).copy()
-    rc = lib().orc_rk4(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st), float(dt),
-                       int(nsteps))
+    fn = {"rk4": lib().orc_rk4, "rk3_tvd": lib().orc_rk3_tvd, "rk2_tvd": lib().orc_rk2_tvd}[scheme]
+    rc = fn(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st), float(dt), int(nsteps))
     if rc:
-        raise ValueError(f"orc_rk4 rc={rc}")
+        raise ValueError(f"{scheme} rc={rc}")
     return st

@@ -16,7 +16,14 @@
  *   Induction_Terms        rhs_mpi.f90:2298-2366 (u x B; the div B diagnostic is separate)
  *   wall ghost values      rhs_mpi.f90:410-455, :578-637
  *   Bound                  rhs_mpi.f90:2395-2491
- *   Step_RK4               step_rk4.f90:339-493 (SLDiffusion branch, T3)
+ *   Step_RK4               step_rk4.f90:339-493 (SLDiffusion branch: Euler diffusion in stage 1)
+ *   Step_RK3_TVD           step_rk4.f90:166-335
+ *   Step_RK2_TVD           step_rk4.f90:31-162
+ *   Compute_Diffusion      diffusion.f90:70-474, Diffusion_R/Th/Phi :476-1495 (SLD)
+ * Parity status: the RHS is pinned by a manufactured solution, hydrostatic balance and wall
+ * invariants; the diffusion by conservation, the minmod identity, continuum geometry and the
+ * corrections' energy/momentum identities, except its wall ghost rules (parity unpinned: only
+ * the Fortran could pin them); the steppers by their order of accuracy.
  * The subdomain has physical walls at both r ends and both theta ends; each wall is handled as
  * the rank touching it would (the Fortran's If (myr1==1) ... ElseIf (myr2==nr) chains need
  * two ranks per direction to visit both walls; here both are applied).
@@ -861,5 +868,86 @@ int orc_rk4(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
         memcpy(y0, y1, sizeof(double) * NV);
     }
     free(y1); free(k); free(save); free(dd); free(B);
+    return rc;
+}
+
+/* Stage 1 shared by the TVD schemes: k1 = RHS(1)(y0), the Euler diffusion D of y0 (with B as
+ * RHS(1) leaves it) when SLDiffusion, and y1 = Bound(y0 + dt k1). */
+static int tvd_first_stage(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr,
+                           const double *y0, double dt, long NV, long N, double *y1, double *k, double *D,
+                           double *B, double *tmp)
+{
+    long i;
+    int rc = 0;
+    memcpy(y1, y0, sizeof(double) * NV);
+    rc |= orc_rhs(g, p, pr, y1, k, B);
+    if (p->sld_on) {
+        memcpy(tmp, y1, sizeof(double) * NV);
+        if (p->magnetic) memcpy(tmp + 5 * N, B, sizeof(double) * 3 * N);
+        rc |= orc_diffusion(g, p, pr, tmp, y1 + 5 * N, D);
+    }
+    for (i = 0; i < NV; i++) y1[i] = y0[i] + dt * k[i];
+    orc_bound(g, p, pr, y1);
+    return rc;
+}
+
+/* nsteps of Step_RK2_TVD (step_rk4.f90:31-162): y1 = y0 + dt k1;
+ * y = y0/2 + y1/2 + (dt/2) k2 (+ dt D). */
+int orc_rk2_tvd(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, double *y0, double dt,
+                int nsteps)
+{
+    const int nv = p->magnetic ? 8 : 5;
+    const long N = (long)g->nr * g->nth * g->nphi, NV = nv * N;
+    double *y1 = (double *)malloc(sizeof(double) * NV), *k = (double *)malloc(sizeof(double) * NV);
+    double *D = (double *)calloc(NV, sizeof(double)), *tmp = (double *)malloc(sizeof(double) * NV);
+    double *B = (double *)malloc(sizeof(double) * 3 * N);
+    long i;
+    int step, rc = 0;
+    if (!y1 || !k || !D || !tmp || !B) return 12;
+    for (step = 0; step < nsteps && !rc; step++) {
+        double dtcur;
+        rc |= tvd_first_stage(g, p, pr, y0, dt, NV, N, y1, k, D, B, tmp);
+        dtcur = 0.5 * dt;
+        rc |= orc_rhs(g, p, pr, y1, k, NULL);
+        for (i = 0; i < NV; i++) y1[i] = 0.5 * y0[i] + 0.5 * y1[i] + dtcur * k[i];
+        if (p->sld_on)
+            for (i = 0; i < NV; i++) y1[i] = y1[i] + dt * D[i];
+        orc_bound(g, p, pr, y1);
+        memcpy(y0, y1, sizeof(double) * NV);
+    }
+    free(y1); free(k); free(D); free(tmp); free(B);
+    return rc;
+}
+
+/* nsteps of Step_RK3_TVD (step_rk4.f90:166-335): y1 = y0 + dt k1;
+ * y2 = 3/4 y0 + 1/4 y1 + (dt/4) k2; y = 1/3 y0 + 2/3 y2 + (2dt/3) k3 (+ dt D). */
+int orc_rk3_tvd(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, double *y0, double dt,
+                int nsteps)
+{
+    const int nv = p->magnetic ? 8 : 5;
+    const long N = (long)g->nr * g->nth * g->nphi, NV = nv * N;
+    const double onethird = 1.0 / 3.0, twothirds = 2.0 / 3.0;
+    double *y1 = (double *)malloc(sizeof(double) * NV), *k = (double *)malloc(sizeof(double) * NV);
+    double *D = (double *)calloc(NV, sizeof(double)), *tmp = (double *)malloc(sizeof(double) * NV);
+    double *B = (double *)malloc(sizeof(double) * 3 * N);
+    long i;
+    int step, rc = 0;
+    if (!y1 || !k || !D || !tmp || !B) return 12;
+    for (step = 0; step < nsteps && !rc; step++) {
+        double dtcur;
+        rc |= tvd_first_stage(g, p, pr, y0, dt, NV, N, y1, k, D, B, tmp);
+        dtcur = dt / 4.0;
+        rc |= orc_rhs(g, p, pr, y1, k, NULL);
+        for (i = 0; i < NV; i++) y1[i] = 0.75 * y0[i] + 0.25 * y1[i] + dtcur * k[i];
+        orc_bound(g, p, pr, y1);
+        dtcur = 2.0 * dt / 3.0;
+        rc |= orc_rhs(g, p, pr, y1, k, NULL);
+        for (i = 0; i < NV; i++) y1[i] = onethird * y0[i] + twothirds * y1[i] + dtcur * k[i];
+        if (p->sld_on)
+            for (i = 0; i < NV; i++) y1[i] = y1[i] + dt * D[i];
+        orc_bound(g, p, pr, y1);
+        memcpy(y0, y1, sizeof(double) * NV);
+    }
+    free(y1); free(k); free(D); free(tmp); free(B);
     return rc;
 }

@@ -77,8 +77,20 @@ def test_rhs_f32_near_background(mhd, shape):
     assert _rel_fields(got64, ref).max() < 1e-10
     got32 = sol.rhs(sol.to_f32(x)).double().cpu().numpy()
     e32 = _rel_fields(got32, ref)
-    print("f32 near-background rel err per field", e32)
-    assert e32.max() < 2e-5, e32
+    # what rounding the input alone to the FP32 fluctuation form does to the exact RHS: the
+    # density tendency near the stretched bottom wall carries rho_bar (20x the top value) times
+    # the one-sided wall derivative of u_r, so it is ~1e-4 sensitive to that rounding
+    bg = sol.background("cpu").numpy()
+    xr = (st - bg).astype(np.float32).astype(np.float64) + bg
+    cond = _rel_fields(oracle.rhs(m, xr), ref)
+    print("f32 near-background rel err per field", e32, "input-rounding sensitivity", cond)
+    assert e32[0] < 20 * cond[0] + 1e-6, (e32, cond)
+    # u_r: gravity and the radial pressure gradient are each ~g = GM/r^2 and cancel to the
+    # tendency; FP32 arithmetic leaves eps32 * g per term
+    eps32 = float(np.finfo(np.float32).eps)
+    assert e32[3] < 64 * eps32 * m.gravity.max() / np.abs(ref[3]).max(), e32
+    others = [1, 2, 4, 5, 6, 7]
+    assert e32[others].max() < 2e-6, e32
     sol.close()
 
 

@@ -14,7 +14,8 @@ No reference test touches the diffusion, so the oracle is pinned by what the sch
     rho u and temperature unchanged (:395-428), viscous heating turns the kinetic energy the
     velocity diffusion removes into heat rho T ds (:375-393), Ohmic heating is
     -(1/2) B . curl(dA/dt) / (4 pi rho T) (:430-461, the curl checked against the RHS's B);
-  * Step_RK4's SLDiffusion branch adds exactly dt * D to the step (step_rk4.f90:366-383).
+  * the SLDiffusion branch of Step_RK4 / Step_RK3_TVD / Step_RK2_TVD adds exactly dt * D to
+    the step (step_rk4.f90:366-383, :197-214, :60-77).
 The wall ghost rules of Diffusion_R/Th (diffusion.f90:586-711, 1008-1120) are restated, not
 pinned by a closed form: only the Fortran could pin them (parity unpinned).
 """
@@ -216,15 +217,18 @@ def test_ohmic_heating_against_curl_of_da():
     assert np.abs(got[sl] - expect[sl]).max() < 1e-10 * np.abs(expect[sl]).max()
 
 
-def test_rk4_adds_dt_times_diffusion():
+@pytest.mark.parametrize("scheme", ["rk4", "rk3_tvd", "rk2_tvd"])
+def test_step_adds_dt_times_diffusion(scheme):
+    """The stage states never see D: each scheme's step moves by exactly dt * D(y0) more
+    (step_rk4.f90:366-383, :197-214 with :318-323, :60-77 with :135-140)."""
     kw = dict(**SMALL)
     m_sld = Model(Config(**kw, sld_coef=0.3))
     m_0 = Model(Config(**kw, sld_coef=0.0))
     st = _state(m_sld, seed=5)
     dt = 2e-4
     D = oracle.diffusion(m_sld, st)
-    y1 = oracle.rk4(m_sld, st, dt, 1)
-    y0 = oracle.rk4(m_0, st, dt, 1)
+    y1 = oracle.rk4(m_sld, st, dt, 1, scheme)
+    y0 = oracle.rk4(m_0, st, dt, 1, scheme)
     sl = (slice(None), slice(1, -1), slice(None), slice(1, -1))       # off the rows Bound sets
     diff = (y1 - y0)[sl]
     want = dt * D[sl]

@@ -124,3 +124,18 @@ def test_rk4_fourth_order_in_time():
     y4 = oracle.rk4(m, st, dt / 4, 8)
     ratio = np.abs(y1 - y4).max() / np.abs(y2 - y4).max()
     assert 14 < ratio < 20, ratio
+
+
+@pytest.mark.parametrize("scheme,expect", [("rk3_tvd", 9.0), ("rk2_tvd", 5.0)])
+def test_tvd_schemes_order_in_time(scheme, expect):
+    """Step_RK3_TVD / Step_RK2_TVD: Richardson ratio (1-4^-p)/(2^-p-4^-p) = 9 (p=3), 5 (p=2)."""
+    c = Config(nr=20, nth=16, nphi=16, omega0=0.3, damping_time=1e9)
+    m = Model(c)
+    st, ex, terms, Bex = mms.evaluate(m, amp=0.2)
+    st = oracle.bound(m, st)
+    dt = 4e-4
+    y1 = oracle.rk4(m, st, dt, 2, scheme)
+    y2 = oracle.rk4(m, st, dt / 2, 4, scheme)
+    y4 = oracle.rk4(m, st, dt / 4, 8, scheme)
+    ratio = np.abs(y1 - y4).max() / np.abs(y2 - y4).max()
+    assert abs(ratio / expect - 1) < 0.15, ratio
