@@ -91,6 +91,32 @@ def cpu_baseline(sld_coef, seconds_budget=20.0):
                       f"recipe, {steps} step(s) in {el:.1f} s, OpenMP threads={threads}"}
 
 
+def precision_study(sol, y0, dt, steps, nv, every=10):
+    """FP64 and FP32-bulk runs from the same state; per field, the max and RMS of |y32 - y64|
+    relative to max|y64| and RMS(y64), every `every` steps."""
+    import torch
+    y64 = y0.clone()
+    y32 = y0.clone()
+    rows = []
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        sol.rk4(y64, dt, k, "f64")
+        sol.rk4(y32, dt, k, "f32")
+        done += k
+        mx, rms = [], []
+        for iv in range(nv):
+            d = y32[iv] - y64[iv]
+            mx.append(d.abs().max().item() / max(y64[iv].abs().max().item(), 1e-300))
+            rms.append(d.pow(2).mean().sqrt().item() / max(y64[iv].pow(2).mean().sqrt().item(), 1e-300))
+        rows.append({"step": done, "max_rel": [float(f"{x:.3e}") for x in mx],
+                     "rms_rel": [float(f"{x:.3e}") for x in rms]})
+    torch.cuda.synchronize()
+    worst = max(max(r["max_rel"]) for r in rows)
+    return {"steps": steps, "fields": ["rho", "w", "v", "u", "s", "A_theta", "A_phi", "A_r"][:nv],
+            "max_rel_err_over_run": worst, "meets_1e-5": worst <= 1e-5, "every": rows}
+
+
 def _clean(o):
     """NaN/inf (a precision path that was not run) -> null, so the line stays strict JSON."""
     if isinstance(o, dict):
@@ -114,6 +140,8 @@ def main():
     ap.add_argument("--sld-coef", type=float, default=0.1,
                     help="SLD diffusion coefficient (SLDiffusion is on; 0 keeps its cost with a zero tendency)")
     ap.add_argument("--paths", default="f64,f32", help="precisions to run: f64,f32 | f32 | f64")
+    ap.add_argument("--study-steps", type=int, default=100,
+                    help="FP32-vs-FP64 precision study length (untimed; errors every 10 steps; 0 = off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -169,6 +197,10 @@ def main():
                "rms_rel_err_state": max(rms_state), "per_field_max_rel_err": [float(f"{x:.3e}") for x in rel_state],
                "finite": finite}
 
+    study = None
+    if args.study_steps > 0 and len(paths) == 2:
+        study = precision_study(sol, y0, dt, args.study_steps, c.nv)
+
     prof_steps = max(2, min(args.steps, 5))
     tab32 = profile_phases(sol, y0, dt, prof_steps, "f32") if "f32" in paths else None
     tab64 = profile_phases(sol, y0, dt, prof_steps, "f64") if "f64" in paths else None
@@ -215,6 +247,7 @@ def main():
         "fp64": {"value": round(val64, 2), "unit": "Mpt-steps/s", "ms_per_step": round(ms64, 4)},
         "fp32_speedup_vs_fp64": round(ms64 / ms32, 3) if len(paths) == 2 else None,
         "fp32_vs_fp64": cmp,
+        "precision_study": study,
         "hbm_tbs_algorithmic": {"f32": round(val32 * 1e6 * STEP_BYTES_MIXED / 1e12, 3),
                                 "f64": round(val64 * 1e6 * STEP_BYTES_F64 / 1e12, 3)},
         "phases_ms_per_step": {p: ({k: round(v[0] / prof_steps, 4) for k, v in t.items()} if t else None)

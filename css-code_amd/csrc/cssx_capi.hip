@@ -38,8 +38,8 @@ int get_plan(const PlanKey &key, int prec, const void **out) {
         if (MR < 0) return CSSX_EINVAL;
         build_plan_contig(key, MR, plan);
     } else {
-        const int M = choose_M(key.n);
-        if (M < 0) return CSSX_EINVAL;
+        const int M = key.seg;
+        if (M <= 0 || seg_count(key.n, M) > kMaxSeg) return CSSX_EINVAL;
         build_plan(key, seg_count(key.n, M), plan);
     }
     void *d = nullptr;
@@ -97,7 +97,8 @@ static int lanes_per_pencil(int n) {
 template <typename T>
 static void fill_launch_geom(DerivLaunch<T> &L, const AxisGeom &g) {
     L.n = g.n;
-    L.S = seg_count(g.n, choose_M(g.n));
+    L.M = strided_M(g.n);
+    L.S = seg_count(g.n, L.M);
     L.na = g.na; L.nb = g.nb; L.sa = g.sa; L.sb = g.sb; L.srow = g.srow;
     L.njobs = 0;
 }
@@ -114,6 +115,7 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
         PlanKey key{g.n, s.order, s.lo_kind == END_WALL ? END_WALL : END_HALO,
                     s.hi_kind == END_WALL ? END_WALL : END_HALO, (s.lo_kind == END_WALL || s.hi_kind == END_WALL) ? s.bdry : 0};
         key.variant = contig ? 1 : 0;
+        key.seg = contig ? 0 : L.M;
         const void *plan = nullptr;
         int rc = get_plan(key, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
         if (rc) return rc;
@@ -196,6 +198,7 @@ extern "C" int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out) {
                         for (int var = 0; var < 2; var++) {
                             if (var == 1 && (ax != CSSX_AXIS_THETA || choose_MR(ns[ax]) < 0)) continue;
                             key.variant = var;
+                            key.seg = var == 0 ? strided_M(ns[ax]) : 0;
                             for (int prec = 0; prec < 2; prec++) {
                                 int rc = get_plan(key, prec, &p);
                                 if (rc) {
@@ -289,7 +292,7 @@ extern "C" int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int 
     int S;
     if (variant == 0) {
         if (choose_M(n) < 0) return CSSX_EINVAL;
-        S = seg_count(n, choose_M(n));
+        S = seg_count(n, strided_M(n));
         build_plan(key, S, plan);
     } else {
         if (choose_MR(n) < 0) return CSSX_EINVAL;

@@ -46,6 +46,9 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
     const int lo_kind = J.lo_kind, hi_kind = J.hi_kind;
     const long pb_ld = (long)(active ? a : 0) * L.sa + (long)b * L.sb;
     const long srow = L.srow;
+    // Interior rows read 2 neighbours on each side; only the closure rows of the first and
+    // last segment (halo rows, derivatives.f90:684-720) reach 4 rows out.
+    const bool edge = (k == 0) || (k == S - 1);
     T w[W];
 #pragma unroll
     for (int i = 0; i < W; i++) {
@@ -53,16 +56,15 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
         int qc;
         if (lo_kind == END_PERIODIC) qc = q < 0 ? q + n : (q >= n ? q - n : q);
         else qc = q < 0 ? 0 : (q >= n ? n - 1 : q);
-        w[i] = in[pb_ld + (long)qc * srow];
-    }
-    if (pre) {
-#pragma unroll
-        for (int i = 0; i < W; i++) {
-            const int q = st - 4 + i;
-            int qc;
-            if (lo_kind == END_PERIODIC) qc = q < 0 ? q + n : (q >= n ? q - n : q);
-            else qc = q < 0 ? 0 : (q >= n ? n - 1 : q);
-            w[i] *= pre[qc];
+        if (i >= 2 && i < W - 2) {
+            w[i] = in[pb_ld + (long)qc * srow];
+            if (pre) w[i] *= pre[qc];
+        } else {
+            w[i] = T(0);
+            if (edge) {
+                w[i] = in[pb_ld + (long)qc * srow];
+                if (pre) w[i] *= pre[qc];
+            }
         }
     }
     if (k == 0 && lo_kind != END_PERIODIC) {
@@ -237,8 +239,10 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
 
 template <typename T>
 hipError_t launch_deriv(const DerivLaunch<T> &L, int order, hipStream_t stream) {
-    const int M = choose_M(L.n);
-    if (M < 0 || L.S != seg_count(L.n, M) || L.njobs < 1 || L.njobs > kMaxJobs) return hipErrorInvalidValue;
+    const int M = L.M;
+    if ((M != 8 && M != 16 && M != 32 && M != 64) || L.S != seg_count(L.n, M) || L.S > kMaxSeg || L.njobs < 1 ||
+        L.njobs > kMaxJobs)
+        return hipErrorInvalidValue;
     dim3 grid((L.na + 63) / 64, L.nb, L.njobs), block(64 * L.S);
 #define CSSX_LAUNCH(MM)                                                                              \
     do {                                                                                             \

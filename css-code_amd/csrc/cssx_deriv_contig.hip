@@ -20,8 +20,27 @@ __device__ __forceinline__ T shfl(T v, int src) {
     return __shfl(v, src, 64);
 }
 
+// MR consecutive values as 16-byte vector accesses (p 16-byte aligned)
+template <typename T, int MR>
+__device__ __forceinline__ void load_run(const T *__restrict__ p, T (&v)[MR]) {
+    static_assert((MR * sizeof(T)) % 16 == 0, "run must be whole 16-byte vectors");
+    constexpr int NC = MR * (int)sizeof(T) / 16;
+    uint4 t[NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++) t[c] = reinterpret_cast<const uint4 *>(p)[c];
+    __builtin_memcpy(v, t, sizeof(v));
+}
+template <typename T, int MR>
+__device__ __forceinline__ void store_run(T *__restrict__ p, const T (&v)[MR]) {
+    constexpr int NC = MR * (int)sizeof(T) / 16;
+    uint4 t[NC];
+    __builtin_memcpy(t, v, sizeof(v));
+#pragma unroll
+    for (int c = 0; c < NC; c++) reinterpret_cast<uint4 *>(p)[c] = t[c];
+}
+
 template <typename T, int ORDER, int MR>
-__global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
+__global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     extern __shared__ unsigned char smem_raw[];
     T *sp = reinterpret_cast<T *>(smem_raw);
     const DerivJob<T> &J = L.jobs[blockIdx.z];
@@ -29,6 +48,9 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
     const int plen = plan_contig_len(n);
     for (int i = threadIdx.x; i < plen; i += blockDim.x) sp[i] = J.plan[i];
     __syncthreads();
+    // unit-stride pencils whose length is a multiple of MR move as 16-byte vectors
+    const bool vec = L.srow == 1 && n % MR == 0;
+    const int b = blockIdx.y;
     const T *cf = sp, *cl = cf + n, *cu = cl + n, *vL = cu + n, *vR = vL + n;
     const T *rl = sp + 5 * n, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
             *invb = k2 + kPcrSteps * kMaxLanes;
@@ -41,7 +63,6 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
     const int j = lane & (Lw - 1);
     const int ppw = 64 / Lw;
     const long a = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * ppw + lane / Lw;
-    const int b = blockIdx.y;
     const bool pen_ok = a < L.na;
     const bool active = pen_ok && j < nl;
     const int st = j * MR;
@@ -53,10 +74,14 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
     const T *__restrict__ in = J.in;
     const long pb_ld = (pen_ok ? a : 0) * L.sa + (long)b * L.sb;
     T v[MR];
+    if (vec) {
+        if (j < nl) load_run<T, MR>(in + pb_ld + st, v);
+    } else {
 #pragma unroll
-    for (int k = 0; k < MR; k++) {
-        const int q = min(st + k, n - 1);
-        v[k] = in[pb_ld + (long)q * L.srow];
+        for (int k = 0; k < MR; k++) {
+            const int q = min(st + k, n - 1);
+            v[k] = in[pb_ld + (long)q * L.srow];
+        }
     }
     if (J.pre) {
 #pragma unroll
@@ -195,7 +220,7 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
     const T xr = xI;
 
     // ---- correct and store ------------------------------------------------------------------------
-    if (!active) return;
+    if (active) {
     T *__restrict__ out = J.out;
     const T *__restrict__ post = J.post;
     const T scale = J.scale, beta = J.beta;
@@ -212,9 +237,14 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
         for (int jj = 0; jj < MR; jj++)
             if (jj < m) o[jj] += beta * out[pbase + (long)(st + jj) * L.srow];
     }
+    if (vec) {
+        store_run<T, MR>(out + pbase + st, o);
+    } else {
 #pragma unroll
-    for (int jj = 0; jj < MR; jj++)
-        if (jj < m) out[pbase + (long)(st + jj) * L.srow] = o[jj];
+        for (int jj = 0; jj < MR; jj++)
+            if (jj < m) out[pbase + (long)(st + jj) * L.srow] = o[jj];
+    }
+    }   // active
 }
 
 template <typename T>
@@ -223,8 +253,10 @@ hipError_t launch_deriv_contig(const DerivLaunch<T> &L, int order, hipStream_t s
     if (MR < 0 || L.njobs < 1 || L.njobs > kMaxJobs) return hipErrorInvalidValue;
     const int Lw = L.S;
     const int ppw = 64 / Lw;
-    const int wpb = 4;
-    dim3 grid((unsigned)((L.na + (long)ppw * wpb - 1) / ((long)ppw * wpb)), L.nb, L.njobs), block(64 * wpb);
+    // 16 wavefronts per workgroup share one staging of the plan (5n + 960 values)
+    const int wpb = 16;
+    const unsigned gx = (unsigned)((L.na + (long)ppw * wpb - 1) / ((long)ppw * wpb));
+    dim3 grid(gx, L.nb, L.njobs), block(64 * wpb);
     const size_t shm = (size_t)plan_contig_len(L.n) * sizeof(T);
 #define CSSX_LAUNCH_C(MM)                                                                                \
     do {                                                                                                 \

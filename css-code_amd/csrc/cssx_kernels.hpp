@@ -33,6 +33,7 @@ struct DerivJob {
 template <typename T>
 struct DerivLaunch {
     int n, S;           // pencil length, segments per pencil (contiguous kernel: lanes per pencil)
+    int M;              // strided kernel: rows-per-segment template size (S = ceil(n/M))
     int na, nb;         // pencils: fast index a (lanes) x slow index b (grid.y)
     long sa, sb, srow;  // element strides of a, b and along the pencil
     int njobs;

@@ -2,8 +2,19 @@
 #include "cssx_plan.hpp"
 
 #include <cmath>
+#include <cstdlib>
 
 namespace cssx {
+
+int strided_M(int n) {
+    static const int forced = [] {
+        const char *e = std::getenv("CSSX_DERIV_M");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
+    }();
+    if (forced && seg_count(n, forced) <= kMaxSeg && n >= forced) return forced;
+    return choose_M(n);
+}
 
 void lhs_coefficients(int order, int n, int lo, int hi, int bdry, std::vector<double> &l,
                       std::vector<double> &u) {

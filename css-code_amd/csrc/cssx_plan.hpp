@@ -24,8 +24,10 @@ enum EndKind : int { END_WALL = 0, END_HALO = 1, END_PERIODIC = 2 };
 struct PlanKey {
     int n, order, lo, hi, bdry;
     int variant = 0;   // 0: strided (waves = segments), 1: contiguous (lanes = segments)
+    int seg = 0;       // strided: rows-per-segment size M the plan is cut for
     bool operator<(const PlanKey &o) const {
         if (variant != o.variant) return variant < o.variant;
+        if (seg != o.seg) return seg < o.seg;
         if (n != o.n) return n < o.n;
         if (order != o.order) return order < o.order;
         if (lo != o.lo) return lo < o.lo;
@@ -64,6 +66,12 @@ inline int choose_M(int n) {
     if (n <= 16 * 64) return 64;
     return -1;
 }
+
+// Rows-per-segment size the strided kernel uses for a pencil of n rows: choose_M, unless the
+// environment variable CSSX_DERIV_M (8, 16, 32 or 64) names a size that fits (S <= kMaxSeg);
+// read once per process.  A tuning hook for the segment length, which trades the serial
+// sweep length against the halo rows re-read per segment.
+int strided_M(int n);
 
 // Tridiagonal coefficients l_j, u_j (diagonal 1) of the compact system.
 //   order 1: alpha = 1/3, wall rows derivatives.f90:108-125 (row 0 upper = 4 for codes 1,3

@@ -70,6 +70,8 @@ struct Scratch {
     T *y1 = nullptr;     // RK stage state (nv fields)
     T *k = nullptr;      // RK stage tendency (nv fields)
     T *dif = nullptr;    // SLD diffusion tendency of stage 1 (nv fields, SLDiffusion runs)
+    T *acc = nullptr;    // FP32 runs: RK4 tendency accumulator (nv fields)
+    T *y0f = nullptr;    // FP32 runs: the FP64 state y0 in stage precision, fluctuation form
     bool ready = false;
 };
 
@@ -550,6 +552,45 @@ __global__ __launch_bounds__(256) void k_cast_state(Consts<TS> c, int nv, const 
 }
 
 
+// RK4 stage of the mixed-precision path (step_rk4.f90:339-493): stage states, tendencies and the
+// tendency accumulator acc = D + k1/6 + k2/3 + k3/3 + k4/6 in FP32, the state y0 in FP64.
+// Stages 1-3 form y1 = y0f + c dt k from the FP32 copy y0f of y0 (fluctuation form) and
+// apply Bound in that form; stage 4 forms y0 + dt acc in FP64, applies Bound there and
+// refreshes y0f.  Per value: 16-20 bytes at stage 1, 20 at stages 2-3, 28 at stage 4.
+__global__ __launch_bounds__(256) void k_rk_stage_mixed(Consts<float> c, BoundFlags f,
+                                                        const double *__restrict__ sr1_64,
+                                                        const float *__restrict__ sr1_32,
+                                                        const double *__restrict__ rbar64,
+                                                        const double *__restrict__ sbar64, int stage, double dt,
+                                                        double *__restrict__ y0, float *__restrict__ y0f,
+                                                        float *__restrict__ acc, const float *__restrict__ k,
+                                                        const float *__restrict__ dif, float *__restrict__ y1,
+                                                        int nv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N * nv) return;
+    const int iv = (int)(i / c.N);
+    const long j = i - (long)iv * c.N;
+    int it, ip, ir;
+    coords(j, c, it, ip, ir);
+    const float kk = k[i];
+    if (stage < 4) {
+        float a;
+        if (stage == 1) a = (dif ? dif[i] : 0.0f) + kk / 6.0f;
+        else a = acc[i] + kk / 3.0f;
+        const float h = (float)(stage == 3 ? dt : 0.5 * dt);
+        const float y = y0f[i] + h * kk;
+        acc[i] = a;
+        y1[i] = bound_value<float>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_32);
+    } else {
+        const float a = acc[i] + kk / 6.0f;
+        const double y = y0[i] + dt * (double)a;
+        const double yb = bound_value<double>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_64);
+        const double bg = iv == 0 ? rbar64[ir] : iv == 4 ? sbar64[ir] : 0.0;
+        y0[i] = yb;
+        y0f[i] = (float)(yb - bg);
+    }
+}
+
 // Stage of Step_RK3_TVD / Step_RK2_TVD (step_rk4.f90:166-335, :31-162) with Bound:
 //   mode 0 (stage 1)  y = y0 + dtc k
 //   mode 1            y = a0 y0 + a1 ys + dtc k      (ys = the previous stage state, FP64)
@@ -777,6 +818,9 @@ static int ensure_scratch(cssx_mhd *m) {
     if (hipMalloc(&S.y1, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (hipMalloc(&S.k, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (m->p.sld_on && hipMalloc(&S.dif, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
+    if (sizeof(T) == 4 && (hipMalloc(&S.acc, sizeof(T) * N * m->nv) != hipSuccess ||
+                           hipMalloc(&S.y0f, sizeof(T) * N * m->nv) != hipSuccess))
+        return CSSX_ENOMEM;
     if (hipMemset(S.wall, 0, sizeof(T) * W.total) != hipSuccess) return CSSX_EHIP;
     S.ready = true;
     return CSSX_OK;
@@ -952,7 +996,7 @@ static SldParams<T> make_sld(const cssx_mhd *m) {
 
 // Compute_Diffusion(1): dif (nv fields) = the SLD tendency of state st, B = curl A with the
 // Compute_B wall conditions (magnetic runs).  dif must not alias the derivative scratch (the
-// Ohmic curl uses slots 0-8, the SLD speed slot 18).
+// Ohmic curl uses slots 0-8, the face speeds 15-17, the SLD speed slot 18).
 template <typename T>
 static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream) {
     int rc = ensure_scratch<T>(m);
@@ -969,10 +1013,13 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
     const double b_speed = ts * N * (1.0 + (sp.dstype == 1 ? 3.0 + (sp.magnetic ? 4.0 : 0.0) : 0.0));
     CSSX_TIMED(CSSX_PHASE_SLD, stream, b_speed,
                hipLaunchKernelGGL(k_sld_speed<T>, dim3(nb), dim3(256), 0, stream, c, P, sp, st, B, vcc));
+    T *vt = S.D + 15 * N;   // face speeds, 3 fields (slots 15-17)
+    CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * 4.0,
+               hipLaunchKernelGGL(k_sld_faces<T>, dim3(nb), dim3(256), 0, stream, c, sp, (const T *)vcc, vt));
     const double nd = (double)(sp.last - sp.first + 1);
-    CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (3.0 * nd + (nv - nd)),
+    CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (2.0 * nd + 3.0 + (nv - nd)),
                hipLaunchKernelGGL(k_sld_flux<T>, dim3(nb, nv), dim3(256), 0, stream, c, P, sp, st,
-                                  (const T *)vcc, dif));
+                                  (const T *)vt, dif));
     T *C = nullptr;
     if (sp.ohmic) {
         C = S.D + 6 * N;
@@ -990,40 +1037,51 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
     Scratch<T> &S = scratch<T>(m);
-    Prof<T> &P = prof<T>(m);
     const Consts<T> c = make_consts<T>(m);
     const BoundFlags f = make_bound_flags(m);
     const long NV = m->N * m->nv;
     const unsigned nb = nblocks(NV, 256);
     const bool sld = m->p.sld_on != 0;
     T *dif = sld ? S.dif : nullptr;
-    const bool f32 = sizeof(T) != sizeof(double);
-    // algorithmic bytes of one stage update: read k (+ save after stage 1) and y0, write save
-    // (stages 1-3) and the new stage state (y1, or y0 and its FP32 copy at stage 4)
-    auto rk_bytes = [&](int stage) {
-        double b = sizeof(T) + 8.0 + (stage > 1 ? 8.0 : 0.0) + (stage < 4 ? 8.0 + sizeof(T) : 8.0 + (f32 ? 4.0 : 0.0));
-        return b * (double)NV;
-    };
-    if (f32)
+    if constexpr (sizeof(T) == sizeof(double)) {
+        // algorithmic bytes of one stage update: read k (+ save after stage 1, + D at stage 1)
+        // and y0, write save and y1 (stages 1-3) or y0 (stage 4)
+        auto rk_bytes = [&](int stage) {
+            return (8.0 * (2 + (stage > 1) + (stage == 1 && sld)) + (stage < 4 ? 16.0 : 8.0)) * (double)NV;
+        };
+        for (int step = 0; step < nsteps; step++) {
+            if ((rc = rhs_impl<T>(m, (const T *)y0, S.k, nullptr, stream))) return rc;
+            // Compute_Diffusion(1) on vars1 with B as RHS(1) leaves it (step_rk4.f90:366-383)
+            if (sld && (rc = sld_impl<T>(m, (const T *)y0, S.B, dif, stream))) return rc;
+            for (int stage = 1; stage <= 4; stage++) {
+                if (stage > 1 && (rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
+                CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
+                           hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f,
+                                              (const double *)m->p64.sr1, (const double *)m->p64.rbar,
+                                              (const double *)m->p64.sbar, stage, dt, y0, m->save, (const T *)S.k,
+                                              (const T *)(stage == 1 ? dif : nullptr), S.y1, m->nv));
+            }
+        }
+    } else {
+        auto rk_bytes = [&](int stage) {
+            const double b = stage == 1 ? 16.0 + (sld ? 4.0 : 0.0) : stage < 4 ? 20.0 : 28.0;
+            return b * (double)NV;
+        };
         hipLaunchKernelGGL(k_cast_state<T>, dim3(nb), dim3(256), 0, stream, c, m->nv, (const double *)m->p64.rbar,
-                           (const double *)m->p64.sbar, (const double *)y0, S.y1);
-    for (int step = 0; step < nsteps; step++) {
-        // stage 1 reads vars1 = vars0: the FP64 run reads y0 directly, the FP32 run its cast
-        const T *in1 = f32 ? S.y1 : (const T *)y0;
-        if ((rc = rhs_impl<T>(m, in1, S.k, nullptr, stream))) return rc;
-        // Compute_Diffusion(1) on vars1 with B as RHS(1) leaves it (step_rk4.f90:366-383)
-        if (sld && (rc = sld_impl<T>(m, in1, S.B, dif, stream))) return rc;
-        CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(1),
-                   hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f, (const double *)m->p64.sr1,
-                                      (const double *)m->p64.rbar, (const double *)m->p64.sbar, 1, dt, y0, m->save,
-                                      (const T *)S.k, (const T *)dif, S.y1, m->nv));
-        for (int stage = 2; stage <= 4; stage++) {
-            if ((rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
-            CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
-                       hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f,
-                                          (const double *)m->p64.sr1, (const double *)m->p64.rbar,
-                                          (const double *)m->p64.sbar, stage, dt, y0, m->save, (const T *)S.k,
-                                          (const T *)nullptr, S.y1, m->nv));
+                           (const double *)m->p64.sbar, (const double *)y0, S.y0f);
+        for (int step = 0; step < nsteps; step++) {
+            // stage 1 reads vars1 = vars0 through its FP32 copy
+            if ((rc = rhs_impl<T>(m, S.y0f, S.k, nullptr, stream))) return rc;
+            if (sld && (rc = sld_impl<T>(m, S.y0f, S.B, dif, stream))) return rc;
+            for (int stage = 1; stage <= 4; stage++) {
+                if (stage > 1 && (rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
+                CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
+                           hipLaunchKernelGGL(k_rk_stage_mixed, dim3(nb), dim3(256), 0, stream, c, f,
+                                              (const double *)m->p64.sr1, (const float *)m->p32.sr1,
+                                              (const double *)m->p64.rbar, (const double *)m->p64.sbar, stage, dt,
+                                              y0, S.y0f, S.acc, (const float *)S.k,
+                                              (const float *)(stage == 1 ? dif : nullptr), S.y1, m->nv));
+            }
         }
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
@@ -1145,7 +1203,7 @@ extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
     for (void *p : {(void *)m->p64.buf, (void *)m->p32.buf, (void *)m->s64.D, (void *)m->s64.B, (void *)m->s64.wall,
                     (void *)m->s64.y1, (void *)m->s64.k, (void *)m->s32.D, (void *)m->s32.B, (void *)m->s32.wall,
                     (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red, (void *)m->s64.dif,
-                    (void *)m->s32.dif})
+                    (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f})
         if (p) (void)hipFree(p);
     delete m;
 }

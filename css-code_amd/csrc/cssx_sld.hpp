@@ -3,6 +3,7 @@
 //
 // Three launches plus the Ohmic curl:
 //   k_sld_speed   vcc = speed^2(r) [+ |u|^2 + B^2/(4 pi rho) for dstype 1]   (:177-302)
+//   k_sld_faces   the face speeds sld_coef*sqrt(mean vcc) of every cell's lower faces
 //   k_sld_flux    one field per blockIdx.y: the six face fluxes of every cell in the three
 //                 directions, Van Albada limited, summed in the reference's order r, theta, phi
 //   (curl of the A tendency, the same batched derivative sweeps as Compute_B without its wall
@@ -29,11 +30,18 @@ struct SldParams {
     int bot_pc, top_pc, th_normal, th_pc;
 };
 
+// 1/x: IEEE division in FP64 (the reference's arithmetic), the hardware reciprocal (1 ulp) in
+// the FP32 path
+template <typename T>
+__device__ __forceinline__ T recip(T x) { return T(1) / x; }
+template <>
+__device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Van Albada slope of diffusion.f90:744-762, operation for operation
 template <typename T>
 __device__ __forceinline__ T van_albada(T q, T qph, T tiny) {
     T rt = fabs(q);
-    rt = T(1) / fmax(rt, tiny);
+    rt = recip(fmax(rt, tiny));
     rt = q * rt * rt;
     rt = rt * qph;
     rt = fmin(T(1), rt);
@@ -71,6 +79,27 @@ __global__ __launch_bounds__(256) void k_sld_speed(Consts<T> c, Prof<T> P, SldPa
         }
     }
     vcc[i] = v;
+}
+
+// face speeds sld_coef*sqrt((vcc_a + vcc_b)/2) of the lower r, theta and phi face of every
+// cell (diffusion.f90:836, :1246, :1466); ghost cells copy the nearest cell (:290-302), phi
+// wraps.  The wall faces get a finite placeholder: their flux weight is zero.
+template <typename T>
+__global__ __launch_bounds__(256) void k_sld_faces(Consts<T> c, SldParams<T> sp, const T *__restrict__ vcc,
+                                                   T *__restrict__ vt) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    int it, ip, ir;
+    coords(i, c, it, ip, ir);
+    const long plane = (long)c.nth * c.nphi;
+    const T h = T(0.5), vc = vcc[i];
+    const T vrm = vcc[ir > 0 ? i - plane : i];
+    const T vtm = vcc[it > 0 ? i - 1 : i];
+    const int ipm = ip > 0 ? ip - 1 : c.nphi - 1;
+    const T vpm = vcc[i + (long)(ipm - ip) * c.nth];
+    vt[i] = sp.coef * sqrt(h * (vc + vrm));
+    vt[c.N + i] = sp.coef * sqrt(h * (vc + vtm));
+    vt[2 * c.N + i] = sp.coef * sqrt(h * (vpm + vc));
 }
 
 // value of field f at a point of radius index ir: rho and s relative to the background
@@ -133,7 +162,7 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_sld_flux(Consts<T> c, Prof<T> P, SldParams<T> sp,
-                                                  const T *__restrict__ st, const T *__restrict__ vcc,
+                                                  const T *__restrict__ st, const T *__restrict__ vt,
                                                   T *__restrict__ D) {
     const long lb = xcd_block(blockIdx.x, gridDim.x);
     const long i = lb * blockDim.x + threadIdx.x;
@@ -152,13 +181,12 @@ __global__ __launch_bounds__(256) void k_sld_flux(Consts<T> c, Prof<T> P, SldPar
     const long row = i - it;                               // (ip, ir) row of the theta pencil
     const T *fp = st + (long)f * c.N;
     const T tiny = sp.tiny[f];
-    // ---- face speeds sld_coef*sqrt((vcc_a + vcc_b)/2); ghost speeds copy the nearest cell ----
-    const T vc = vcc[i];
-    const T vrm = vcc[ir > 0 ? i - plane : i], vrp = vcc[ir < nr - 1 ? i + plane : i];
-    const T vtm = vcc[it > 0 ? i - 1 : i], vtp = vcc[it < nth - 1 ? i + 1 : i];
-    const int ipm = ip > 0 ? ip - 1 : nphi - 1, ipp = ip < nphi - 1 ? ip + 1 : 0;
-    const T vpm = vcc[i + (long)(ipm - ip) * nth], vpp = vcc[i + (long)(ipp - ip) * nth];
-    const T h = T(0.5);
+    // ---- face speeds: lower face of this cell, lower face of the next cell ----
+    const long N = c.N;
+    const int ipp = ip < nphi - 1 ? ip + 1 : 0;
+    const T sr_l = vt[i], sr_h = vt[ir < nr - 1 ? i + plane : i];
+    const T st_l = vt[N + i], st_h = vt[N + (it < nth - 1 ? i + 1 : i)];
+    const T sp_l = vt[2 * N + i], sp_h = vt[2 * N + i + (long)(ipp - ip) * nth];
     // ---- r (Diffusion_R :745-858) ----
     T Dr;
     {
@@ -179,8 +207,8 @@ __global__ __launch_bounds__(256) void k_sld_flux(Consts<T> c, Prof<T> P, SldPar
             if (ir == nr - 1) w[4] = g;
         }
         const T rl = P.rfc[ir], rh = P.rfc[ir + 1];
-        const T Fl = (sp.coef * sqrt(h * (vc + vrm))) * (sld_delue(w[0], w[1], w[2], w[3], tiny) * rl * rl);
-        const T Fh = (sp.coef * sqrt(h * (vrp + vc))) * (sld_delue(w[1], w[2], w[3], w[4], tiny) * rh * rh);
+        const T Fl = sr_l * (sld_delue(w[0], w[1], w[2], w[3], tiny) * rl * rl);
+        const T Fh = sr_h * (sld_delue(w[1], w[2], w[3], w[4], tiny) * rh * rh);
         Dr = sp.mr[f] * (P.rv[ir] * (Fh - Fl));
     }
     // ---- theta (Diffusion_Th :1122-1262) ----
@@ -202,8 +230,8 @@ __global__ __launch_bounds__(256) void k_sld_flux(Consts<T> c, Prof<T> P, SldPar
             w[nth + 2 - it] = g;
             if (it == nth - 1) w[4] = g;
         }
-        const T Fl = (sp.coef * sqrt(h * (vc + vtm)) * P.sph[it]) * sld_delue(w[0], w[1], w[2], w[3], tiny);
-        const T Fh = (sp.coef * sqrt(h * (vtp + vc)) * P.sph[it + 1]) * sld_delue(w[1], w[2], w[3], w[4], tiny);
+        const T Fl = (st_l * P.sph[it]) * sld_delue(w[0], w[1], w[2], w[3], tiny);
+        const T Fh = (st_h * P.sph[it + 1]) * sld_delue(w[1], w[2], w[3], w[4], tiny);
         Dt = sp.mt[f] * ((P.rfac[ir] * P.ct[it]) * (Fh - Fl));
     }
     // ---- phi, periodic (Diffusion_Phi :1362-1494) ----
@@ -216,8 +244,8 @@ __global__ __launch_bounds__(256) void k_sld_flux(Consts<T> c, Prof<T> P, SldPar
             j = j < 0 ? j + nphi : (j >= nphi ? j - nphi : j);
             w[d + 2] = sld_val(c, P, fp, f, i + (long)(j - ip) * nth, ir);
         }
-        const T Fl = (sp.coef * sqrt(h * (vpm + vc))) * sld_delue(w[0], w[1], w[2], w[3], tiny);
-        const T Fh = (sp.coef * sqrt(h * (vc + vpp))) * sld_delue(w[1], w[2], w[3], w[4], tiny);
+        const T Fl = sp_l * sld_delue(w[0], w[1], w[2], w[3], tiny);
+        const T Fh = sp_h * sld_delue(w[1], w[2], w[3], w[4], tiny);
         const T geom = P.gt[it] * (P.rfac[ir] * sp.invdphi);
         Dp = geom * (sp.mt[f] * (Fh - Fl));
     }
