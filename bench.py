@@ -134,9 +134,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--nr", type=int, default=128)
-    ap.add_argument("--nth", type=int, default=256)
-    ap.add_argument("--nphi", type=int, default=512)
+    ap.add_argument("--config", choices=["c2", "c4"], default="c2",
+                    help="c2: one 128x256x512 subdomain (the headline); c4: the 256x512x1024 precision study")
+    ap.add_argument("--nr", type=int, default=0)
+    ap.add_argument("--nth", type=int, default=0)
+    ap.add_argument("--nphi", type=int, default=0)
     ap.add_argument("--sld-coef", type=float, default=0.1,
                     help="SLD diffusion coefficient (SLDiffusion is on; 0 keeps its cost with a zero tendency)")
     ap.add_argument("--paths", default="f64,f32", help="precisions to run: f64,f32 | f32 | f64")
@@ -145,6 +147,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
+    shape = {"c2": (128, 256, 512), "c4": (256, 512, 1024)}[args.config]
+    args.nr, args.nth, args.nphi = (args.nr or shape[0], args.nth or shape[1], args.nphi or shape[2])
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "decomposition is a design, not a code path", file=sys.stderr)
@@ -166,7 +170,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}))
         return
-    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=args.sld_coef)
+    seeds = {"c2": (2, 3), "c4": (5, 6)}[args.config]     # entropy / vector-potential seeds
+    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=args.sld_coef, seed_s=seeds[0], seed_a=seeds[1])
     m = Model(c)
     y0 = torch.as_tensor(m.initial_state(), device="cuda").contiguous()
     sol = Solver(m)
