@@ -22,9 +22,78 @@
 
 namespace cssx {
 
+// element at a wave-uniform row base plus a 32-bit per-lane byte offset
+template <typename T>
+__device__ __forceinline__ T &at_row(const T *row, unsigned byte_ofs) {
+    return *reinterpret_cast<T *>(const_cast<char *>(reinterpret_cast<const char *>(row)) + byte_ofs);
+}
+
+// Raw buffer access to one field: the lane's byte offset stays in one VGPR for every row and
+// the row's byte offset is a scalar register (buffer_load ... v_lane, s[rsrc], s_row offen),
+// so a sweep row costs one memory instruction and no address arithmetic per lane.  The
+// resource spans 2^32-1 bytes; the host checks that a launch's offsets fit.
+typedef unsigned int cssx_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, -1, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vofs, sofs, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vofs, sofs, 0));
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, vofs, sofs, 0);
+}
+__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(cssx_u32x2, v), r, vofs, sofs, 0);
+}
+
+// v from lane l of the wavefront as a wave-uniform value (v_readlane)
+__device__ __forceinline__ float lane_val(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_val(double v, int l) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// closure rows of the pencil's last two rows; jj = local index of row n-1 in the window
+template <typename T, int ORDER>
+__device__ __forceinline__ void hi_closure(const T *w, int jj, const DerivJob<T> &J, int bd, T hi_val, T &sp_hi0,
+                                           T &sp_hi1) {
+    // window: row n-1 -> w[jj+4], n-2 -> w[jj+3], ..., ghosts n.. -> w[jj+5..]
+    if (J.hi_kind == END_WALL) {
+        if constexpr (ORDER == 1) {
+            T rn2 = -Rows<T, 1>::wall1(w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj]);
+            T val = (bd == 3 || bd == 4) ? hi_val * J.neu_mul : T(0);
+            T rn1 = Rows<T, 1>::wall0(bd == 1 || bd == 2, val, w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], -rn2);
+            sp_hi0 = (bd == 1 || bd == 2) ? -rn1 : rn1;
+            sp_hi1 = rn2;
+        } else {
+            T val = (bd == 0) ? T(0) : hi_val * J.neu_mul;
+            sp_hi0 = Rows<T, 2>::wall0(bd == 1 || bd == 2, val, w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj],
+                                       w[jj - 1]);
+            sp_hi1 = Rows<T, 2>::wall1(w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj]);
+        }
+    } else {
+        if constexpr (ORDER == 1)
+            sp_hi0 = Rows<T, 1>::halo_hi0(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7], w[jj + 8],
+                                          J.halo_s);
+        else
+            sp_hi0 = Rows<T, 2>::halo_hi0(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7], w[jj + 8]);
+    }
+}
+
 template <typename T, int ORDER, int M>
 __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
-    constexpr int W = M + 8;
+    constexpr int W = M + 8;   // window w[i] = row st-4+i
     const DerivJob<T> &J = L.jobs[blockIdx.z];
     const int lane = threadIdx.x & 63;
     const int k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // segment (wavefront) index
@@ -33,156 +102,189 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
     const int b = blockIdx.y;
     const bool active = a < L.na;
     const int st = seg_start(n, S, k);
-    const int m = seg_start(n, S, k + 1) - st;  // rows in this segment
-    const int mi = (k < S - 1) ? m - 1 : m;     // interior rows (last row = interface)
-    const long pbase = (long)a * L.sa + (long)b * L.sb;
-    const long gbase = (long)a * J.g_a + (long)b * J.g_b;
-
-    // ---- 1. load rows st-4 .. st+M+3 ------------------------------------------------------
-    // Every load is issued unconditionally from a clamped (or wrapped) row of a valid pencil, so
-    // the whole window is in flight at once; rows outside the pencil are fixed up afterwards.
-    const T *__restrict__ in = J.in;
-    const T *__restrict__ pre = J.pre;
+    const int m = seg_start(n, S, k + 1) - st;   // rows in this segment
+    const bool first = k == 0, last = k == S - 1;
+    const bool full = m == M;                     // every segment is full when S divides n
     const int lo_kind = J.lo_kind, hi_kind = J.hi_kind;
-    const long pb_ld = (long)(active ? a : 0) * L.sa + (long)b * L.sb;
+    const long gbase = (long)a * J.g_a + (long)b * J.g_b;
+    const unsigned lofs = (unsigned)((active ? a : 0) * L.sa) * (unsigned)sizeof(T);   // lane offset, bytes
+    const long bofs = (long)b * L.sb;                                                   // pencil-row offset
     const long srow = L.srow;
-    // Interior rows read 2 neighbours on each side; only the closure rows of the first and
-    // last segment (halo rows, derivatives.f90:684-720) reach 4 rows out.
-    const bool edge = (k == 0) || (k == S - 1);
+    const unsigned rowb = (unsigned)srow * (unsigned)sizeof(T);                         // row stride, bytes
+    const __amdgpu_buffer_rsrc_t rin = field_rsrc(J.in + bofs);
+    // Plan coefficients: lane j holds those of segment row j (and of interface row j), so the
+    // sweeps read them with v_readlane instead of keeping ~5M of them in scalar registers.
+    const T *cf = J.plan, *cl = cf + n, *cu = cl + n, *vL = cu + n, *vR = vL + n;
+    const T *rl = J.plan + 5 * n, *ru = rl + kMaxSeg, *rf = ru + kMaxSeg, *rcl = rf + kMaxSeg,
+            *rcu = rcl + kMaxSeg;
+    const int crow = min(st + (lane < M ? lane : M - 1), n - 1);
+    const T p_cf = cf[crow], p_cl = cl[crow], p_cu = cu[crow], p_vL = vL[crow], p_vR = vR[crow];
+    const T p_post = J.post ? J.post[crow] : T(1);
+    const int cq = lane < kMaxSeg ? lane : kMaxSeg - 1;
+    const T p_rl = rl[cq], p_ru = ru[cq], p_rf = rf[cq], p_rcl = rcl[cq], p_rcu = rcu[cq];
+    // the row a window slot reads: wrapped on a periodic pencil, clamped otherwise (rows
+    // outside the pencil are replaced below); wave-uniform, so scalar arithmetic
+    auto row_of = [&](int i) {
+        const int q = st - 4 + i;
+        if (lo_kind == END_PERIODIC) return q < 0 ? q + n : (q >= n ? q - n : q);
+        return q < 0 ? 0 : (q >= n ? n - 1 : q);
+    };
+
+    // ---- 1. window ---------------------------------------------------------------------------
+    // Interior rows read 2 neighbours on each side (slots 2 .. M+5); the 4-row reach of the
+    // closure rows (slots 0, 1, M+6, M+7) is loaded for the first and last segment only.
     T w[W];
 #pragma unroll
-    for (int i = 0; i < W; i++) {
-        const int q = st - 4 + i;
-        int qc;
-        if (lo_kind == END_PERIODIC) qc = q < 0 ? q + n : (q >= n ? q - n : q);
-        else qc = q < 0 ? 0 : (q >= n ? n - 1 : q);
-        if (i >= 2 && i < W - 2) {
-            w[i] = in[pb_ld + (long)qc * srow];
-            if (pre) w[i] *= pre[qc];
-        } else {
-            w[i] = T(0);
-            if (edge) {
-                w[i] = in[pb_ld + (long)qc * srow];
-                if (pre) w[i] *= pre[qc];
-            }
-        }
+    for (int i = 2; i < W - 2; i++) w[i] = bload<T>(rin, lofs, (unsigned)row_of(i) * rowb);
+    w[0] = w[1] = w[W - 2] = w[W - 1] = T(0);
+    if (first || last) {
+        w[0] = bload<T>(rin, lofs, (unsigned)row_of(0) * rowb);
+        w[1] = bload<T>(rin, lofs, (unsigned)row_of(1) * rowb);
+        w[W - 2] = bload<T>(rin, lofs, (unsigned)row_of(W - 2) * rowb);
+        w[W - 1] = bload<T>(rin, lofs, (unsigned)row_of(W - 1) * rowb);
     }
-    if (k == 0 && lo_kind != END_PERIODIC) {
+    if (J.pre) {
+#pragma unroll
+        for (int i = 0; i < W; i++) w[i] *= J.pre[row_of(i)];
+    }
+    if (first && lo_kind != END_PERIODIC) {
 #pragma unroll
         for (int i = 0; i < 4; i++)
             w[i] = (lo_kind == END_HALO && active) ? J.glo[gbase + (long)i * J.g_k] : T(0);
     }
-    if (k == S - 1 && hi_kind != END_PERIODIC) {
+    if (last && hi_kind != END_PERIODIC) {
 #pragma unroll
         for (int i = 0; i < W; i++) {
             const int q = st - 4 + i;
             if (q >= n) w[i] = (hi_kind == END_HALO && active && q - n < 4) ? J.ghi[gbase + (long)(q - n) * J.g_k] : T(0);
         }
     }
-    // wall values (physical first derivative prescribed at a Neumann end)
-    T lo_val = T(0), hi_val = T(0);
-    if (active && k == 0 && lo_kind == END_WALL && J.glo) lo_val = J.glo[gbase];
-    if (active && k == S - 1 && hi_kind == END_WALL && J.ghi) hi_val = J.ghi[gbase];
 
-    // ---- 2. closure rows (pencil ends) -------------------------------------------------------
-    const int bd = J.bdry;
-    T sp_lo0 = T(0), sp_lo1 = T(0), sp_hi1 = T(0), sp_hi0 = T(0);
-    const bool lo_w = (k == 0) && J.lo_kind == END_WALL;
-    const bool hi_w = (k == S - 1) && J.hi_kind == END_WALL;
-    if (k == 0) {
-        if (J.lo_kind == END_WALL) {
-            if constexpr (ORDER == 1) {
-                sp_lo1 = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
-                T val = (bd == 0) ? T(0) : lo_val * J.neu_mul;
-                sp_lo0 = Rows<T, 1>::wall0(bd == 1 || bd == 3, val, w[4], w[5], w[6], w[7], sp_lo1,
-                                           J.lo_r0_val != 0);
-            } else {
-                T val = (bd == 0) ? T(0) : -lo_val * J.neu_mul;
-                sp_lo0 = Rows<T, 2>::wall0(bd == 1 || bd == 3, val, w[4], w[5], w[6], w[7], w[8], w[9]);
-                sp_lo1 = Rows<T, 2>::wall1(w[4], w[5], w[6], w[7], w[8]);
-            }
-        } else {
-            if constexpr (ORDER == 1)
-                sp_lo0 = Rows<T, 1>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6], J.halo_s);
-            else
-                sp_lo0 = Rows<T, 2>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
-        }
-    }
-    if (k == S - 1) {
-        // rows n-2 and n-1 sit at local rows m-2, m-1 of the last segment
-#pragma unroll
-        for (int jj = 1; jj < M; jj++) {
-            if (jj == m - 1) {
-                // window: row n-1 -> w[jj+4], n-2 -> w[jj+3], ..., ghosts n.. -> w[jj+5..]
-                if (J.hi_kind == END_WALL) {
-                    if constexpr (ORDER == 1) {
-                        T rn2 = -Rows<T, 1>::wall1(w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj]);
-                        T val = (bd == 3 || bd == 4) ? hi_val * J.neu_mul : T(0);
-                        T rn1 = Rows<T, 1>::wall0(bd == 1 || bd == 2, val, w[jj + 4], w[jj + 3], w[jj + 2],
-                                                  w[jj + 1], -rn2);
-                        sp_hi0 = (bd == 1 || bd == 2) ? -rn1 : rn1;
-                        sp_hi1 = rn2;
-                    } else {
-                        T val = (bd == 0) ? T(0) : hi_val * J.neu_mul;
-                        sp_hi0 = Rows<T, 2>::wall0(bd == 1 || bd == 2, val, w[jj + 4], w[jj + 3], w[jj + 2],
-                                                   w[jj + 1], w[jj], w[jj - 1]);
-                        sp_hi1 = Rows<T, 2>::wall1(w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj]);
-                    }
-                } else {
-                    if constexpr (ORDER == 1)
-                        sp_hi0 = Rows<T, 1>::halo_hi0(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6],
-                                                      w[jj + 7], w[jj + 8], J.halo_s);
-                    else
-                        sp_hi0 = Rows<T, 2>::halo_hi0(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6],
-                                                      w[jj + 7], w[jj + 8]);
-                }
-            }
-        }
-    }
-
-    // ---- 3. right-hand side rows fused with the segment-local forward sweep ---------------------
-    const T *cf = J.plan, *cl = cf + n, *cu = cl + n, *vL = cu + n, *vR = vL + n;
-    const T *rl = J.plan + 5 * n, *ru = rl + kMaxSeg, *rf = ru + kMaxSeg, *rcl = rf + kMaxSeg,
-            *rcu = rcl + kMaxSeg;
+    __builtin_amdgcn_sched_barrier(0);
+    // ---- 2. right-hand side rows: the Lele interior stencil, then the closure rows ----------------
     T d[M];
-    T dI = T(0), y = T(0);
 #pragma unroll
     for (int jj = 0; jj < M; jj++) {
-        // window index of row st+jj is jj+4
-        T v;
         if constexpr (ORDER == 1)
-            v = Rows<T, 1>::interior(w[jj + 2], w[jj + 3], w[jj + 5], w[jj + 6]);
+            d[jj] = Rows<T, 1>::interior(w[jj + 2], w[jj + 3], w[jj + 5], w[jj + 6]);
         else
-            v = Rows<T, 2>::interior(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6]);
-        if (jj == 0 && k == 0) v = sp_lo0;
-        if (jj == 1 && lo_w) v = sp_lo1;
-        if (k == S - 1) {
-            if (jj == m - 1) v = sp_hi0;
-            if (jj == m - 2 && hi_w) v = sp_hi1;
-        }
-        if (J.rhs_out && active && jj < m) J.rhs_out[pbase + (long)(st + jj) * L.srow] = v * J.rhs_scale;
-        if (jj == m - 1) dI = v;
-        if (jj < mi) {
-            y = cf[st + jj] * v - cl[st + jj] * y;
-            d[jj] = y;
-        } else {
-            d[jj] = T(0);
+            d[jj] = Rows<T, 2>::interior(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6]);
+    }
+    const int bd = J.bdry;
+    if (first) {
+        if (lo_kind == END_WALL) {
+            const T lo_val = (active && J.glo) ? J.glo[gbase] : T(0);   // prescribed wall derivative
+            if constexpr (ORDER == 1) {
+                const T r1 = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
+                const T val = (bd == 0) ? T(0) : lo_val * J.neu_mul;
+                d[0] = Rows<T, 1>::wall0(bd == 1 || bd == 3, val, w[4], w[5], w[6], w[7], r1, J.lo_r0_val != 0);
+                d[1] = r1;
+            } else {
+                const T val = (bd == 0) ? T(0) : -lo_val * J.neu_mul;
+                d[0] = Rows<T, 2>::wall0(bd == 1 || bd == 3, val, w[4], w[5], w[6], w[7], w[8], w[9]);
+                d[1] = Rows<T, 2>::wall1(w[4], w[5], w[6], w[7], w[8]);
+            }
+        } else if (lo_kind == END_HALO) {
+            if constexpr (ORDER == 1)
+                d[0] = Rows<T, 1>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6], J.halo_s);
+            else
+                d[0] = Rows<T, 2>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+        } else {   // periodic: the seam row is a halo row with the wrapped rows as ghosts
+            if constexpr (ORDER == 1)
+                d[0] = Rows<T, 1>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6], J.halo_s);
+            else
+                d[0] = Rows<T, 2>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
         }
     }
-    __builtin_amdgcn_sched_barrier(0);
-    T xn = T(0);
+    if (last) {
+        const T hi_val = (active && hi_kind == END_WALL && J.ghi) ? J.ghi[gbase] : T(0);
+        const bool hw = hi_kind == END_WALL;
+        T sp0 = T(0), sp1 = T(0);
+        if (full) {
+            hi_closure<T, ORDER>(w, M - 1, J, bd, hi_val, sp0, sp1);
+            d[M - 1] = sp0;
+            if (hw) d[M - 2] = sp1;
+        } else {
 #pragma unroll
-    for (int jj = M - 1; jj >= 0; jj--) {
-        if (jj < mi) {
-            xn = d[jj] - cu[st + jj] * xn;
+            for (int jj = 1; jj < M; jj++)
+                if (jj == m - 1) hi_closure<T, ORDER>(w, jj, J, bd, hi_val, sp0, sp1);
+#pragma unroll
+            for (int jj = 0; jj < M; jj++) {
+                if (jj == m - 1) d[jj] = sp0;
+                if (hw && jj == m - 2) d[jj] = sp1;
+            }
+        }
+    }
+    if (J.rhs_out && active) {   // CSS-Python compact_fd6 leaves the assembled rows in darr
+#pragma unroll
+        for (int jj = 0; jj < M; jj++)
+            if (jj < m) at_row(J.rhs_out + bofs + (long)(st + jj) * srow, lofs) = d[jj] * J.rhs_scale;
+    }
+
+    // ---- 3. segment-local solve (the last row of every segment but the pencil's last is an
+    //         interface row, solved in step 4) -----------------------------------------------------
+    T dI = T(0), xfirst, xlast = T(0);
+    // scheduling fences keep each phase's coefficient reads (readlanes) next to their use
+    __builtin_amdgcn_sched_barrier(0);
+    if (full && !last) {
+        dI = d[M - 1];
+        T y = T(0);
+#pragma unroll
+        for (int jj = 0; jj < M - 1; jj++) {
+            y = lane_val(p_cf, jj) * d[jj] - lane_val(p_cl, jj) * y;
+            d[jj] = y;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        T xn = T(0);
+#pragma unroll
+        for (int jj = M - 2; jj >= 0; jj--) {
+            xn = d[jj] - lane_val(p_cu, jj) * xn;
             d[jj] = xn;
         }
-    }
-    T xfirst = d[0], xlast = T(0);
+        xlast = d[M - 2];
+    } else if (full) {
+        T y = T(0);
 #pragma unroll
-    for (int jj = 0; jj < M; jj++)
-        if (jj == mi - 1) xlast = d[jj];
+        for (int jj = 0; jj < M; jj++) {
+            y = lane_val(p_cf, jj) * d[jj] - lane_val(p_cl, jj) * y;
+            d[jj] = y;
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        T xn = T(0);
+#pragma unroll
+        for (int jj = M - 1; jj >= 0; jj--) {
+            xn = d[jj] - lane_val(p_cu, jj) * xn;
+            d[jj] = xn;
+        }
+        xlast = d[M - 1];
+    } else {
+        const int mi = last ? m : m - 1;   // rows solved locally
+#pragma unroll
+        for (int jj = 0; jj < M; jj++)
+            if (jj == m - 1) dI = d[jj];
+        T y = T(0);
+#pragma unroll
+        for (int jj = 0; jj < M; jj++) {
+            if (jj < mi) {
+                y = cf[st + jj] * d[jj] - cl[st + jj] * y;
+                d[jj] = y;
+            }
+        }
+        T xn = T(0);
+#pragma unroll
+        for (int jj = M - 1; jj >= 0; jj--) {
+            if (jj < mi) {
+                xn = d[jj] - cu[st + jj] * xn;
+                d[jj] = xn;
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < M; jj++)
+            if (jj == mi - 1) xlast = d[jj];
+    }
+    xfirst = d[0];
 
+    __builtin_amdgcn_sched_barrier(0);
     // ---- 4. interface system ----------------------------------------------------------------
     __shared__ T sh[3][kMaxSeg][64];
     T xl = T(0), xr = T(0);
@@ -196,8 +298,8 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
 #pragma unroll
         for (int q = 0; q < kMaxSeg - 1; q++) {
             if (q < S - 1) {
-                T R = sh[2][q][lane] - rl[q] * sh[1][q][lane] - ru[q] * sh[0][q + 1][lane];
-                zp = rf[q] * R - rcl[q] * zp;
+                T R = sh[2][q][lane] - lane_val(p_rl, q) * sh[1][q][lane] - lane_val(p_ru, q) * sh[0][q + 1][lane];
+                zp = lane_val(p_rf, q) * R - lane_val(p_rcl, q) * zp;
                 z[q] = zp;
             }
         }
@@ -205,7 +307,7 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
 #pragma unroll
         for (int q = kMaxSeg - 2; q >= 0; q--) {
             if (q < S - 1) {
-                xp = z[q] - rcu[q] * xp;
+                xp = z[q] - lane_val(p_rcu, q) * xp;
                 z[q] = xp;
                 if (q == k - 1) xl = xp;
                 if (q == k) xr = xp;
@@ -216,25 +318,34 @@ __global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
     __builtin_amdgcn_sched_barrier(0);
     // ---- 5. correct and store -----------------------------------------------------------------
     if (!active) return;
-    T *__restrict__ out = J.out;
     const T *__restrict__ post = J.post;
     const T scale = J.scale, beta = J.beta;
-    T o[M];
+    const __amdgpu_buffer_rsrc_t rout = field_rsrc(J.out + bofs);
+    if (full) {
 #pragma unroll
-    for (int jj = 0; jj < M; jj++) {
-        const int j = st + jj < n ? st + jj : n - 1;
-        T x = (jj < mi) ? d[jj] + vL[j] * xl + vR[j] * xr : xr;
-        o[jj] = scale * x;
-        if (post) o[jj] *= post[j];
+        for (int jj = 0; jj < M; jj++) {
+            const int j = st + jj;
+            // the interface row (last row of a non-final segment) is the interface value itself
+            const T x = (jj < M - 1 || last) ? d[jj] + lane_val(p_vL, jj) * xl + lane_val(p_vR, jj) * xr : xr;
+            T o = scale * x;
+            if (post) o *= lane_val(p_post, jj);
+            if (beta != T(0)) o += beta * bload<T>(rout, lofs, (unsigned)j * rowb);
+            bstore(o, rout, lofs, (unsigned)j * rowb);
+        }
+    } else {
+        const int mi = last ? m : m - 1;
+#pragma unroll
+        for (int jj = 0; jj < M; jj++) {
+            if (jj < m) {
+                const int j = st + jj;
+                const T x = (jj < mi) ? d[jj] + vL[j] * xl + vR[j] * xr : xr;
+                T o = scale * x;
+                if (post) o *= post[j];
+                if (beta != T(0)) o += beta * bload<T>(rout, lofs, (unsigned)j * rowb);
+                bstore(o, rout, lofs, (unsigned)j * rowb);
+            }
+        }
     }
-    if (beta != T(0)) {
-#pragma unroll
-        for (int jj = 0; jj < M; jj++)
-            if (jj < m) o[jj] += beta * out[pbase + (long)(st + jj) * srow];
-    }
-#pragma unroll
-    for (int jj = 0; jj < M; jj++)
-        if (jj < m) out[pbase + (long)(st + jj) * srow] = o[jj];
 }
 
 template <typename T>
@@ -242,6 +353,9 @@ hipError_t launch_deriv(const DerivLaunch<T> &L, int order, hipStream_t stream) 
     const int M = L.M;
     if ((M != 8 && M != 16 && M != 32 && M != 64) || L.S != seg_count(L.n, M) || L.S > kMaxSeg || L.njobs < 1 ||
         L.njobs > kMaxJobs)
+        return hipErrorInvalidValue;
+    // the kernel addresses a pencil row with 32-bit byte offsets (lane part + row part)
+    if (((double)(L.na - 1) * (double)L.sa + (double)(L.n - 1) * (double)L.srow + 1.0) * sizeof(T) >= 4294967295.0)
         return hipErrorInvalidValue;
     dim3 grid((L.na + 63) / 64, L.nb, L.njobs), block(64 * L.S);
 #define CSSX_LAUNCH(MM)                                                                              \

@@ -45,6 +45,10 @@ struct Prof {
     // hydrostatic background (nr): rho_bar, s_bar, T_bar and the physical radial derivatives
     // of rho_bar and s_bar by the compact operator with zero wall values
     T *rbar = nullptr, *sbar = nullptr, *tbar = nullptr, *drbar = nullptr, *dsbar = nullptr;
+    // well-balanced radial momentum of the fluctuation form: hres = the background's radial
+    // momentum residual -g - pconst (T_bar/rho_bar) drbar - (gamma-1) T_bar dsbar (zero on
+    // Neumann density walls, truncation-small inside), tr = T_bar/rho_bar
+    T *hres = nullptr, *tr = nullptr;
     // SLD face geometry (diffusion.f90:805-838, 1219-1250, 1316-1348): r face radii rfc (nr+1,
     // 0 on the walls so their flux vanishes), 3/(r_h^3-r_l^3) rv (nr), sin of the theta faces
     // sph (nth+1, 0 on the walls), 1.5(r_h^2-r_l^2)/(r_h^3-r_l^3) rfac (nr),
@@ -86,6 +90,9 @@ struct cssx_mhd {
     double dth, dphi;
     std::vector<double> h_r, h_dr, h_dxdr, h_theta, h_grav, h_eps, h_dsdr1, h_dsdr2, h_sr1, h_sr2, h_rho_bar,
         h_s_bar, h_diff_speed;
+    // background: T_bar, compact radial derivatives of rho_bar / s_bar, radial momentum residual
+    std::vector<double> h_tbar, h_drbar, h_dsbar, h_hres;
+    double wbar_lo = 0.0, wbar_hi = 0.0;   // d rho_bar/dr imposed at Neumann density walls
     cssx::Prof<double> p64;
     cssx::Prof<float> p32;
     cssx::Scratch<double> s64;
@@ -107,6 +114,8 @@ struct Consts {
     T gamm1, invtconst, invCv, pconst, invpconst, Cv, omega0, om2, xi2, gamma, T0, damp;
     int magnetic, top_relax, xi_on;
     int fluct;   // state slots 1 and 5 hold rho - rho_bar(r) and s - s_bar(r) (FP32 path)
+    T wbar_lo, wbar_hi;   // d rho_bar/dr prescribed at the Neumann density walls (0 elsewhere)
+    T dT0top;             // T0 - T_bar at the top radius (FP64 difference)
 };
 
 template <typename T>
@@ -126,6 +135,7 @@ __device__ __forceinline__ float temperature<float>(float rho, float s, const Co
 template <typename T>
 struct Thermo {
     T rho, t;
+    T a, b;   // fluctuation form: T/rho = (T_bar/rho_bar)(1+a), T = T_bar (1+b)
 };
 template <typename T>
 __device__ __forceinline__ Thermo<T> thermo(T rho_st, T s_st, int ir, const Consts<T> &c, const Prof<T> &P) {
@@ -133,12 +143,15 @@ __device__ __forceinline__ Thermo<T> thermo(T rho_st, T s_st, int ir, const Cons
     if (c.fluct) {
         const T rb = P.rbar[ir];
         o.rho = rb + rho_st;
-        T q = T(1) + rho_st / rb;
-        if constexpr (sizeof(T) == 4) o.t = P.tbar[ir] * powf(q, c.gamm1) * expf(s_st * c.invCv);
-        else o.t = P.tbar[ir] * pow(q, c.gamm1) * exp(s_st * c.invCv);
+        // log1p/expm1 keep the relative precision of the small departures a, b themselves
+        const T lq = log1p(rho_st / rb), e1 = s_st * c.invCv;
+        o.b = expm1(c.gamm1 * lq + e1);
+        o.a = expm1((c.gamm1 - T(1)) * lq + e1);
+        o.t = P.tbar[ir] * (T(1) + o.b);
     } else {
         o.rho = rho_st;
         o.t = temperature(rho_st, s_st, c);
+        o.a = o.b = T(0);
     }
     return o;
 }
@@ -149,6 +162,38 @@ __device__ __forceinline__ void coords(long i, const Consts<T> &c, int &it, int 
     long q = i / c.nth;
     ip = (int)(q % c.nphi);
     ir = (int)(q / c.nphi);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Bound (rhs_mpi.f90:2395-2491) as a per-point predicate, fused into the stage update
+// ---------------------------------------------------------------------------------------------
+struct BoundFlags {
+    int theta_stress_free, bot_sf, top_sf, bot_ctemp, magnetic, perf_cond, bot_radial, top_radial;
+};
+
+template <typename T>
+__device__ __forceinline__ T bound_value(int iv, int it, int ip, int ir, T val, const BoundFlags &f, int nth,
+                                         int nr, const T *sr1) {
+    const bool thw = (it == 0 || it == nth - 1);
+    if (iv == 1 && thw) return T(0);                                     // w at theta walls
+    if (iv == 3 && thw && !f.theta_stress_free) return T(0);
+    if (ir == 0) {
+        if ((iv == 1 || iv == 2) && !f.bot_sf) return T(0);
+        if (iv == 3) return T(0);                                        // u_r at r1
+        if (iv == 4 && f.bot_ctemp) return sr1[it];
+    }
+    if (ir == nr - 1) {
+        if ((iv == 1 || iv == 2) && !f.top_sf) return T(0);
+        if (iv == 3) return T(0);                                        // u_r at r2
+    }
+    if (f.magnetic && iv >= 5) {
+        if (thw) {
+            if (f.perf_cond) { if (iv == 6 || iv == 7) return T(0); }
+            else if (iv == 5) return T(0);                               // A_theta at theta walls
+        }
+        if (iv == 7 && ((ir == 0 && f.bot_radial) || (ir == nr - 1 && f.top_radial))) return T(0);
+    }
+    return val;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -263,7 +308,9 @@ __global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const 
     T xdw, xdv, xdu, xdrho, xds, wk, drho_out, dw_out = T(0), dv_out = T(0), du_out = T(0);
     // ---- Source_Theta_Terms (rhs_mpi.f90:812-1027)
     xdw = omsq * sn * cs;
-    xdu = -P.grav[ir] + omsq * sn * sn;
+    // fluctuation form: gravity enters through the background residual hres, the pressure
+    // gradients below through their departures from the background's
+    xdu = (c.fluct ? P.hres[ir] : -P.grav[ir]) + omsq * sn * sn;
     xdrho = -Rho * W * cot * ri;
     xdv = -om2 * cs * W;
     xdu = xdu + ri * W * W;
@@ -349,8 +396,12 @@ __global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const 
         xdu = xdu - U * wk;
         drho_out = drho_out - Rho * wk;
         wk = D[D_RS * N + i];
-        if (c.fluct) wk = wk + P.dsbar[ir];
-        xdu = xdu - c.gamm1 * t * wk;
+        if (c.fluct) {
+            xdu = xdu - c.gamm1 * (t * wk + P.tbar[ir] * th.b * P.dsbar[ir]);
+            wk = wk + P.dsbar[ir];
+        } else {
+            xdu = xdu - c.gamm1 * t * wk;
+        }
         xds = xds - U * wk;
         if (!mag) du_out = xdu;
         if (mag) {
@@ -395,6 +446,16 @@ __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, W
         const T rho0 = a0.rho, rho1 = a1.rho, t0 = a0.t, t1 = a1.t;
         T lo = (lbcr_rho == 2 || lbcr_rho == 4) ? dv[3 * N + q0] * rho0 * c.invpconst / t0 : rho0;
         T hi = (lbcr_rho == 3 || lbcr_rho == 4) ? dv[3 * N + q1] * rho1 * c.invpconst / t1 : rho1;
+        if (c.fluct) {
+            // the sweep differentiates rho - rho_bar, whose background derivative drbar already
+            // carries wbar at these walls: prescribe the departure
+            //   (du/dt without density terms + pconst (T_bar/rho_bar) wbar) rho/(pconst T) - wbar
+            //   = du_partial (rho/T)/pconst - wbar a/(1+a)      (du_partial: hres-based, small)
+            if (lbcr_rho == 2 || lbcr_rho == 4)
+                lo = dv[3 * N + q0] * (rho0 / t0) * c.invpconst - c.wbar_lo * a0.a / (T(1) + a0.a);
+            if (lbcr_rho == 3 || lbcr_rho == 4)
+                hi = dv[3 * N + q1] * (rho1 / t1) * c.invpconst - c.wbar_hi * a1.a / (T(1) + a1.a);
+        }
         wall[W.off_rrho_lo + i] = lo / P.dxdr[0];
         wall[W.off_rrho_hi + i] = hi / P.dxdr[c.nr - 1];
     }
@@ -404,10 +465,73 @@ __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, W
 // 6. density terms, top relaxation, induction, xi_cs (rhs_mpi.f90:1580-1596, :1781-1818,
 //    :2342-2346, :769)
 // ---------------------------------------------------------------------------------------------
+// RK4 stage update fused into the last RHS kernel (step_rk4.f90:392, 418-419, 444-445, 470-471):
+//   stage 1: acc = Dif + k/6   y1 = y0 + (dt/2) k
+//   stage 2: acc += k/3        y1 = y0 + (dt/2) k
+//   stage 3: acc += k/3        y1 = y0 + dt k
+//   stage 4: acc += k/6        y0 = y0 + dt acc
+// The tendency k of every field at a point is consumed where it is produced instead of being
+// stored and re-read.
+// stage 0 = plain RHS (write dv).  FP64 path: acc64 (FP64 accumulator), y1 = stage state.
+// Mixed path (T = float): acc32 (FP32 accumulator), y0f = FP32 copy of y0 (fluctuation form),
+// y1 = FP32 stage state; stage 4 forms y0 + dt acc in FP64.  Bound is applied to every new
+// value.  dif = the SLD tendency (stage 1 only) or null.
 template <typename T>
-__global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const T *__restrict__ st,
+struct StageFuse {
+    int stage;
+    double dt;
+    BoundFlags f;
+    double *y0;
+    double *acc64;
+    float *acc32;
+    T *y0f;
+    const T *dif;
+    T *y1;
+    const double *sr1_64, *rbar64, *sbar64;
+    const T *sr1_t;   // constant-temperature wall entropy in stage precision/form
+};
+
+template <typename T>
+__device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<T> &c, int iv, long i, int it,
+                                            int ip, int ir, T kk_t) {
+    const long j = (long)iv * c.N + i;
+    const int st = F.stage;
+    if constexpr (sizeof(T) == sizeof(double)) {
+        const double kk = (double)kk_t;
+        double s;
+        if (st == 1) s = (F.dif ? (double)F.dif[j] : 0.0) + kk / 6.0;
+        else s = F.acc64[j] + kk / (st == 4 ? 6.0 : 3.0);
+        const double y = st == 4 ? F.y0[j] + F.dt * s : F.y0[j] + (st == 3 ? F.dt : F.dt / 2.0) * kk;
+        const double yb = bound_value<double>(iv, it, ip, ir, y, F.f, c.nth, c.nr, F.sr1_64);
+        if (st < 4) {
+            F.acc64[j] = s;
+            F.y1[j] = (T)yb;
+        } else {
+            F.y0[j] = yb;
+        }
+    } else {
+        const float kk = (float)kk_t;
+        if (st < 4) {
+            const float a = st == 1 ? (F.dif ? (float)F.dif[j] : 0.0f) + kk / 6.0f : F.acc32[j] + kk / 3.0f;
+            const float h = (float)(st == 3 ? F.dt : 0.5 * F.dt);
+            const float y = (float)F.y0f[j] + h * kk;
+            F.acc32[j] = a;
+            F.y1[j] = (T)bound_value<float>(iv, it, ip, ir, y, F.f, c.nth, c.nr, (const float *)F.sr1_t);
+        } else {
+            const float a = F.acc32[j] + kk / 6.0f;
+            const double y = F.y0[j] + F.dt * (double)a;
+            const double yb = bound_value<double>(iv, it, ip, ir, y, F.f, c.nth, c.nr, F.sr1_64);
+            const double bg = iv == 0 ? F.rbar64[ir] : iv == 4 ? F.sbar64[ir] : 0.0;
+            F.y0[j] = yb;
+            F.y0f[j] = (T)(yb - bg);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const T *st,
                                                    const T *__restrict__ B, const T *__restrict__ dth_rho,
-                                                   const T *__restrict__ dr_rho, T *__restrict__ dv) {
+                                                   const T *__restrict__ dr_rho, T *dv, StageFuse<T> F) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.N) return;
     const long N = c.N;
@@ -425,58 +549,55 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
     dw = dw - c.pconst * wk * t;
     // Density_Terms_Radial
     wk = dr_rho[i];
-    if (c.fluct) wk = wk + P.drbar[ir];
-    drho = drho - U * wk;
-    wk = wk * invrho;
-    du = du - c.pconst * t * wk;
+    if (c.fluct) {
+        // pconst (T/rho) (drho'/dr + drbar) minus the background part already in hres
+        du = du - c.pconst * ((t * invrho) * wk + P.tr[ir] * th.a * P.drbar[ir]);
+        wk = wk + P.drbar[ir];
+        drho = drho - U * wk;
+    } else {
+        drho = drho - U * wk;
+        wk = wk * invrho;
+        du = du - c.pconst * t * wk;
+    }
     if (c.top_relax && ir == c.nr - 1) {
         const T dt_ = c.damp * (c.gamma - T(2));
-        drho = drho + Rho * (c.T0 - t) / (dt_ * t);
-        ds = ds - c.Cv * (c.T0 - t) / (dt_ * t);
+        const T dT = c.fluct ? c.dT0top - P.tbar[ir] * th.b : c.T0 - t;   // T0 - T
+        drho = drho + Rho * dT / (dt_ * t);
+        ds = ds - c.Cv * dT / (dt_ * t);
     }
     if (c.xi_on) drho = c.xi2 * drho;
-    dv[i] = drho;
-    dv[N + i] = dw;
-    dv[3 * N + i] = du;
-    dv[4 * N + i] = ds;
+    T da0 = T(0), da1 = T(0), da2 = T(0);
     if (c.magnetic) {
         const T Bt = B[i], Bp = B[N + i], Br = B[2 * N + i];
-        dv[5 * N + i] = V * Br - U * Bp;
-        dv[6 * N + i] = U * Bt - W * Br;
-        dv[7 * N + i] = W * Bp - V * Bt;
+        da0 = V * Br - U * Bp;
+        da1 = U * Bt - W * Br;
+        da2 = W * Bp - V * Bt;
     }
-}
-
-// ---------------------------------------------------------------------------------------------
-// Bound (rhs_mpi.f90:2395-2491) as a per-point predicate, fused into the stage update
-// ---------------------------------------------------------------------------------------------
-struct BoundFlags {
-    int theta_stress_free, bot_sf, top_sf, bot_ctemp, magnetic, perf_cond, bot_radial, top_radial;
-};
-
-template <typename T>
-__device__ __forceinline__ T bound_value(int iv, int it, int ip, int ir, T val, const BoundFlags &f, int nth,
-                                         int nr, const T *sr1) {
-    const bool thw = (it == 0 || it == nth - 1);
-    if (iv == 1 && thw) return T(0);                                     // w at theta walls
-    if (iv == 3 && thw && !f.theta_stress_free) return T(0);
-    if (ir == 0) {
-        if ((iv == 1 || iv == 2) && !f.bot_sf) return T(0);
-        if (iv == 3) return T(0);                                        // u_r at r1
-        if (iv == 4 && f.bot_ctemp) return sr1[it];
-    }
-    if (ir == nr - 1) {
-        if ((iv == 1 || iv == 2) && !f.top_sf) return T(0);
-        if (iv == 3) return T(0);                                        // u_r at r2
-    }
-    if (f.magnetic && iv >= 5) {
-        if (thw) {
-            if (f.perf_cond) { if (iv == 6 || iv == 7) return T(0); }
-            else if (iv == 5) return T(0);                               // A_theta at theta walls
+    if (F.stage == 0) {
+        dv[i] = drho;
+        dv[N + i] = dw;
+        dv[3 * N + i] = du;
+        dv[4 * N + i] = ds;
+        if (c.magnetic) {
+            dv[5 * N + i] = da0;
+            dv[6 * N + i] = da1;
+            dv[7 * N + i] = da2;
         }
-        if (iv == 7 && ((ir == 0 && f.bot_radial) || (ir == nr - 1 && f.top_radial))) return T(0);
+    } else {
+        // st may alias F.y1 (the stage state is updated in place): every read of st at this
+        // point happened above
+        const T dvv = dv[2 * N + i];
+        stage_value(F, c, 0, i, it, ip, ir, drho);
+        stage_value(F, c, 1, i, it, ip, ir, dw);
+        stage_value(F, c, 2, i, it, ip, ir, dvv);
+        stage_value(F, c, 3, i, it, ip, ir, du);
+        stage_value(F, c, 4, i, it, ip, ir, ds);
+        if (c.magnetic) {
+            stage_value(F, c, 5, i, it, ip, ir, da0);
+            stage_value(F, c, 6, i, it, ip, ir, da1);
+            stage_value(F, c, 7, i, it, ip, ir, da2);
+        }
     }
-    return val;
 }
 
 template <typename T>
@@ -491,53 +612,6 @@ __global__ __launch_bounds__(256) void k_bound(Consts<T> c, BoundFlags f, const 
 }
 
 // ---------------------------------------------------------------------------------------------
-// RK4 stage update (step_rk4.f90:392, 418-419, 444-445, 470-471) + Bound.
-//   stage 1: save = Dif + k/6   y1 = y0 + (dt/2) k
-//   stage 2: save += k/3        y1 = y0 + (dt/2) k
-//   stage 3: save += k/3        y1 = y0 + dt k
-//   stage 4: save += k/6        y1 = y0 + dt save; y0 = y1
-// y0/save are FP64; k and y1 are in the stage precision TS.
-// ---------------------------------------------------------------------------------------------
-template <typename TS>
-__global__ __launch_bounds__(256) void k_rk_stage(Consts<TS> c, BoundFlags f, const double *__restrict__ sr1_64,
-                                                  const double *__restrict__ rbar64,
-                                                  const double *__restrict__ sbar64, int stage, double dt,
-                                                  double *__restrict__ y0, double *__restrict__ save,
-                                                  const TS *__restrict__ k, const TS *__restrict__ dif,
-                                                  TS *__restrict__ y1, int nv) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N * nv) return;
-    const int iv = (int)(i / c.N);
-    const long j = i - (long)iv * c.N;
-    int it, ip, ir;
-    coords(j, c, it, ip, ir);
-    const double kk = (double)k[i];
-    double s, y;
-    if (stage == 1) {
-        s = (dif ? (double)dif[i] : 0.0) + kk / 6.0;
-        y = y0[i] + (dt / 2.0) * kk;
-    } else if (stage == 2) {
-        s = save[i] + kk / 3.0;
-        y = y0[i] + (dt / 2.0) * kk;
-    } else if (stage == 3) {
-        s = save[i] + kk / 3.0;
-        y = y0[i] + dt * kk;
-    } else {
-        s = save[i] + kk / 6.0;
-        y = y0[i] + dt * s;
-    }
-    const double yb = bound_value<double>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_64);
-    // FP32 stage states carry rho and s as fluctuations about the FP64 background
-    const double bg = (sizeof(TS) == sizeof(double)) ? 0.0 : (iv == 0 ? rbar64[ir] : iv == 4 ? sbar64[ir] : 0.0);
-    if (stage < 4) {
-        save[i] = s;
-        y1[i] = (TS)(yb - bg);
-    } else {
-        y0[i] = yb;
-        if (sizeof(TS) != sizeof(double)) y1[i] = (TS)(yb - bg);   // next step's stage-1 input
-    }
-}
-
 // y1 = (TS)(y0 - background) (first step of an FP32 run; rho and s as fluctuations)
 template <typename TS>
 __global__ __launch_bounds__(256) void k_cast_state(Consts<TS> c, int nv, const double *__restrict__ rbar64,
@@ -551,45 +625,6 @@ __global__ __launch_bounds__(256) void k_cast_state(Consts<TS> c, int nv, const 
     y1[i] = (TS)(y0[i] - bg);
 }
 
-
-// RK4 stage of the mixed-precision path (step_rk4.f90:339-493): stage states, tendencies and the
-// tendency accumulator acc = D + k1/6 + k2/3 + k3/3 + k4/6 in FP32, the state y0 in FP64.
-// Stages 1-3 form y1 = y0f + c dt k from the FP32 copy y0f of y0 (fluctuation form) and
-// apply Bound in that form; stage 4 forms y0 + dt acc in FP64, applies Bound there and
-// refreshes y0f.  Per value: 16-20 bytes at stage 1, 20 at stages 2-3, 28 at stage 4.
-__global__ __launch_bounds__(256) void k_rk_stage_mixed(Consts<float> c, BoundFlags f,
-                                                        const double *__restrict__ sr1_64,
-                                                        const float *__restrict__ sr1_32,
-                                                        const double *__restrict__ rbar64,
-                                                        const double *__restrict__ sbar64, int stage, double dt,
-                                                        double *__restrict__ y0, float *__restrict__ y0f,
-                                                        float *__restrict__ acc, const float *__restrict__ k,
-                                                        const float *__restrict__ dif, float *__restrict__ y1,
-                                                        int nv) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N * nv) return;
-    const int iv = (int)(i / c.N);
-    const long j = i - (long)iv * c.N;
-    int it, ip, ir;
-    coords(j, c, it, ip, ir);
-    const float kk = k[i];
-    if (stage < 4) {
-        float a;
-        if (stage == 1) a = (dif ? dif[i] : 0.0f) + kk / 6.0f;
-        else a = acc[i] + kk / 3.0f;
-        const float h = (float)(stage == 3 ? dt : 0.5 * dt);
-        const float y = y0f[i] + h * kk;
-        acc[i] = a;
-        y1[i] = bound_value<float>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_32);
-    } else {
-        const float a = acc[i] + kk / 6.0f;
-        const double y = y0[i] + dt * (double)a;
-        const double yb = bound_value<double>(iv, it, ip, ir, y, f, c.nth, c.nr, sr1_64);
-        const double bg = iv == 0 ? rbar64[ir] : iv == 4 ? sbar64[ir] : 0.0;
-        y0[i] = yb;
-        y0f[i] = (float)(yb - bg);
-    }
-}
 
 // Stage of Step_RK3_TVD / Step_RK2_TVD (step_rk4.f90:166-335, :31-162) with Bound:
 //   mode 0 (stage 1)  y = y0 + dtc k
@@ -683,6 +718,9 @@ static Consts<T> make_consts(const cssx_mhd *m) {
     c.xi2 = (T)(m->p.xi_cs * m->p.xi_cs); c.gamma = (T)m->p.gamma; c.T0 = (T)m->p.T0; c.damp = (T)m->p.damping_time;
     c.magnetic = m->p.magnetic; c.top_relax = m->p.top_relax; c.xi_on = m->p.xi_cs < 1.0;
     c.fluct = sizeof(T) == 4;
+    c.wbar_lo = (T)m->wbar_lo;
+    c.wbar_hi = (T)m->wbar_hi;
+    c.dT0top = m->h_tbar.empty() ? T(0) : (T)(m->p.T0 - m->h_tbar[m->nr - 1]);
     return c;
 }
 
@@ -695,10 +733,10 @@ static BoundFlags make_bound_flags(const cssx_mhd *m) {
 }
 
 template <typename T>
-static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &drbar,
-                           const std::vector<double> &dsbar) {
+static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const int nr = m->nr, nth = m->nth;
-    const size_t tot = 14 * (size_t)nr + 10 * (size_t)nth + 2;
+    const std::vector<double> &drbar = m->h_drbar, &dsbar = m->h_dsbar, &hres = m->h_hres;
+    const size_t tot = 16 * (size_t)nr + 10 * (size_t)nth + 2;
     std::vector<T> h(tot);
     size_t o = 0;
     auto put = [&](size_t n, auto gen) {
@@ -715,9 +753,9 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &d
     const size_t o_e = put(nr, [&](size_t i) { return m->h_eps[i]; });
     const size_t o_rb = put(nr, [&](size_t i) { return m->h_rho_bar[i]; });
     const size_t o_sb = put(nr, [&](size_t i) { return m->h_s_bar[i]; });
-    const size_t o_tb = put(nr, [&](size_t i) {
-        return m->invtconst * std::pow(m->h_rho_bar[i], m->gamm1) * std::exp(m->h_s_bar[i] * m->invCv);
-    });
+    const size_t o_tb = put(nr, [&](size_t i) { return m->h_tbar[i]; });
+    const size_t o_tr = put(nr, [&](size_t i) { return m->h_tbar[i] / m->h_rho_bar[i]; });
+    const size_t o_hr = put(nr, [&](size_t i) { return hres.empty() ? 0.0 : hres[i]; });
     const size_t o_drb = put(nr, [&](size_t i) { return drbar.empty() ? 0.0 : drbar[i]; });
     const size_t o_dsb = put(nr, [&](size_t i) { return dsbar.empty() ? 0.0 : dsbar[i]; });
     const size_t o_sn = put(nth, [&](size_t i) { return std::sin(m->h_theta[i]); });
@@ -760,7 +798,7 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &d
     if (hipMemcpy(d, h.data(), tot * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return CSSX_EHIP;
     P.buf = d;
     P.r = d + o_r; P.rinv = d + o_ri; P.dxdr = d + o_x; P.grav = d + o_g; P.eps = d + o_e;
-    P.rbar = d + o_rb; P.sbar = d + o_sb; P.tbar = d + o_tb; P.drbar = d + o_drb; P.dsbar = d + o_dsb;
+    P.rbar = d + o_rb; P.sbar = d + o_sb; P.tbar = d + o_tb; P.tr = d + o_tr; P.hres = d + o_hr; P.drbar = d + o_drb; P.dsbar = d + o_dsb;
     P.sn = d + o_sn; P.csc = d + o_csc; P.cs = d + o_cs; P.cot = d + o_cot;
     P.dsdr1 = d + o_d1; P.dsdr2 = d + o_d2; P.sr1 = d + o_s1;
     P.rfc = d + o_rfc; P.rv = d + o_rv; P.sph = d + o_sph; P.rfac = d + o_rfac; P.ct = d + o_ct; P.gt = d + o_gt;
@@ -770,15 +808,23 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P, const std::vector<double> &d
 
 // Physical radial derivative (times dx/dr) of a radial profile by the FP64 compact operator
 // with zero wall values: the background part of d/dr in the FP32 fluctuation form.
-static int background_derivative(cssx_mhd *m, const double *prof_dev, int code, std::vector<double> &out) {
+static int background_derivative(cssx_mhd *m, const double *prof_dev, int code, double wlo, double whi,
+                                 std::vector<double> &out) {
     double *d = nullptr;
-    if (hipMalloc(&d, sizeof(double) * m->nr) != hipSuccess) return CSSX_ENOMEM;
+    if (hipMalloc(&d, sizeof(double) * (m->nr + 2)) != hipSuccess) return CSSX_ENOMEM;
+    // prescribed physical wall derivatives, in x units (the sweep multiplies by dx/dr)
+    const double wv[2] = {wlo / m->h_dxdr[0], whi / m->h_dxdr[m->nr - 1]};
+    if (hipMemcpy(d + m->nr, wv, sizeof(wv), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return CSSX_EHIP;
+    }
     AxisGeom g;
     g.n = m->nr; g.na = 1; g.nb = 1; g.sa = 1; g.sb = 0; g.srow = 1;
     g.g_k = 0; g.g_a = 0; g.g_b = 0; g.phase = CSSX_PHASE_OTHER;
     DerivSpec s;
     s.order = 1;
     s.in = prof_dev; s.out = d;
+    s.glo = d + m->nr; s.ghi = d + m->nr + 1;
     s.lo_kind = s.hi_kind = END_WALL;
     s.bdry = code;
     s.scale = m->grid.dri;
@@ -891,7 +937,14 @@ static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream, bool zer
 }
 
 template <typename T>
-static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream) {
+static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream);
+
+// RHS(call_num) of state st into dv.  With a stage descriptor F (F.stage > 0) the last kernel
+// applies that RK4 stage instead of writing the full tendency; with sld_dif the SLD diffusion
+// of st is computed into sld_dif once B is known (Compute_Diffusion(1) after RHS(1)).
+template <typename T>
+static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream, const StageFuse<T> *F = nullptr,
+                    T *sld_dif = nullptr) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
     Scratch<T> &S = scratch<T>(m);
@@ -944,6 +997,8 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     // 4. pointwise momentum / entropy / continuity terms
     CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, (m->p.magnetic ? 32.0 : 24.0) * N * sizeof(T),
                hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv));
+    // SLD diffusion while the derivative scratch is free (it needs B and the stage-1 state)
+    if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream))) return rc;
     // 5. density wall derivatives from the momentum tendency
     {
         long nw = std::max((long)m->nr * m->nphi, (long)m->nth * m->nphi);
@@ -957,9 +1012,15 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         DerivSpec rr = spec<T>(m, CSSX_AXIS_R, rho, D + 1 * N, ibcr[1], wall + W.off_rrho_lo, wall + W.off_rrho_hi,
                                nullptr, P.dxdr);
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, &rr, 1, stream))) return rc;
-        CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T),
+        StageFuse<T> F0{};
+        const StageFuse<T> &FF = F ? *F : F0;
+        // plain: 15/21 values per point; a fused stage adds the stage traffic of every field
+        const double fb = !F ? 0.0
+                          : sizeof(T) == 8 ? (FF.stage < 4 ? 40.0 - 8.0 : 32.0 - 8.0) * m->nv
+                                           : (FF.stage == 1 ? 12.0 + (FF.dif ? 4.0 : 0.0) : FF.stage < 4 ? 16.0 : 24.0) * m->nv;
+        CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T) + fb * N,
                    hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N,
-                                      D + 1 * N, dv));
+                                      D + 1 * N, dv, FF));
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
@@ -1038,50 +1099,34 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
     if (rc) return rc;
     Scratch<T> &S = scratch<T>(m);
     const Consts<T> c = make_consts<T>(m);
-    const BoundFlags f = make_bound_flags(m);
     const long NV = m->N * m->nv;
     const unsigned nb = nblocks(NV, 256);
-    const bool sld = m->p.sld_on != 0;
-    T *dif = sld ? S.dif : nullptr;
-    if constexpr (sizeof(T) == sizeof(double)) {
-        // algorithmic bytes of one stage update: read k (+ save after stage 1, + D at stage 1)
-        // and y0, write save and y1 (stages 1-3) or y0 (stage 4)
-        auto rk_bytes = [&](int stage) {
-            return (8.0 * (2 + (stage > 1) + (stage == 1 && sld)) + (stage < 4 ? 16.0 : 8.0)) * (double)NV;
-        };
-        for (int step = 0; step < nsteps; step++) {
-            if ((rc = rhs_impl<T>(m, (const T *)y0, S.k, nullptr, stream))) return rc;
-            // Compute_Diffusion(1) on vars1 with B as RHS(1) leaves it (step_rk4.f90:366-383)
-            if (sld && (rc = sld_impl<T>(m, (const T *)y0, S.B, dif, stream))) return rc;
-            for (int stage = 1; stage <= 4; stage++) {
-                if (stage > 1 && (rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
-                CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
-                           hipLaunchKernelGGL(k_rk_stage<T>, dim3(nb), dim3(256), 0, stream, c, f,
-                                              (const double *)m->p64.sr1, (const double *)m->p64.rbar,
-                                              (const double *)m->p64.sbar, stage, dt, y0, m->save, (const T *)S.k,
-                                              (const T *)(stage == 1 ? dif : nullptr), S.y1, m->nv));
-            }
-        }
-    } else {
-        auto rk_bytes = [&](int stage) {
-            const double b = stage == 1 ? 16.0 + (sld ? 4.0 : 0.0) : stage < 4 ? 20.0 : 28.0;
-            return b * (double)NV;
-        };
+    T *dif = m->p.sld_on ? S.dif : nullptr;
+    constexpr bool mixed = sizeof(T) != sizeof(double);
+    StageFuse<T> F{};
+    F.dt = dt;
+    F.f = make_bound_flags(m);
+    F.y0 = y0;
+    F.acc64 = m->save;
+    F.acc32 = mixed ? (float *)S.acc : nullptr;
+    F.y0f = mixed ? S.y0f : nullptr;
+    F.y1 = S.y1;
+    F.sr1_64 = m->p64.sr1;
+    F.rbar64 = m->p64.rbar;
+    F.sbar64 = m->p64.sbar;
+    F.sr1_t = prof<T>(m).sr1;
+    if (mixed)
         hipLaunchKernelGGL(k_cast_state<T>, dim3(nb), dim3(256), 0, stream, c, m->nv, (const double *)m->p64.rbar,
                            (const double *)m->p64.sbar, (const double *)y0, S.y0f);
-        for (int step = 0; step < nsteps; step++) {
-            // stage 1 reads vars1 = vars0 through its FP32 copy
-            if ((rc = rhs_impl<T>(m, S.y0f, S.k, nullptr, stream))) return rc;
-            if (sld && (rc = sld_impl<T>(m, S.y0f, S.B, dif, stream))) return rc;
-            for (int stage = 1; stage <= 4; stage++) {
-                if (stage > 1 && (rc = rhs_impl<T>(m, S.y1, S.k, nullptr, stream))) return rc;
-                CSSX_TIMED(CSSX_PHASE_RK_UPDATE, stream, rk_bytes(stage),
-                           hipLaunchKernelGGL(k_rk_stage_mixed, dim3(nb), dim3(256), 0, stream, c, f,
-                                              (const double *)m->p64.sr1, (const float *)m->p32.sr1,
-                                              (const double *)m->p64.rbar, (const double *)m->p64.sbar, stage, dt,
-                                              y0, S.y0f, S.acc, (const float *)S.k,
-                                              (const float *)(stage == 1 ? dif : nullptr), S.y1, m->nv));
-            }
+    // stage 1 reads vars1 = vars0 (its FP32 copy on the mixed path); each stage's update is
+    // fused into the RHS's last kernel; Compute_Diffusion(1) runs inside stage 1's RHS
+    const T *in1 = mixed ? (const T *)S.y0f : (const T *)y0;
+    for (int step = 0; step < nsteps; step++) {
+        for (int stage = 1; stage <= 4; stage++) {
+            F.stage = stage;
+            F.dif = stage == 1 ? dif : nullptr;
+            if ((rc = rhs_impl<T>(m, stage == 1 ? in1 : S.y1, S.k, nullptr, stream, &F, stage == 1 ? dif : nullptr)))
+                return rc;
         }
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
@@ -1177,18 +1222,33 @@ extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *par
     if (pr->diffusion_speed) m->h_diff_speed.assign(pr->diffusion_speed, pr->diffusion_speed + nr + 2);
     m->dth = (m->h_theta[nth - 1] - m->h_theta[0]) / (nth - 1);
     m->dphi = 1.0 / grid->dpi;
+    m->h_tbar.resize(nr);
+    for (int i = 0; i < nr; i++)
+        m->h_tbar[i] = m->invtconst * std::pow(m->h_rho_bar[i], m->gamm1) * std::exp(m->h_s_bar[i] * m->invCv);
     // background derivatives need the FP64 profiles and the plans on the device first
-    std::vector<double> none, drbar, dsbar;
-    int rc = upload_profiles<double>(m, m->p64, none, none);
+    int rc = upload_profiles<double>(m, m->p64);
     if (!rc) {
         cssx_ctx *ctx = nullptr;
         rc = cssx_ctx_create(grid, &ctx);
         cssx_ctx_destroy(ctx);
     }
-    if (!rc) rc = background_derivative(m, m->p64.rbar, params->ibcr[1], drbar);
-    if (!rc) rc = background_derivative(m, m->p64.sbar, params->ibcr[5], dsbar);
-    if (!rc) rc = upload_profiles<double>(m, m->p64, drbar, dsbar);
-    if (!rc) rc = upload_profiles<float>(m, m->p32, drbar, dsbar);
+    // s_bar with zero wall derivatives; rho_bar with the wall derivatives that make the
+    // background's radial momentum balance exactly on Neumann density walls (the density wall
+    // rule of Density_Terms_Radial, rhs_mpi.f90:1750-1773, applied to the background)
+    if (!rc) rc = background_derivative(m, m->p64.sbar, params->ibcr[5], 0.0, 0.0, m->h_dsbar);
+    if (!rc) {
+        const int code = params->ibcr[1];
+        auto du_bar = [&](int i) { return -m->h_grav[i] - m->gamm1 * m->h_tbar[i] * m->h_dsbar[i]; };
+        auto wall = [&](int i) { return du_bar(i) * m->h_rho_bar[i] / (m->pconst * m->h_tbar[i]); };
+        m->wbar_lo = (code == 2 || code == 4) ? wall(0) : 0.0;
+        m->wbar_hi = (code == 3 || code == 4) ? wall(nr - 1) : 0.0;
+        rc = background_derivative(m, m->p64.rbar, code, m->wbar_lo, m->wbar_hi, m->h_drbar);
+        m->h_hres.resize(nr);
+        for (int i = 0; i < nr && !rc; i++)
+            m->h_hres[i] = du_bar(i) - m->pconst * (m->h_tbar[i] / m->h_rho_bar[i]) * m->h_drbar[i];
+    }
+    if (!rc) rc = upload_profiles<double>(m, m->p64);
+    if (!rc) rc = upload_profiles<float>(m, m->p32);
     if (!rc && hipMalloc(&m->save, sizeof(double) * m->N * m->nv) != hipSuccess) rc = CSSX_ENOMEM;
     if (rc) {
         cssx_mhd_destroy(m);
