@@ -210,20 +210,34 @@ def main():
     tab32 = profile_phases(sol, y0, dt, prof_steps, "f32") if "f32" in paths else None
     tab64 = profile_phases(sol, y0, dt, prof_steps, "f64") if "f64" in paths else None
 
+    families = {"k_deriv": ("deriv_r", "deriv_theta", "deriv_phi"), "k_rhs_main": ("rhs_main",),
+                "k_rhs_final": ("rhs_final",), "k_sld": ("sld",), "pointwise_other": ("pointwise",),
+                "rk_stage": ("rk_update",)}
+
+    def family_rows(tab):
+        rows = {}
+        for name, ph in families.items():
+            ms = sum(tab[k][0] for k in ph)
+            n = sum(tab[k][1] for k in ph)
+            by = sum(tab[k][2] for k in ph)
+            if n:
+                avg = ms / n
+                ach = (by / n) / (avg * 1e-3) / 1e9
+                rows[name] = {"ms_per_step": round(ms / prof_steps, 4), "launches": n, "avg_launch_ms": round(avg, 5),
+                              "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+        return rows
+
     def roofline(tab):
+        """Roofline of the dominant kernel family (largest summed time): achieved = algorithmic
+        bytes per launch / average launch time, both from the per-launch hipEvent timer."""
         if tab is None:
             return None
-        # dominant kernel = the phase with the largest summed time; k_deriv covers the three axes
-        dphases = ("deriv_r", "deriv_theta", "deriv_phi")
-        dom = max(tab, key=lambda k: tab[k][0])
-        ms_d = sum(tab[k][0] for k in dphases)
-        n_d = sum(tab[k][1] for k in dphases)
-        b_d = sum(tab[k][2] for k in dphases)
-        name = "k_deriv" if dom in dphases or ms_d >= tab[dom][0] else dom
-        if name == "k_deriv":
-            ms, n, by = ms_d, n_d, b_d
-        else:
-            ms, n, by = tab[dom]
+        rows = family_rows(tab)
+        name = max(rows, key=lambda k: rows[k]["ms_per_step"])
+        ph = families[name]
+        ms = sum(tab[k][0] for k in ph)
+        n = sum(tab[k][1] for k in ph)
+        by = sum(tab[k][2] for k in ph)
         avg_ms = ms / max(n, 1)
         achieved = (by / max(n, 1)) / (avg_ms * 1e-3) / 1e9
         return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -259,6 +273,7 @@ def main():
                                for p, t in (("f32", tab32), ("f64", tab64))},
         "roofline": roofline(tab32),
         "roofline_f64": roofline(tab64),
+        "kernel_families": {p: (family_rows(t) if t else None) for p, t in (("f32", tab32), ("f64", tab64))},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(c.sld_coef)

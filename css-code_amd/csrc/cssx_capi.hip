@@ -78,11 +78,14 @@ void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g) {
     }
 }
 
-// The contiguous kernel serves pencils with unit row stride; its D2 upper-wall closure needs
-// five rows below row n-1 inside one lane's window, i.e. the last lane must own >= 2 rows.
+// The contiguous kernel serves pencils with unit row stride, and strided pencils longer than
+// 512 rows, whose strided segments (64 rows) would not fit in registers: there each lane reads
+// its MR rows with the row stride, and neighbouring pencils of the same workgroup share the
+// cache lines.  Its D2 upper-wall closure needs five rows below row n-1 inside one lane's
+// window, i.e. the last lane must own >= 2 rows.
 static bool use_contig(const AxisGeom &g, int order, int hi_kind) {
     const int MR = choose_MR(g.n);
-    if (g.srow != 1 || MR < 0) return false;
+    if (MR < 0 || (g.srow != 1 && g.n <= 512)) return false;
     if (order == 2 && hi_kind == END_WALL && g.n % MR == 1) return false;
     return true;
 }
@@ -196,7 +199,8 @@ extern "C" int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out) {
                         PlanKey key{ns[ax], order, lo ? END_HALO : END_WALL, hi ? END_HALO : END_WALL, bd};
                         const void *p;
                         for (int var = 0; var < 2; var++) {
-                            if (var == 1 && (ax != CSSX_AXIS_THETA || choose_MR(ns[ax]) < 0)) continue;
+                            if (var == 1 && (choose_MR(ns[ax]) < 0 || (ax != CSSX_AXIS_THETA && ns[ax] <= 512)))
+                                continue;
                             key.variant = var;
                             key.seg = var == 0 ? strided_M(ns[ax]) : 0;
                             for (int prec = 0; prec < 2; prec++) {

@@ -995,7 +995,7 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, mag ? 6 : 4, stream))) return rc;
     }
     // 4. pointwise momentum / entropy / continuity terms
-    CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, (m->p.magnetic ? 32.0 : 24.0) * N * sizeof(T),
+    CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (m->p.magnetic ? 32.0 : 24.0) * N * sizeof(T),
                hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv));
     // SLD diffusion while the derivative scratch is free (it needs B and the stage-1 state)
     if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream))) return rc;
@@ -1018,7 +1018,7 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         const double fb = !F ? 0.0
                           : sizeof(T) == 8 ? (FF.stage < 4 ? 40.0 - 8.0 : 32.0 - 8.0) * m->nv
                                            : (FF.stage == 1 ? 12.0 + (FF.dif ? 4.0 : 0.0) : FF.stage < 4 ? 16.0 : 24.0) * m->nv;
-        CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T) + fb * N,
+        CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T) + fb * N,
                    hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N,
                                       D + 1 * N, dv, FF));
     }

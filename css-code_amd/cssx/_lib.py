@@ -19,7 +19,8 @@ CSSX_EHIP = 100
 PREC_F64 = 0
 PREC_F32 = 1
 AXIS_R, AXIS_THETA, AXIS_PHI = 0, 1, 2
-PHASES = ("deriv_r", "deriv_theta", "deriv_phi", "pointwise", "rk_update", "other", "sld")
+PHASES = ("deriv_r", "deriv_theta", "deriv_phi", "pointwise", "rk_update", "other", "sld", "rhs_main",
+          "rhs_final")
 
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)

@@ -186,11 +186,13 @@ int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out,
 #define CSSX_PHASE_DERIV_R 0
 #define CSSX_PHASE_DERIV_THETA 1
 #define CSSX_PHASE_DERIV_PHI 2
-#define CSSX_PHASE_POINTWISE 3     /* curl assembly, wall values, RHS terms */
+#define CSSX_PHASE_POINTWISE 3     /* curl assembly and other pointwise kernels */
 #define CSSX_PHASE_RK_UPDATE 4     /* stage update + Bound */
 #define CSSX_PHASE_OTHER 5
 #define CSSX_PHASE_SLD 6           /* SLD diffusion: speed, fluxes, corrections (not its curl) */
-#define CSSX_NPHASE 7
+#define CSSX_PHASE_RHS_MAIN 7      /* the momentum / entropy / continuity term kernel */
+#define CSSX_PHASE_RHS_FINAL 8     /* density terms, u x B, and the fused RK4 stage update */
+#define CSSX_NPHASE 9
 int cssx_timing_enable(int on);
 int cssx_timing_read(double *ms, long *count, double *bytes);
 
