@@ -15,9 +15,6 @@
 
 namespace cssx {
 
-// slow-index pencils one wavefront walks with the same plan registers
-constexpr int kContigPB = 8;
-
 template <typename T>
 __device__ __forceinline__ T shfl(T v, int src) {
     return __shfl(v, src, 64);
@@ -43,49 +40,30 @@ __device__ __forceinline__ void store_run(T *__restrict__ p, const T (&v)[MR]) {
 }
 
 template <typename T, int ORDER, int MR>
-__global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
+__global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
+    extern __shared__ unsigned char smem_raw[];
+    T *sp = reinterpret_cast<T *>(smem_raw);
     const DerivJob<T> &J = L.jobs[blockIdx.z];
     const int n = L.n;
+    const int plen = plan_contig_len(n);
+    for (int i = threadIdx.x; i < plen; i += blockDim.x) sp[i] = J.plan[i];
+    __syncthreads();
     // unit-stride pencils whose length is a multiple of MR move as 16-byte vectors
     const bool vec = L.srow == 1 && n % MR == 0;
+    const int b = blockIdx.y;
+    const T *cf = sp, *cl = cf + n, *cu = cl + n, *vL = cu + n, *vR = vL + n;
+    const T *rl = sp + 5 * n, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
+            *invb = k2 + kPcrSteps * kMaxLanes;
+
     constexpr int W = MR + 8;
     const int Lw = L.S;                       // lanes per pencil group (power of two)
     const int nl = (n + MR - 1) / MR;         // lanes holding rows
     const int nI = nl - 1;                    // interface equations
     const int lane = threadIdx.x & 63;
-    const int j0 = lane & (Lw - 1);
+    const int j = lane & (Lw - 1);
     const int ppw = 64 / Lw;
     const long a = ((long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * ppw + lane / Lw;
     const bool pen_ok = a < L.na;
-    const int st0 = j0 * MR;
-    // This lane's plan coefficients (its MR rows, its interface row, its PCR multipliers) stay
-    // in registers while the wavefront walks kContigPB pencils of the slow index b.
-    const T *cf = J.plan, *cl = cf + n, *cu = cl + n, *vL = cu + n, *vR = vL + n;
-    const T *rl = J.plan + 5 * n, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
-            *invb = k2 + kPcrSteps * kMaxLanes;
-    T c_f[MR], c_l[MR], c_u[MR], c_vl[MR], c_vr[MR];
-#pragma unroll
-    for (int jj = 0; jj < MR; jj++) {
-        const int row = min(st0 + jj, n - 1);
-        c_f[jj] = cf[row]; c_l[jj] = cl[row]; c_u[jj] = cu[row]; c_vl[jj] = vL[row]; c_vr[jj] = vR[row];
-    }
-    const int jc = min(j0, kMaxLanes - 1);
-    const T c_rl = rl[jc], c_ru = ru[jc], c_ib = invb[jc];
-    T c_k1[kPcrSteps], c_k2[kPcrSteps];
-#pragma unroll
-    for (int q = 0; q < kPcrSteps; q++) {
-        c_k1[q] = k1[q * kMaxLanes + jc];
-        c_k2[q] = k2[q * kMaxLanes + jc];
-    }
-    int steps = 0;
-    while ((1 << steps) < nI) steps++;
-    const int b_end = min(L.nb, (int)(blockIdx.y + 1) * kContigPB);
-#pragma unroll 1
-    for (int b = blockIdx.y * kContigPB; b < b_end; b++) {
-    // An opaque copy of the lane's row index keeps everything derived from it inside the loop:
-    // hoisted out, the per-lane closure masks and addresses would double the register count.
-    int j = j0;
-    asm volatile("" : "+v"(j));
     const bool active = pen_ok && j < nl;
     const int st = j * MR;
     const int m = active ? min(MR, n - st) : 0;
@@ -209,7 +187,7 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
         if (jj < mi) {
-            y = c_f[jj] * d[jj] - c_l[jj] * y;
+            y = cf[st + jj] * d[jj] - cl[st + jj] * y;
             d[jj] = y;
         }
     }
@@ -217,7 +195,7 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = MR - 1; jj >= 0; jj--) {
         if (jj < mi) {
-            xn = d[jj] - c_u[jj] * xn;
+            xn = d[jj] - cu[st + jj] * xn;
             d[jj] = xn;
         }
     }
@@ -227,16 +205,16 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
     // ---- interface system by parallel cyclic reduction -----------------------------------------
     const T xf_next = shfl(xfirst, lane + 1);
     T R = T(0);
-    if (has_iface) R = dI - c_rl * xlast - c_ru * xf_next;
-#pragma unroll
-    for (int s = 0; s < kPcrSteps; s++) {
-        if (s >= steps) break;
+    if (has_iface) R = dI - rl[j] * xlast - ru[j] * xf_next;
+    int steps = 0;
+    while ((1 << steps) < nI) steps++;
+    for (int s = 0; s < steps; s++) {
         const int dd = 1 << s;
         const T Rm = shfl(R, lane - dd);
         const T Rp = shfl(R, lane + dd);
-        if (has_iface) R = R - c_k1[s] * Rm - c_k2[s] * Rp;
+        if (has_iface) R = R - k1[s * kMaxLanes + j] * Rm - k2[s * kMaxLanes + j] * Rp;
     }
-    const T xI = has_iface ? R * c_ib : T(0);
+    const T xI = has_iface ? R * invb[j] : T(0);
     T xl = shfl(xI, lane - 1);
     if (j == 0) xl = T(0);
     const T xr = xI;
@@ -250,7 +228,7 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
         const int row = min(st + jj, n - 1);
-        T x = (jj < mi) ? d[jj] + c_vl[jj] * xl + c_vr[jj] * xr : xr;
+        T x = (jj < mi) ? d[jj] + vL[row] * xl + vR[row] * xr : xr;
         o[jj] = scale * x;
         if (post) o[jj] *= post[row];
     }
@@ -267,7 +245,6 @@ __global__ __launch_bounds__(256) void k_deriv_contig(const DerivLaunch<T> L) {
             if (jj < m) out[pbase + (long)(st + jj) * L.srow] = o[jj];
     }
     }   // active
-    }   // b
 }
 
 template <typename T>
@@ -276,11 +253,11 @@ hipError_t launch_deriv_contig(const DerivLaunch<T> &L, int order, hipStream_t s
     if (MR < 0 || L.njobs < 1 || L.njobs > kMaxJobs) return hipErrorInvalidValue;
     const int Lw = L.S;
     const int ppw = 64 / Lw;
-    const int wpb = 4;
+    // 16 wavefronts per workgroup share one staging of the plan (5n + 960 values)
+    const int wpb = 16;
     const unsigned gx = (unsigned)((L.na + (long)ppw * wpb - 1) / ((long)ppw * wpb));
-    const unsigned gy = (unsigned)((L.nb + kContigPB - 1) / kContigPB);
-    dim3 grid(gx, gy, L.njobs), block(64 * wpb);
-    const size_t shm = 0;
+    dim3 grid(gx, L.nb, L.njobs), block(64 * wpb);
+    const size_t shm = (size_t)plan_contig_len(L.n) * sizeof(T);
 #define CSSX_LAUNCH_C(MM)                                                                                \
     do {                                                                                                 \
         if (order == 1) hipLaunchKernelGGL((k_deriv_contig<T, 1, MM>), grid, block, shm, stream, L);     \
