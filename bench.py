@@ -117,6 +117,85 @@ def precision_study(sol, y0, dt, steps, nv, every=10):
             "max_rel_err_over_run": worst, "meets_1e-5": worst <= 1e-5, "every": rows}
 
 
+def run_c5(args):
+    """Learned closure on the 256x512x1024 recipe (seeds 5/6) box-filtered by 4 to 64x128x256:
+    training snapshots from a fine FP32-bulk run, training on the GPU, the a-priori gain on a
+    held-out later snapshot, then fine vs coarse (plain SLD) vs coarse + closure runs from the
+    same initial state, their statistics against the filtered fine run and their wall times."""
+    import numpy as np
+    import torch
+
+    from cssx import closure as CL
+    from cssx.mhd import Solver
+    from cssx.model import Config, Model
+    factor = 4
+    fc = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=args.sld_coef, seed_s=5, seed_a=6)
+    fine = Solver(Model(fc))
+    coarse = Solver(Model(CL.coarse_config(fc, factor)))
+    y = torch.as_tensor(fine.model.initial_state(), device="cuda").contiguous()
+    dt = fine.cfl(y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    snaps = []
+    for k in range(args.c5_snapshots):
+        if k:
+            fine.rk4(y, dt, args.c5_every, "f32")
+        snaps.append(CL.snapshot(fine, coarse, y, factor, k * args.c5_every * dt))
+        print(f"c5: snapshot {k}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    del y
+    fine.close()
+    coarse.close()
+    ts = CL.build_training_set(snaps[:-1], args.c5_samples, seed=7)
+    n = ts.X.shape[0]
+    net = CL.Network(128, 128, max_rows=262144, seed=7)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = CL.train(net, ts, epochs=args.c5_epochs, batch=8192, lr=1e-3, seed=7)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"c5: trained {args.c5_epochs} epochs in {t_train:.2f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}",
+          file=sys.stderr, flush=True)
+    before, after = CL.apriori_gain(net, snaps[-1], ts)
+    del snaps, ts
+    torch.cuda.empty_cache()
+    recs, times = CL.compare(fc, factor, net, args.c5_steps, args.c5_stat_every, "f32")
+    ep, kp = CL.stat_errors(recs, "coarse")
+    ec, kc = CL.stat_errors(recs, "coarse_closure")
+    flops_train = 3 * 90112.0 * n * len(hist)
+    ach = flops_train / t_train / 1e12
+    speedup = times["fine_s"] / max(times["coarse_closure_s"], 1e-12)
+    worst_c = float(max(ec.max(), kc))
+    out = {
+        "metric": METRIC + " [closure speedup]",
+        "value": round(speedup, 2),
+        "unit": "x (wall filtered-fine / wall coarse+closure)",
+        "n_gpus": 1, "steps": args.c5_steps, "warmup": 0, "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "bf16 network (f32 accumulate), f32-bulk solver", "data": "synthetic",
+        "config": {"workload": f"closure_{fc.nr}x{fc.nth}x{fc.nphi}_to_{fc.nr // 4}x{fc.nth // 4}x{fc.nphi // 4}",
+                   "factor": factor, "snapshots": args.c5_snapshots, "snapshot_every_fine_steps": args.c5_every,
+                   "samples": n, "epochs": len(hist), "batch": 8192, "lr": 1e-3, "network": "216-128-128-8 GELU",
+                   "coarse_steps": args.c5_steps, "dt_coarse": times["dt_coarse"], "dt_fine": times["dt_fine"]},
+        "wall_s": {"fine_run_and_filter": round(times["fine_s"], 3), "coarse_plain": round(times["coarse_plain_s"], 3),
+                   "coarse_closure": round(times["coarse_closure_s"], 3), "training_data": round(t_data, 3),
+                   "training": round(t_train, 3)},
+        "stats_max_rel_err": {"coarse_plain": {"fields": [float(f"{x:.3e}") for x in ep], "ke": float(f"{kp:.3e}")},
+                              "coarse_closure": {"fields": [float(f"{x:.3e}") for x in ec], "ke": float(f"{kc:.3e}")},
+                              "closure_within_5pct": worst_c <= 0.05},
+        "apriori_residual_rms": {"fields": list(CL.FIELDS), "plain": [float(f"{x:.3e}") for x in before],
+                                 "closure": [float(f"{x:.3e}") for x in after],
+                                 "ratio": [float(f"{x:.3f}") for x in after / before]},
+        "training_loss": [float(f"{x:.4g}") for x in hist],
+        "roofline": {"kernel": "closure training (k_gemm_bf16 + data movement)", "bound": "mfma",
+                     "achieved": round(ach, 3), "peak": 2500.0, "unit": "TFLOP/s", "frac": round(ach / 2500.0, 6),
+                     "traffic": None},
+        "records": recs,
+    }
+    net.close()
+    print(json.dumps(_clean(out)))
+
+
 def _clean(o):
     """NaN/inf (a precision path that was not run) -> null, so the line stays strict JSON."""
     if isinstance(o, dict):
@@ -134,7 +213,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["c2", "c4"], default="c2",
+    ap.add_argument("--c5-snapshots", type=int, default=6, help="closure: fine snapshots (the last is held out)")
+    ap.add_argument("--c5-every", type=int, default=20, help="closure: fine steps between snapshots")
+    ap.add_argument("--c5-samples", type=int, default=2_000_000, help="closure: training rows")
+    ap.add_argument("--c5-epochs", type=int, default=10)
+    ap.add_argument("--c5-steps", type=int, default=100, help="closure: coarse steps of the comparison runs")
+    ap.add_argument("--c5-stat-every", type=int, default=10)
+    ap.add_argument("--config", choices=["c2", "c4", "c5"], default="c2",
                     help="c2: one 128x256x512 subdomain (the headline); c4: the 256x512x1024 precision study")
     ap.add_argument("--nr", type=int, default=0)
     ap.add_argument("--nth", type=int, default=0)
@@ -147,7 +232,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
-    shape = {"c2": (128, 256, 512), "c4": (256, 512, 1024)}[args.config]
+    shape = {"c2": (128, 256, 512), "c4": (256, 512, 1024), "c5": (256, 512, 1024)}[args.config]
     args.nr, args.nth, args.nphi = (args.nr or shape[0], args.nth or shape[1], args.nphi or shape[2])
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -161,6 +246,9 @@ def main():
     from cssx.model import Config, Model
 
     torch.cuda.set_device(0)
+    if args.config == "c5":
+        run_c5(args)
+        return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
         a = torch.ones(n, dtype=torch.float32, device="cuda")

@@ -34,9 +34,13 @@ int get_plan(const PlanKey &key, int prec, const void **out) {
     }
     std::vector<double> plan;
     if (key.variant == 1) {
-        const int MR = choose_MR(key.n);
-        if (MR < 0) return CSSX_EINVAL;
+        const int MR = key.seg;
+        if (MR != 4 && MR != 8 && MR != 16) return CSSX_EINVAL;
+        if ((key.n + MR - 1) / MR > kMaxLanes) return CSSX_EINVAL;
         build_plan_contig(key, MR, plan);
+    } else if (key.variant == 2) {
+        if (key.seg <= 0 || (key.n + key.seg - 1) / key.seg > kMaxLanes) return CSSX_EINVAL;
+        build_plan_contig(key, key.seg, plan);
     } else {
         const int M = key.seg;
         if (M <= 0 || seg_count(key.n, M) > kMaxSeg) return CSSX_EINVAL;
@@ -83,15 +87,15 @@ void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g) {
 // its MR rows with the row stride, and neighbouring pencils of the same workgroup share the
 // cache lines.  Its D2 upper-wall closure needs five rows below row n-1 inside one lane's
 // window, i.e. the last lane must own >= 2 rows.
-static bool use_contig(const AxisGeom &g, int order, int hi_kind) {
-    const int MR = choose_MR(g.n);
+static bool use_contig(const AxisGeom &g, int order, int hi_kind, bool f64) {
+    const int MR = choose_MR(g.n, f64);
     if (MR < 0 || (g.srow != 1 && g.n <= 512)) return false;
     if (order == 2 && hi_kind == END_WALL && g.n % MR == 1) return false;
     return true;
 }
 
-static int lanes_per_pencil(int n) {
-    const int nl = (n + choose_MR(n) - 1) / choose_MR(n);
+static int lanes_per_pencil(int n, int MR) {
+    const int nl = (n + MR - 1) / MR;
     int Lw = 1;
     while (Lw < nl) Lw *= 2;
     return Lw;
@@ -110,15 +114,24 @@ template <typename T>
 static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStream_t stream) {
     DerivLaunch<T> L;
     fill_launch_geom(L, g);
-    bool contig = true;
-    for (int i = 0; i < nspec; i++) contig = contig && use_contig(g, (&sp)[i].order, (&sp)[i].hi_kind);
-    if (contig) L.S = lanes_per_pencil(g.n);
+    // long strided pencils: the wide kernel (several segments per wavefront); its D2 upper-wall
+    // closure, like the contiguous kernel's, needs the last segment to own >= 2 rows
+    int WM = 0, WG = 0;
+    if (g.srow != 1) wide_shape(g.n, sizeof(T) == 8, WM, WG);
+    bool wide = WM > 0;
+    for (int i = 0; i < nspec && wide; i++)
+        if ((&sp)[i].order == 2 && (&sp)[i].hi_kind == END_WALL && g.n % WM == 1) wide = false;
+    bool contig = !wide;
+    const bool f64 = sizeof(T) == 8;
+    for (int i = 0; i < nspec; i++) contig = contig && use_contig(g, (&sp)[i].order, (&sp)[i].hi_kind, f64);
+    if (contig) { L.M = choose_MR(g.n, f64); L.S = lanes_per_pencil(g.n, L.M); }
+    if (wide) { L.M = WM; L.S = (g.n + WM - 1) / WM; }
     for (int i = 0; i < nspec; i++) {
         const DerivSpec &s = (&sp)[i];
         PlanKey key{g.n, s.order, s.lo_kind == END_WALL ? END_WALL : END_HALO,
                     s.hi_kind == END_WALL ? END_WALL : END_HALO, (s.lo_kind == END_WALL || s.hi_kind == END_WALL) ? s.bdry : 0};
-        key.variant = contig ? 1 : 0;
-        key.seg = contig ? 0 : L.M;
+        key.variant = wide ? 2 : contig ? 1 : 0;
+        key.seg = L.M;
         const void *plan = nullptr;
         int rc = get_plan(key, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
         if (rc) return rc;
@@ -134,7 +147,8 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
         if (L.njobs == kMaxJobs || i == nspec - 1) {
             const int phase = g.phase;
             const int tok = timing_begin(phase, stream);
-            hipError_t e = contig ? launch_deriv_contig<T>(L, s.order, stream) : launch_deriv<T>(L, s.order, stream);
+            hipError_t e = wide ? launch_deriv_wide<T>(L, s.order, stream)
+                                : contig ? launch_deriv_contig<T>(L, s.order, stream) : launch_deriv<T>(L, s.order, stream);
             timing_end(tok, stream, (double)L.njobs * g.n * (double)g.na * g.nb * 2.0 * sizeof(T));
             if (e != hipSuccess) return CSSX_EHIP;
             L.njobs = 0;
@@ -202,8 +216,8 @@ extern "C" int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out) {
                             if (var == 1 && (choose_MR(ns[ax]) < 0 || (ax != CSSX_AXIS_THETA && ns[ax] <= 512)))
                                 continue;
                             key.variant = var;
-                            key.seg = var == 0 ? strided_M(ns[ax]) : 0;
                             for (int prec = 0; prec < 2; prec++) {
+                                key.seg = var == 0 ? strided_M(ns[ax]) : choose_MR(ns[ax], prec == CSSX_PREC_F64);
                                 int rc = get_plan(key, prec, &p);
                                 if (rc) {
                                     delete c;
@@ -211,6 +225,19 @@ extern "C" int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out) {
                                 }
                             }
                         }
+                        if (ax != CSSX_AXIS_THETA)
+                            for (int prec = 0; prec < 2; prec++) {
+                                int WM, WG;
+                                wide_shape(ns[ax], prec == CSSX_PREC_F64, WM, WG);
+                                if (WM == 0) continue;
+                                key.variant = 2;
+                                key.seg = WM;
+                                int rc = get_plan(key, prec, &p);
+                                if (rc) {
+                                    delete c;
+                                    return rc;
+                                }
+                            }
                     }
     *out = c;
     return CSSX_OK;

@@ -16,53 +16,12 @@
 // point, the algorithmic minimum.
 #include <hip/hip_runtime.h>
 
+#include "cssx_bufio.hpp"
 #include "cssx_kernels.hpp"
 #include "cssx_plan.hpp"
 #include "cssx_rows.hpp"
 
 namespace cssx {
-
-// element at a wave-uniform row base plus a 32-bit per-lane byte offset
-template <typename T>
-__device__ __forceinline__ T &at_row(const T *row, unsigned byte_ofs) {
-    return *reinterpret_cast<T *>(const_cast<char *>(reinterpret_cast<const char *>(row)) + byte_ofs);
-}
-
-// Raw buffer access to one field: the lane's byte offset stays in one VGPR for every row and
-// the row's byte offset is a scalar register (buffer_load ... v_lane, s[rsrc], s_row offen),
-// so a sweep row costs one memory instruction and no address arithmetic per lane.  The
-// resource spans 2^32-1 bytes; the host checks that a launch's offsets fit.
-typedef unsigned int cssx_u32x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void *base) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, -1, 0x00020000);
-}
-template <typename T>
-__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs);
-template <>
-__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vofs, sofs, 0));
-}
-template <>
-__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vofs, sofs, 0));
-}
-__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, vofs, sofs, 0);
-}
-__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned vofs, unsigned sofs) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(cssx_u32x2, v), r, vofs, sofs, 0);
-}
-
-// v from lane l of the wavefront as a wave-uniform value (v_readlane)
-__device__ __forceinline__ float lane_val(float v, int l) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-}
-__device__ __forceinline__ double lane_val(double v, int l) {
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
 
 // closure rows of the pencil's last two rows; jj = local index of row n-1 in the window
 template <typename T, int ORDER>

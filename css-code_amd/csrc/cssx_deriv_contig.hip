@@ -45,14 +45,23 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     T *sp = reinterpret_cast<T *>(smem_raw);
     const DerivJob<T> &J = L.jobs[blockIdx.z];
     const int n = L.n;
-    const int plen = plan_contig_len(n);
-    for (int i = threadIdx.x; i < plen; i += blockDim.x) sp[i] = J.plan[i];
+    // The five row tables are staged transposed, row j*MR+jj of lane j at [jj*nl + j], so the
+    // lanes of a wavefront read consecutive LDS words (no bank conflicts); the interface tables
+    // follow them unchanged.
+    const int nlT = (n + MR - 1) / MR, nT = nlT * MR;
+    for (int i = threadIdx.x; i < 5 * n; i += blockDim.x) {
+        const int c = i / n, r = i - c * n;
+        const int jl = r / MR, jr = r - jl * MR;
+        sp[c * nT + jr * nlT + jl] = J.plan[i];
+    }
+    constexpr int tail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
+    for (int i = threadIdx.x; i < tail; i += blockDim.x) sp[5 * nT + i] = J.plan[5 * n + i];
     __syncthreads();
     // unit-stride pencils whose length is a multiple of MR move as 16-byte vectors
     const bool vec = L.srow == 1 && n % MR == 0;
     const int b = blockIdx.y;
-    const T *cf = sp, *cl = cf + n, *cu = cl + n, *vL = cu + n, *vR = vL + n;
-    const T *rl = sp + 5 * n, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
+    const T *cf = sp, *cl = cf + nT, *cu = cl + nT, *vL = cu + nT, *vR = vL + nT;
+    const T *rl = sp + 5 * nT, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
             *invb = k2 + kPcrSteps * kMaxLanes;
 
     constexpr int W = MR + 8;
@@ -98,15 +107,23 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     }
 #pragma unroll
     for (int k = 0; k < MR; k++) w[4 + k] = v[k];
-    // rows outside the pencil: ghosts (halo end), or zero (wall end; not read by wall rows)
+    // rows outside the pencil: ghosts (halo end), the wrapped rows (periodic pencil), or zero
+    // (wall end; not read by wall rows)
 #pragma unroll
     for (int i = 0; i < W; i++) {
         const int q = st - 4 + i;
         if (q < 0 || q >= n) {
             T g = T(0);
-            if (pen_ok) {
-                if (q < 0 && q >= -4 && J.lo_kind == END_HALO) g = J.glo[gbase + (long)(q + 4) * J.g_k];
-                if (q >= n && q < n + 4 && J.hi_kind == END_HALO) g = J.ghi[gbase + (long)(q - n) * J.g_k];
+            if (pen_ok && q >= -4 && q < n + 4) {
+                if (q < 0) {
+                    if (J.lo_kind == END_HALO) g = J.glo[gbase + (long)(q + 4) * J.g_k];
+                    else if (J.lo_kind == END_PERIODIC) g = in[pbase + (long)(q + n) * L.srow];
+                } else {
+                    if (J.hi_kind == END_HALO) g = J.ghi[gbase + (long)(q - n) * J.g_k];
+                    else if (J.hi_kind == END_PERIODIC) g = in[pbase + (long)(q - n) * L.srow];
+                }
+                if (J.pre && (q < 0 ? J.lo_kind : J.hi_kind) == END_PERIODIC)
+                    g *= J.pre[q < 0 ? q + n : q - n];
             }
             w[i] = g;
         }
@@ -187,7 +204,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
         if (jj < mi) {
-            y = cf[st + jj] * d[jj] - cl[st + jj] * y;
+            y = cf[jj * nlT + j] * d[jj] - cl[jj * nlT + j] * y;
             d[jj] = y;
         }
     }
@@ -195,7 +212,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = MR - 1; jj >= 0; jj--) {
         if (jj < mi) {
-            xn = d[jj] - cu[st + jj] * xn;
+            xn = d[jj] - cu[jj * nlT + j] * xn;
             d[jj] = xn;
         }
     }
@@ -228,7 +245,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
         const int row = min(st + jj, n - 1);
-        T x = (jj < mi) ? d[jj] + vL[row] * xl + vR[row] * xr : xr;
+        T x = (jj < mi) ? d[jj] + vL[jj * nlT + j] * xl + vR[jj * nlT + j] * xr : xr;
         o[jj] = scale * x;
         if (post) o[jj] *= post[row];
     }
@@ -249,15 +266,16 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 
 template <typename T>
 hipError_t launch_deriv_contig(const DerivLaunch<T> &L, int order, hipStream_t stream) {
-    const int MR = choose_MR(L.n);
-    if (MR < 0 || L.njobs < 1 || L.njobs > kMaxJobs) return hipErrorInvalidValue;
+    const int MR = L.M;
+    if ((MR != 4 && MR != 8 && MR != 16) || (L.n + MR - 1) / MR > kMaxLanes || L.njobs < 1 || L.njobs > kMaxJobs)
+        return hipErrorInvalidValue;
     const int Lw = L.S;
     const int ppw = 64 / Lw;
     // 16 wavefronts per workgroup share one staging of the plan (5n + 960 values)
     const int wpb = 16;
     const unsigned gx = (unsigned)((L.na + (long)ppw * wpb - 1) / ((long)ppw * wpb));
     dim3 grid(gx, L.nb, L.njobs), block(64 * wpb);
-    const size_t shm = (size_t)plan_contig_len(L.n) * sizeof(T);
+    const size_t shm = (size_t)(5 * MR * ((L.n + MR - 1) / MR) + 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes) * sizeof(T);
 #define CSSX_LAUNCH_C(MM)                                                                                \
     do {                                                                                                 \
         if (order == 1) hipLaunchKernelGGL((k_deriv_contig<T, 1, MM>), grid, block, shm, stream, L);     \

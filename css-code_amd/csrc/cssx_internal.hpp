@@ -4,6 +4,8 @@
 
 #include "cssx_plan.hpp"
 
+struct cssx_mlp;
+
 namespace cssx {
 
 // Pencil geometry of one axis on the reference layout (see axis_geometry).
@@ -32,6 +34,10 @@ struct DerivSpec {
 // per-phase event timing (cssx_timing.hip); token < 0 = disabled
 int timing_begin(int phase, hipStream_t stream);
 void timing_end(int token, hipStream_t stream, double bytes);
+
+// learned closure tendency of a whole state (cssx_closure.hip)
+int closure_tendency(cssx_mlp *n, int prec, const void *st, int fluct, const void *rbar, const void *sbar, int nr,
+                     int nphi, int nth, float *ext, hipStream_t s);
 
 void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g);
 int get_plan(const PlanKey &key, int prec, const void **out);

@@ -33,7 +33,7 @@ struct DerivJob {
 template <typename T>
 struct DerivLaunch {
     int n, S;           // pencil length, segments per pencil (contiguous kernel: lanes per pencil)
-    int M;              // strided kernel: rows-per-segment template size (S = ceil(n/M))
+    int M;              // strided kernels: rows-per-segment template size (S = ceil(n/M))
     int na, nb;         // pencils: fast index a (lanes) x slow index b (grid.y)
     long sa, sb, srow;  // element strides of a, b and along the pencil
     int njobs;
@@ -44,5 +44,9 @@ template <typename T>
 hipError_t launch_deriv(const DerivLaunch<T> &L, int order, hipStream_t stream);
 template <typename T>
 hipError_t launch_deriv_contig(const DerivLaunch<T> &L, int order, hipStream_t stream);
+template <typename T>
+hipError_t launch_deriv_wide(const DerivLaunch<T> &L, int order, hipStream_t stream);
+// (M, G) of the wide strided kernel for a pencil of n rows (M = 0: k_deriv serves it)
+void wide_shape(int n, bool f64, int &M, int &G);
 
 }  // namespace cssx

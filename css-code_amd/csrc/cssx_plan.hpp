@@ -23,8 +23,9 @@ enum EndKind : int { END_WALL = 0, END_HALO = 1, END_PERIODIC = 2 };
 
 struct PlanKey {
     int n, order, lo, hi, bdry;
-    int variant = 0;   // 0: strided (waves = segments), 1: contiguous (lanes = segments)
-    int seg = 0;       // strided: rows-per-segment size M the plan is cut for
+    int variant = 0;   // 0: strided (waves = segments), 1: contiguous (lanes = segments),
+                       // 2: wide strided (contiguous-kernel plan cut at M rows)
+    int seg = 0;       // strided / wide: rows-per-segment size M the plan is cut for
     bool operator<(const PlanKey &o) const {
         if (variant != o.variant) return variant < o.variant;
         if (seg != o.seg) return seg < o.seg;
@@ -97,9 +98,11 @@ void build_plan(const PlanKey &key, int S, std::vector<double> &plan);
 constexpr int kMaxLanes = 64;
 constexpr int kPcrSteps = 6;
 __host__ __device__ inline int plan_contig_len(int n) { return 5 * n + 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes; }
-// rows per lane for a pencil of n rows (4, 8 or 16), -1 if n > 1024
-inline int choose_MR(int n) {
-    if (n <= 64 * 4) return 4;
+// rows per lane for a pencil of n rows (4, 8 or 16), -1 if n > 1024.  FP32 takes 8 rows per
+// lane from 256 rows down (two pencils per wavefront, half the shuffles and PCR steps per
+// row); FP64 keeps 4 there, where 8 would cost it occupancy.
+inline int choose_MR(int n, bool f64 = true) {
+    if (n <= 64 * 4 && f64) return 4;
     if (n <= 64 * 8) return 8;
     if (n <= 64 * 16) return 16;
     return -1;

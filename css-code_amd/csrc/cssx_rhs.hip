@@ -100,6 +100,8 @@ struct cssx_mhd {
     double *save = nullptr;   // FP64 RK accumulator (nv fields)
     double *red = nullptr;    // reduction partials
     int nred = 0;
+    cssx_mlp *closure = nullptr;   // attached learned closure (not owned) or null
+    float *ext = nullptr;          // its tendency (8 fields, FP32)
 };
 
 namespace cssx {
@@ -531,7 +533,8 @@ __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<
 template <typename T>
 __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const T *st,
                                                    const T *__restrict__ B, const T *__restrict__ dth_rho,
-                                                   const T *__restrict__ dr_rho, T *dv, StageFuse<T> F) {
+                                                   const T *__restrict__ dr_rho, T *dv, StageFuse<T> F,
+                                                   const float *__restrict__ ext) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.N) return;
     const long N = c.N;
@@ -573,8 +576,22 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
         da1 = U * Bt - W * Br;
         da2 = W * Bp - V * Bt;
     }
+    T dvv = (F.stage != 0 || ext) ? dv[2 * N + i] : T(0);
+    if (ext) {   // learned closure tendency
+        drho += (T)ext[i];
+        dw += (T)ext[N + i];
+        dvv += (T)ext[2 * N + i];
+        du += (T)ext[3 * N + i];
+        ds += (T)ext[4 * N + i];
+        if (c.magnetic) {
+            da0 += (T)ext[5 * N + i];
+            da1 += (T)ext[6 * N + i];
+            da2 += (T)ext[7 * N + i];
+        }
+    }
     if (F.stage == 0) {
         dv[i] = drho;
+        if (ext) dv[2 * N + i] = dvv;
         dv[N + i] = dw;
         dv[3 * N + i] = du;
         dv[4 * N + i] = ds;
@@ -586,7 +603,6 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
     } else {
         // st may alias F.y1 (the stage state is updated in place): every read of st at this
         // point happened above
-        const T dvv = dv[2 * N + i];
         stage_value(F, c, 0, i, it, ip, ir, drho);
         stage_value(F, c, 1, i, it, ip, ir, dw);
         stage_value(F, c, 2, i, it, ip, ir, dvv);
@@ -1014,13 +1030,20 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, &rr, 1, stream))) return rc;
         StageFuse<T> F0{};
         const StageFuse<T> &FF = F ? *F : F0;
+        const float *ext = nullptr;
+        if (m->closure) {
+            rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, c.fluct, P.rbar,
+                                  P.sbar, m->nr, m->nphi, m->nth, m->ext, stream);
+            if (rc) return rc;
+            ext = m->ext;
+        }
         // plain: 15/21 values per point; a fused stage adds the stage traffic of every field
         const double fb = !F ? 0.0
                           : sizeof(T) == 8 ? (FF.stage < 4 ? 40.0 - 8.0 : 32.0 - 8.0) * m->nv
                                            : (FF.stage == 1 ? 12.0 + (FF.dif ? 4.0 : 0.0) : FF.stage < 4 ? 16.0 : 24.0) * m->nv;
         CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T) + fb * N,
                    hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N,
-                                      D + 1 * N, dv, FF));
+                                      D + 1 * N, dv, FF, ext));
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
@@ -1263,9 +1286,18 @@ extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
     for (void *p : {(void *)m->p64.buf, (void *)m->p32.buf, (void *)m->s64.D, (void *)m->s64.B, (void *)m->s64.wall,
                     (void *)m->s64.y1, (void *)m->s64.k, (void *)m->s32.D, (void *)m->s32.B, (void *)m->s32.wall,
                     (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red, (void *)m->s64.dif,
-                    (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f})
+                    (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f, (void *)m->ext})
         if (p) (void)hipFree(p);
     delete m;
+}
+
+extern "C" int cssx_mhd_set_closure(cssx_mhd *m, cssx_mlp *net) {
+    if (!m) return CSSX_EINVAL;
+    if (net && m->nv != 8) return CSSX_EINVAL;
+    if (net && !m->ext && hipMalloc((void **)&m->ext, (size_t)8 * m->N * sizeof(float)) != hipSuccess)
+        return CSSX_ENOMEM;
+    m->closure = net;
+    return CSSX_OK;
 }
 
 extern "C" int cssx_rhs(cssx_mhd *m, int prec, const void *state, void *rhs, void *b_out, void *stream) {

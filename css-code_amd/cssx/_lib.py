@@ -24,6 +24,7 @@ PHASES = ("deriv_r", "deriv_theta", "deriv_phi", "pointwise", "rk_update", "othe
 
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class CssxError(RuntimeError):
@@ -66,6 +67,30 @@ SIGNATURES = {
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cssx_plan_tables": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                           _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    # learned closure
+    "cssx_mlp_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
+                         ctypes.POINTER(_vp)], ctypes.c_int),
+    "cssx_mlp_destroy": ([_vp], None),
+    "cssx_mlp_num_params": ([_vp], ctypes.c_long),
+    "cssx_mlp_set_params": ([_vp, _fp], ctypes.c_int),
+    "cssx_mlp_get_params": ([_vp, _fp, _fp], ctypes.c_int),
+    "cssx_mlp_set_norm": ([_vp, _fp, _fp, _fp, _fp], ctypes.c_int),
+    "cssx_mlp_forward": ([_vp, _vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "cssx_mlp_grad": ([_vp, _vp, _vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
+    "cssx_mlp_train_step": ([_vp, _vp, _vp, ctypes.c_int, ctypes.c_float, _vp, _vp], ctypes.c_int),
+    "cssx_gemm_bf16": ([_vp, ctypes.c_long, ctypes.c_long, _vp, ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                        ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_long, _vp], ctypes.c_int),
+    "cssx_box_filter": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         _vp, _vp], ctypes.c_int),
+    "cssx_closure_features": ([_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, ctypes.c_long, ctypes.c_long, _vp, _vp, _vp], ctypes.c_int),
+    "cssx_closure_targets": ([_vp, _vp, ctypes.c_int, ctypes.c_long, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cssx_mhd_set_closure": ([_vp, _vp], ctypes.c_int),
+    "cssx_colstats": ([_vp, ctypes.c_long, ctypes.c_int, _dp, _dp, _vp], ctypes.c_int),
+    "cssx_field_stats": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                          _vp, _vp, _dp, _dp, _vp], ctypes.c_int),
+    "cssx_gather_rows": ([_vp, ctypes.c_long, _vp, ctypes.c_long, _vp, _vp], ctypes.c_int),
+    "cssx_standardize_rows": ([_vp, ctypes.c_long, ctypes.c_int, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
 }
 
 

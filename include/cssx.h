@@ -206,6 +206,66 @@ int cssx_timing_read(double *ms, long *count, double *bytes);
 int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, int variant, double *plan_out,
                      int *nseg);
 
+/* ---- learned subgrid closure ------------------------------------------------------------------
+ * No counterpart in the reference: a coarse-grid alternative to (or correction on top of) the
+ * SLD subgrid diffusion of diffusion.f90, for callers that run a coarse grid in place of a
+ * fine one.  A 216 -> h1 -> h2 -> 8 GELU network maps the 3x3x3 stencil of the 8 fields
+ * (rho and s as fluctuations about the hydrostatic background) to the 8-field tendency that the
+ * coarse RHS misses.  Matrices run on the BF16 matrix cores with FP32 accumulation; the master
+ * weights and Adam moments are FP32.  Parameters, flattened (host order):
+ *   W1 (216 x h1, row-major), b1 (h1), W2 (h1 x h2), b2 (h2), W3 (h2 x 8), b3 (8)
+ * Features of point p (row p): column q = field*27 + (dr+1)*9 + (dphi+1)*3 + (dth+1), the
+ * neighbour clamped at r and theta walls and wrapped in phi, standardized as (x - mean)/std.
+ * Network inputs are BF16 row-major (rows x 216); outputs/targets FP32 row-major (rows x 8),
+ * standardized; the closure tendency is output*y_std + y_mean. */
+typedef struct cssx_mlp cssx_mlp;
+int cssx_mlp_create(int nin, int h1, int h2, int nout, int max_rows, unsigned long long seed, cssx_mlp **out);
+void cssx_mlp_destroy(cssx_mlp *net);
+long cssx_mlp_num_params(const cssx_mlp *net);
+int cssx_mlp_set_params(cssx_mlp *net, const float *host_params);
+/* either pointer may be NULL; grads are those of the last cssx_mlp_grad / _train_step */
+int cssx_mlp_get_params(cssx_mlp *net, float *host_params, float *host_grads);
+int cssx_mlp_set_norm(cssx_mlp *net, const float *x_mean, const float *x_std, const float *y_mean,
+                      const float *y_std);
+int cssx_mlp_forward(cssx_mlp *net, const void *x_bf16, int rows, float *y, void *stream);
+/* gradient of the mean squared error over rows x 8 outputs; *loss (device float) may be NULL */
+int cssx_mlp_grad(cssx_mlp *net, const void *x_bf16, const float *target, int rows, float *loss, void *stream);
+/* cssx_mlp_grad + one Adam step (beta 0.9/0.999, eps 1e-8, bias-corrected) */
+int cssx_mlp_train_step(cssx_mlp *net, const void *x_bf16, const float *target, int rows, float lr, float *loss,
+                        void *stream);
+/* C (M x N, FP32, row-major, ldc) = A B with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn],
+ * BF16 inputs; ksplit > 1 splits K over workgroups (requires ldc == N) */
+int cssx_gemm_bf16(const void *A, long sam, long sak, const void *B, long sbk, long sbn, int M, int N, int K,
+                   int ksplit, float *C, long ldc, void *stream);
+/* mean over factor^3 blocks: (nv, nr, nphi, nth) -> FP32 (nv, nr/f, nphi/f, nth/f) */
+int cssx_box_filter(const void *in, int prec, int nv, int nr, int nphi, int nth, int factor, float *out,
+                    void *stream);
+/* features of rows [row0, row0+rows) of a state (8 fields; fluct = 1 when slots 0 and 4 already
+ * hold fluctuations, else rho_bar/s_bar (nr, same precision) are subtracted): standardized BF16
+ * into x_bf16 (needs net's norm) and/or raw FP32 into x_raw */
+int cssx_closure_features(cssx_mlp *net, const void *state, int prec, int fluct, const void *rho_bar,
+                          const void *s_bar, int nr, int nphi, int nth, long row0, long rows, void *x_bf16,
+                          float *x_raw, void *stream);
+/* residual targets: raw[p][iv] = fine_filtered[iv][p] - coarse_rhs[iv][p] (row-major FP32) and
+ * target = (raw - y_mean)/y_std; either output may be NULL */
+int cssx_closure_targets(const float *fine_filtered, const void *coarse_rhs, int prec, long npts,
+                         const float *y_mean, const float *y_std, float *target, float *raw, void *stream);
+/* column mean and standard deviation (population) of a row-major FP32 matrix; synchronizes */
+int cssx_colstats(const float *X, long rows, int cols, double *mean, double *stddev, void *stream);
+/* per-field mean and root mean square of a field-major state (rho and s about the background
+ * rho_bar/s_bar (nr) unless fluct); synchronizes */
+int cssx_field_stats(const void *state, int prec, int nv, int nr, int nphi, int nth, int fluct, const void *rho_bar,
+                     const void *s_bar, double *mean, double *rms, void *stream);
+/* (x - mean[c]) / stddev[c] of a row-major FP32 (rows x cols) matrix (mean, stddev: device FP32)
+ * into BF16 and/or FP32 outputs (either may be NULL) */
+int cssx_standardize_rows(const float *x, long rows, int cols, const float *mean, const float *stddev,
+                          void *out_bf16, float *out_f32, void *stream);
+/* dst row i = src row idx[i] (idx: device int64), rows of row_bytes bytes (a multiple of 4) */
+int cssx_gather_rows(const void *src, long row_bytes, const long *idx, long n, void *dst, void *stream);
+/* attach (net != NULL) or detach a closure: every later RHS evaluation of m (cssx_rhs and the
+ * steppers' stages) adds the network's tendency of its input state.  net is not owned. */
+int cssx_mhd_set_closure(cssx_mhd *m, cssx_mlp *net);
+
 #ifdef __cplusplus
 }
 #endif

@@ -158,3 +158,43 @@ def test_deriv_analytic_sphere_c1(cssx_mod):
     assert np.abs(gt - ft_).max() < 2e-6 * np.abs(ft_).max()
     assert np.abs(gp - fp).max() < 1e-8 * np.abs(fp).max()
     ops.close()
+
+
+LONG_CASES = []
+for _axis, _shapes in ((0, [(1024, 8, 16), (600, 8, 12)]), (1, [(8, 8, 1024), (8, 9, 600)]),
+                       (2, [(8, 1024, 16), (8, 600, 12), (8, 513, 16)])):
+    for _shape in _shapes:
+        for _order in (1, 2):
+            for _lo, _hi in ([(1, 1), (0, 0), (1, 0)] if _axis < 2 else [(0, 0), (None, None)]):
+                LONG_CASES.append((_shape, (_axis, _order, _lo, _hi, 1 if _axis < 2 and (_lo or _hi) else 0)))
+
+
+@pytest.mark.parametrize("shape,case", LONG_CASES)
+def test_deriv_long_pencils_match_oracle(cssx_mod, shape, case):
+    """Pencils longer than 512 rows on every axis (the 256x512x1024 workload's phi axis is 1024
+    long, periodic): walls, halo ghosts and the periodic wrap against the oracle."""
+    _, O = cssx_mod
+    axis, order, lo, hi, bdry = case
+    print(f"long pencil {shape} {case}", flush=True)
+    nr, nphi, nth = shape
+    rng = np.random.default_rng(hash((shape, case, 7)) % (2 ** 32))
+    f = rng.standard_normal(shape)
+    dri, dti, dpi = 1.3 * (nr - 1), 0.7 * (nth - 1), 2.1 * nphi
+    periodic = axis == 2 and lo is None
+    walls_r = (bool(lo), bool(hi)) if axis == 0 else (True, True)
+    walls_t = (bool(lo), bool(hi)) if axis == 1 else (True, True)
+    ops = O.Operators(nr, nth, nphi, dri, dti, dpi, r_walls=walls_r, th_walls=walls_t)
+    glo, ghi = (None, None) if periodic else _ghost_buffers(axis, shape, rng, lo, hi)
+    scale = [dri, dti, dpi][axis]
+    sc = scale if order == 1 else scale * scale
+    hneu = scale if order == 1 else dri
+    ref = oracle.deriv_field(axis, order, f, lo_wall=bool(lo), hi_wall=bool(hi), bdry=bdry, ghost_lo=glo,
+                             ghost_hi=ghi, scale=sc, hneu=hneu)
+    for prec, tol in ((torch.float64, 1e-12), (torch.float32, 3e-5)):
+        ft = torch.as_tensor(f, dtype=prec, device="cuda")
+        gl = None if glo is None else torch.as_tensor(glo, dtype=prec, device="cuda")
+        gh = None if ghi is None else torch.as_tensor(ghi, dtype=prec, device="cuda")
+        got = ops.deriv(ft, axis, order, bdry, gl, gh).double().cpu().numpy()
+        err = _rel(got, ref)
+        assert err < tol, (shape, case, prec, err)
+    ops.close()
