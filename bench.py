@@ -196,6 +196,25 @@ def run_c5(args):
     print(json.dumps(_clean(out)))
 
 
+def measured_traffic(workload):
+    """The rocprofv3 FETCH_SIZE/WRITE_SIZE summary committed for this workload (FP32-bulk path):
+    the JSON block of profiles/hbm_traffic_<workload>.md, or None."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    i = text.find("```json")
+    if i < 0:
+        return None
+    j = text.find("```", i + 7)
+    try:
+        d = json.loads(text[i + 7:j])
+    except ValueError:
+        return None
+    d["source"] = os.path.relpath(path, ROOT)
+    return d
+
+
 def _clean(o):
     """NaN/inf (a precision path that was not run) -> null, so the line stays strict JSON."""
     if isinstance(o, dict):
@@ -335,6 +354,12 @@ def main():
 
     val32 = npts / (ms32 * 1e-3) / 1e6
     val64 = npts / (ms64 * 1e-3) / 1e6
+    workload = f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field"
+    tr = measured_traffic(workload)
+    roof32 = roofline(tab32)
+    if roof32 is not None and tr is not None and roof32["kernel"] in tr.get("families", {}):
+        roof32["traffic"] = tr["families"][roof32["kernel"]]["bytes_per_launch"]
+        roof32["traffic_source"] = tr["source"]
     out = {
         "metric": METRIC,
         "value": round(val32, 2) if "f32" in paths else round(val64, 2),
@@ -348,7 +373,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if "f32" in paths else "f64",
         "data": "synthetic",
-        "config": {"workload": f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field", "nr": c.nr, "nth": c.nth,
+        "config": {"workload": workload, "nr": c.nr, "nth": c.nth,
                    "nphi": c.nphi, "fields": c.nv, "accumulate": "f64", "dt": dt, "sld_coef": c.sld_coef,
                    "parallelism": "single-gpu"},
         "fp64": {"value": round(val64, 2), "unit": "Mpt-steps/s", "ms_per_step": round(ms64, 4)},
@@ -357,9 +382,11 @@ def main():
         "precision_study": study,
         "hbm_tbs_algorithmic": {"f32": round(val32 * 1e6 * STEP_BYTES_MIXED / 1e12, 3),
                                 "f64": round(val64 * 1e6 * STEP_BYTES_F64 / 1e12, 3)},
+        # rocprof FETCH+WRITE bytes of one FP32-bulk step (committed summary) over this run's step time
+        "hbm_tbs_measured_f32": (round(tr["total_GB_per_step"] / ms32, 3) if tr and "f32" in paths else None),
         "phases_ms_per_step": {p: ({k: round(v[0] / prof_steps, 4) for k, v in t.items()} if t else None)
                                for p, t in (("f32", tab32), ("f64", tab64))},
-        "roofline": roofline(tab32),
+        "roofline": roof32,
         "roofline_f64": roofline(tab64),
         "kernel_families": {p: (family_rows(t) if t else None) for p, t in (("f32", tab32), ("f64", tab64))},
     }

@@ -33,6 +33,13 @@ class CssxError(RuntimeError):
         super().__init__(f"{where}: cssx error {code} ({strerror(code)})")
 
 
+class Patch(ctypes.Structure):
+    """cssx_patch (include/cssx.h)."""
+    _fields_ = [("nr", ctypes.c_int), ("nth", ctypes.c_int), ("nphi", ctypes.c_int), ("in_", ctypes.c_void_p),
+                ("d_r", ctypes.c_void_p), ("d_th", ctypes.c_void_p), ("d_ph", ctypes.c_void_p),
+                ("bdry", ctypes.c_int), ("dri", ctypes.c_double), ("dti", ctypes.c_double), ("dpi", ctypes.c_double)]
+
+
 class Grid(ctypes.Structure):
     """cssx_grid (include/cssx.h)."""
     _fields_ = [("nr", ctypes.c_int), ("nth", ctypes.c_int), ("nphi", ctypes.c_int),
@@ -67,6 +74,13 @@ SIGNATURES = {
                           ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     "cssx_plan_tables": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                           _dp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+    # batched patches
+    "cssx_patchset_create": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "cssx_patchset_destroy": ([_vp], None),
+    "cssx_patchset_run": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
+    "cssx_patchset_autotune": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int), _fp], ctypes.c_int),
+    "cssx_patch_fill": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _vp],
+                        ctypes.c_int),
     # learned closure
     "cssx_mlp_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
                          ctypes.POINTER(_vp)], ctypes.c_int),

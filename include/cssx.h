@@ -206,6 +206,33 @@ int cssx_timing_read(double *ms, long *count, double *bytes);
 int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, int variant, double *plan_out,
                      int *nseg);
 
+/* ---- batched patches ---------------------------------------------------------------------------
+ * Many independent subdomains ("patches") of uneven shape f(nth, nphi, nr), each with walls on
+ * all six faces (the wall closures of derivatives.f90 with boundary code bdry on every axis,
+ * prescribed wall derivatives zero), differentiated along r, theta and phi in ONE persistent
+ * launch per cssx_patchset_run.  Pencil lengths 8..512.  Outputs that are NULL are skipped.
+ * Results are in index units times dri/dti/dpi, as cssx_deriv's. */
+typedef struct cssx_patch {
+    int nr, nth, nphi;
+    const void *in;
+    void *d_r, *d_th, *d_ph;
+    int bdry;
+    double dri, dti, dpi;
+} cssx_patch;
+typedef struct cssx_patchset cssx_patchset;
+/* builds every plan and the work-item tables of all (M, MR) variants on the current device;
+ * the patch pointers are fixed from here on */
+int cssx_patchset_create(const cssx_patch *patches, int npatch, int prec, cssx_patchset **out);
+void cssx_patchset_destroy(cssx_patchset *ps);
+/* one persistent launch over every item; variant -1 = the autotuned (or default) one */
+int cssx_patchset_run(cssx_patchset *ps, int variant, void *stream);
+/* times each variant (reps runs), keeps the fastest; synchronizes.  ms_out (4 floats) may be NULL */
+int cssx_patchset_autotune(cssx_patchset *ps, int reps, void *stream, int *best, float *ms_out);
+/* seeded N(0,1) field (SplitMix64 + Box-Muller over the flat index) smoothed by [1,2,1]/4 along
+ * each axis (edge values repeated); tmp: scratch of the same size */
+int cssx_patch_fill(void *out, void *tmp, int prec, int nr, int nth, int nphi, unsigned long long seed,
+                    void *stream);
+
 /* ---- learned subgrid closure ------------------------------------------------------------------
  * No counterpart in the reference: a coarse-grid alternative to (or correction on top of) the
  * SLD subgrid diffusion of diffusion.f90, for callers that run a coarse grid in place of a
