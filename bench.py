@@ -147,7 +147,10 @@ def run_c5(args):
     del y
     fine.close()
     coarse.close()
-    ts = CL.build_training_set(snaps[:-1], args.c5_samples, seed=7)
+    # interior-time snapshots 1 (validation: the per-field gate) and 3 (test) are held out
+    val_id, test_id = 1, 3
+    train_snaps = [sn for i, sn in enumerate(snaps) if i not in (val_id, test_id)]
+    ts = CL.build_training_set(train_snaps, args.c5_samples, seed=7)
     n = ts.X.shape[0]
     net = CL.Network(128, 128, max_rows=262144, seed=7)
     torch.cuda.synchronize()
@@ -157,8 +160,9 @@ def run_c5(args):
     t_train = time.perf_counter() - t0
     print(f"c5: trained {args.c5_epochs} epochs in {t_train:.2f} s, loss {hist[0]:.4f} -> {hist[-1]:.4f}",
           file=sys.stderr, flush=True)
-    before, after = CL.apriori_gain(net, snaps[-1], ts)
-    del snaps, ts
+    mask = CL.field_gate(net, snaps[val_id], ts)
+    before, after = CL.apriori_gain(net, snaps[test_id], ts, mask)
+    del snaps, ts, train_snaps
     torch.cuda.empty_cache()
     recs, times = CL.compare(fc, factor, net, args.c5_steps, args.c5_stat_every, "f32")
     ep, kp = CL.stat_errors(recs, "coarse")
@@ -183,7 +187,9 @@ def run_c5(args):
         "stats_max_rel_err": {"coarse_plain": {"fields": [float(f"{x:.3e}") for x in ep], "ke": float(f"{kp:.3e}")},
                               "coarse_closure": {"fields": [float(f"{x:.3e}") for x in ec], "ke": float(f"{kc:.3e}")},
                               "closure_within_5pct": worst_c <= 0.05},
-        "apriori_residual_rms": {"fields": list(CL.FIELDS), "plain": [float(f"{x:.3e}") for x in before],
+        "apriori_residual_rms": {"fields": list(CL.FIELDS), "gate": [bool(x) for x in mask],
+                                 "held_out": {"validation": val_id, "test": test_id},
+                                 "plain": [float(f"{x:.3e}") for x in before],
                                  "closure": [float(f"{x:.3e}") for x in after],
                                  "ratio": [float(f"{x:.3f}") for x in after / before]},
         "training_loss": [float(f"{x:.4g}") for x in hist],
@@ -232,7 +238,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--c5-snapshots", type=int, default=6, help="closure: fine snapshots (the last is held out)")
+    ap.add_argument("--c5-snapshots", type=int, default=8,
+                    help="closure: fine snapshots (1 and 3 are held out for the gate and the test)")
     ap.add_argument("--c5-every", type=int, default=20, help="closure: fine steps between snapshots")
     ap.add_argument("--c5-samples", type=int, default=2_000_000, help="closure: training rows")
     ap.add_argument("--c5-epochs", type=int, default=10)

@@ -549,8 +549,8 @@ extern "C" int cssx_mlp_set_norm(cssx_mlp *n, const float *x_mean, const float *
         if (!(x_std[i] > 0.0f)) return CSSX_EINVAL;
         xi[i] = 1.0f / x_std[i];
     }
-    for (int i = 0; i < n->nout; i++)
-        if (!(y_std[i] > 0.0f)) return CSSX_EINVAL;
+    for (int i = 0; i < n->nout; i++)   // y_std 0 switches that output's tendency off
+        if (!(y_std[i] >= 0.0f)) return CSSX_EINVAL;
     bool ok = hipMemcpy(n->xmean, x_mean, n->nin * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(n->xistd, xi.data(), n->nin * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(n->ymean, y_mean, n->nout * 4, hipMemcpyHostToDevice) == hipSuccess &&

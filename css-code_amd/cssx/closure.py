@@ -311,10 +311,9 @@ def train(net: Network, ts: TrainingSet, epochs: int, batch: int = 8192, lr: flo
     return hist
 
 
-def apriori_gain(net: Network, snap: Snapshot, ts: TrainingSet):
-    """Per-field RMS of the residual without and with the network's correction on a snapshot
-    (standardized with the training set's statistics): gain = 1 - rms(r - net)/rms(r)."""
-    xb = standardize(snap.x_raw, ts.x_mean, ts.x_std, True)
+def predict(net: Network, x_raw: torch.Tensor, ts: TrainingSet) -> torch.Tensor:
+    """Un-standardized network tendency (rows, 8) of raw feature rows (training-set scaling)."""
+    xb = standardize(x_raw, ts.x_mean, ts.x_std, True)
     n = xb.shape[0]
     pred = torch.empty((n, NOUT), dtype=torch.float32, device=xb.device)
     for r0 in range(0, n, net.max_rows):
@@ -322,10 +321,27 @@ def apriori_gain(net: Network, snap: Snapshot, ts: TrainingSet):
         pred[r0:r1] = net.forward(xb[r0:r1])
     ym = torch.as_tensor(ts.y_mean, dtype=torch.float32, device=xb.device)
     ys = torch.as_tensor(ts.y_std, dtype=torch.float32, device=xb.device)
-    p = pred * ys + ym
+    return pred * ys + ym
+
+
+def apriori_gain(net: Network, snap: Snapshot, ts: TrainingSet, mask=None):
+    """Per-field RMS of a snapshot's residual without and with the network's correction (fields
+    with mask False keep the plain residual)."""
+    p = predict(net, snap.x_raw, ts)
+    if mask is not None:
+        p = p * torch.as_tensor(np.asarray(mask, np.float32), device=p.device)
     before = snap.r_raw.pow(2).mean(0).sqrt().cpu().numpy()
     after = (snap.r_raw - p).pow(2).mean(0).sqrt().cpu().numpy()
     return before, after
+
+
+def field_gate(net: Network, validation: Snapshot, ts: TrainingSet):
+    """Fields whose residual the network reduces on a validation snapshot; the others fall back to
+    the plain coarse RHS (their correction is switched off in the network's norm)."""
+    before, after = apriori_gain(net, validation, ts)
+    mask = after < before
+    net.set_norm(ts.x_mean, ts.x_std, np.where(mask, ts.y_mean, 0.0), np.where(mask, ts.y_std, 0.0))
+    return mask
 
 
 # ---- coarse runs ----------------------------------------------------------------------------------

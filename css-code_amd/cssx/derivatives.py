@@ -69,10 +69,13 @@ def compact_fd6(dri, arr, b1, b2, ibc, dtype=0, darr=None, axis=-1):
         if not needed:
             return None
         bt = _to_dev(b, dev)
-        if bt.ndim == 0 or bt.shape[0] != 4:
+        if bt.numel() == 4:
+            return bt.reshape(4, 1).expand(4, npen).contiguous()
+        # per-pencil ghosts: shaped like arr with 4 along `axis`, or (4,) + the pencil shape
+        gax = ax if bt.ndim == a.ndim else 0
+        if bt.ndim == 0 or bt.shape[gax] != 4 or bt.numel() != 4 * npen:
             raise ValueError("compact_fd6: dtype %d needs 4 ghost values per pencil" % dtype)
-        return bt.reshape(4, -1).expand(4, npen).contiguous() if bt.numel() == 4 else \
-            torch.movedim(bt, ax if bt.ndim == a.ndim else 0, 0).reshape(4, npen).contiguous()
+        return torch.movedim(bt, gax, 0).reshape(4, npen).contiguous()
 
     g1 = ghosts(b1, dtype in (1, 3))
     g2 = ghosts(b2, dtype in (2, 3))

@@ -244,7 +244,8 @@ int cssx_patch_fill(void *out, void *tmp, int prec, int nr, int nth, int nphi, u
  * Features of point p (row p): column q = field*27 + (dr+1)*9 + (dphi+1)*3 + (dth+1), the
  * neighbour clamped at r and theta walls and wrapped in phi, standardized as (x - mean)/std.
  * Network inputs are BF16 row-major (rows x 216); outputs/targets FP32 row-major (rows x 8),
- * standardized; the closure tendency is output*y_std + y_mean. */
+ * standardized; the closure tendency is output*y_std + y_mean (y_std = y_mean = 0 switches a
+ * field's correction off, e.g. where a validation snapshot shows no gain). */
 typedef struct cssx_mlp cssx_mlp;
 int cssx_mlp_create(int nin, int h1, int h2, int nout, int max_rows, unsigned long long seed, cssx_mlp **out);
 void cssx_mlp_destroy(cssx_mlp *net);
