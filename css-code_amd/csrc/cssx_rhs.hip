@@ -679,12 +679,19 @@ __global__ __launch_bounds__(256) void k_tvd_stage(Consts<TS> c, BoundFlags f, c
 // Compute_Step_Size (rhs_mpi.f90:145-185, stratified branch): per-block minima of the sound/flow
 // crossing time and of the Alfven-included crossing time.
 // ---------------------------------------------------------------------------------------------
+// Compute_Step_Size (rhs_mpi.f90:92-275) per point: the sound/flow and Alfven crossing times
+// (block minima) and the conservation sums E, L, m, m r sin(theta) (block sums) of
+// rhs_mpi.f90:156-183: with dV = r^2 sin(th) dr dth dphi,
+//   m += rho dV,  mrsint += r sin^3(th) rho r^2 dr dth dphi,
+//   L += r rho sqrt(w^2 + sin^2(th) (v + r sin(th) Omega0)^2) dV,
+//   E += rho (Cv T + |u|^2/2) dV + |B|^2/(8 pi) dV.
+// red holds 6 values per block: min, min, E, L, m, mrsint.
 __global__ __launch_bounds__(256) void k_cfl(Consts<double> c, Prof<double> P, const double *__restrict__ dr,
                                              double dth, double dphi, double Cp, const double *__restrict__ st,
                                              const double *__restrict__ B, double *__restrict__ red) {
-    __shared__ double sc[256], sb[256];
+    __shared__ double sh[6][256];
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double vc = 1e4, vb = 1e4;
+    double vc = 1e4, vb = 1e4, e = 0.0, l = 0.0, ms = 0.0, mr = 0.0;
     if (i < c.N) {
         int it, ip, ir;
         coords(i, c, it, ip, ir);
@@ -693,27 +700,37 @@ __global__ __launch_bounds__(256) void k_cfl(Consts<double> c, Prof<double> P, c
         const double dxmin = fmin(dr[ir], radius * fmin(dth, sinth * dphi));
         const double rho = st[i], w = st[N + i], v = st[2 * N + i], u = st[3 * N + i], s = st[4 * N + i];
         const double t = temperature(rho, s, c);
+        const double dvol = radius * radius * sinth * dr[ir] * dth * dphi;
+        ms = rho * dvol;
+        mr = radius * sinth * sinth * ms;
+        const double vv = v + radius * sinth * c.omega0;
+        l = radius * rho * sqrt(w * w + sinth * sinth * vv * vv) * dvol;
+        e = rho * (c.Cv * t + 0.5 * (u * u + w * w + v * v)) * dvol;
         double ut = u * u + v * v + w * w + (c.gamm1 * Cp * c.xi2) * t;
         vc = dxmin / sqrt(ut);
         if (c.magnetic) {
             const double b2 = B[i] * B[i] + B[N + i] * B[N + i] + B[2 * N + i] * B[2 * N + i];
             vb = dxmin / sqrt(ut + b2 * kOneOver4Pi / rho);
+            e += 0.5 * b2 * dvol * kOneOver4Pi;
         }
     }
-    sc[threadIdx.x] = vc;
-    sb[threadIdx.x] = vb;
+    sh[0][threadIdx.x] = vc;
+    sh[1][threadIdx.x] = vb;
+    sh[2][threadIdx.x] = e;
+    sh[3][threadIdx.x] = l;
+    sh[4][threadIdx.x] = ms;
+    sh[5][threadIdx.x] = mr;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
-            sc[threadIdx.x] = fmin(sc[threadIdx.x], sc[threadIdx.x + o]);
-            sb[threadIdx.x] = fmin(sb[threadIdx.x], sb[threadIdx.x + o]);
+            sh[0][threadIdx.x] = fmin(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+            sh[1][threadIdx.x] = fmin(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+            for (int q = 2; q < 6; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        red[2 * blockIdx.x] = sc[0];
-        red[2 * blockIdx.x + 1] = sb[0];
-    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; q++) red[6 * blockIdx.x + q] = sh[q][0];
 }
 
 }  // namespace cssx
@@ -1366,19 +1383,17 @@ extern "C" int cssx_rk2_tvd(cssx_mhd *m, int prec, void *y0, double dt, int nste
     return CSSX_EINVAL;
 }
 
-extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream) {
-    if (!m || !y0_f64 || !dt_out || !(cfl_safety > 0.0)) return CSSX_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
+// the block partials of k_cfl for state y (6 per block, host vector)
+static int cfl_partials(cssx_mhd *m, const double *y, hipStream_t st, std::vector<double> &h) {
     int rc = ensure_scratch<double>(m);
     if (rc) return rc;
-    const double *y = (const double *)y0_f64;
     const long N = m->N;
     if (m->p.magnetic && (rc = curl_impl<double>(m, y + 5 * N, m->s64.B, st))) return rc;
     const unsigned nb = nblocks(N, 256);
     if (m->nred < (int)nb) {
         if (m->red) (void)hipFree(m->red);
         m->red = nullptr;
-        if (hipMalloc(&m->red, sizeof(double) * 2 * nb) != hipSuccess) return CSSX_ENOMEM;
+        if (hipMalloc(&m->red, sizeof(double) * 6 * nb) != hipSuccess) return CSSX_ENOMEM;
         m->nred = (int)nb;
     }
     double *dr = nullptr;
@@ -1389,17 +1404,36 @@ extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, doub
     }
     hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, make_consts<double>(m), m->p64, (const double *)dr, m->dth,
                        m->dphi, m->p.Cp, y, (const double *)m->s64.B, m->red);
-    std::vector<double> h(2 * (size_t)nb);
-    hipError_t e = hipMemcpyAsync(h.data(), m->red, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st);
+    h.assign(6 * (size_t)nb, 0.0);
+    hipError_t e = hipMemcpyAsync(h.data(), m->red, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(dr);
-    if (e != hipSuccess) return CSSX_EHIP;
+    return e == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream) {
+    if (!m || !y0_f64 || !dt_out || !(cfl_safety > 0.0)) return CSSX_EINVAL;
+    std::vector<double> h;
+    int rc = cfl_partials(m, (const double *)y0_f64, (hipStream_t)stream, h);
+    if (rc) return rc;
     double dtc = 1e4, dtb = 1e4;
-    for (unsigned b = 0; b < nb; b++) {
-        dtc = std::min(dtc, h[2 * b]);
-        dtb = std::min(dtb, h[2 * b + 1]);
+    for (size_t b = 0; b < h.size() / 6; b++) {
+        dtc = std::min(dtc, h[6 * b]);
+        dtb = std::min(dtb, h[6 * b + 1]);
     }
     const double dtd = 1e10;   // no explicit diffusion limit without Laplacian (rhs_mpi.f90:187-191)
     *dt_out = std::min(std::min(dtb, dtc), dtd) * cfl_safety;
     return std::isfinite(*dt_out) ? CSSX_OK : CSSX_ENAN;
+}
+
+extern "C" int cssx_conservation(cssx_mhd *m, const void *y0_f64, double *out4, void *stream) {
+    if (!m || !y0_f64 || !out4) return CSSX_EINVAL;
+    std::vector<double> h;
+    int rc = cfl_partials(m, (const double *)y0_f64, (hipStream_t)stream, h);
+    if (rc) return rc;
+    double acc[4] = {0, 0, 0, 0};
+    for (size_t b = 0; b < h.size() / 6; b++)
+        for (int q = 0; q < 4; q++) acc[q] += h[6 * b + 2 + q];
+    for (int q = 0; q < 4; q++) out4[q] = acc[q];
+    return (std::isfinite(acc[0]) && std::isfinite(acc[1])) ? CSSX_OK : CSSX_ENAN;
 }

@@ -66,6 +66,7 @@ SIGNATURES = {
     "cssx_bound": ([_vp, ctypes.c_int, _vp, _vp], ctypes.c_int),
     "cssx_rk4": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
     "cssx_cfl": ([_vp, _vp, ctypes.c_double, ctypes.POINTER(ctypes.c_double), _vp], ctypes.c_int),
+    "cssx_conservation": ([_vp, _vp, _dp, _vp], ctypes.c_int),
     "cssx_diffusion": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cssx_rk3_tvd": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),
     "cssx_rk2_tvd": ([_vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, _vp], ctypes.c_int),

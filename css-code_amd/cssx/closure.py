@@ -249,12 +249,23 @@ def coarse_config(fine: Config, factor: int) -> Config:
     return Config(**d)
 
 
+def filter_state(y: torch.Tensor, fine: Solver, coarse: Solver, factor: int) -> torch.Tensor:
+    """The coarse FP64 state of a fine FP64 state: the box-filtered fluctuations (rho and s about
+    the fine hydrostatic background, the other fields as they are) on top of the coarse grid's
+    own background.  Filtering the full fields instead would hand the coarse grid a density
+    profile out of its discrete hydrostatic balance (the filter averages the stratification over
+    a cell, and the coarse points are not the filter's cell centres), whose spurious buoyancy
+    would swamp every subgrid effect."""
+    yf = box_filter(y - fine.background(y.device), factor).double()
+    return (yf + coarse.background(y.device)).contiguous()
+
+
 def snapshot(fine: Solver, coarse: Solver, y: torch.Tensor, factor: int, t: float) -> Snapshot:
     """Filtered fine RHS minus coarse RHS of the filtered FP64 state y (one time)."""
     F = fine.rhs(y)
     Fb = box_filter(F, factor)
     del F
-    Uc = box_filter(y, factor).double().contiguous()
+    Uc = filter_state(y, fine, coarse, factor)
     Rc = coarse.rhs(Uc)
     return Snapshot(features(Uc, coarse.model), residual_rows(Fb, Rc), t)
 
@@ -335,11 +346,12 @@ def apriori_gain(net: Network, snap: Snapshot, ts: TrainingSet, mask=None):
     return before, after
 
 
-def field_gate(net: Network, validation: Snapshot, ts: TrainingSet):
-    """Fields whose residual the network reduces on a validation snapshot; the others fall back to
-    the plain coarse RHS (their correction is switched off in the network's norm)."""
+def field_gate(net: Network, validation: Snapshot, ts: TrainingSet, threshold: float = 1.0):
+    """Fields whose residual RMS the network brings below `threshold` times its plain value on a
+    validation snapshot; the others fall back to the plain coarse RHS (their correction is
+    switched off in the network's norm)."""
     before, after = apriori_gain(net, validation, ts)
-    mask = after < before
+    mask = after < threshold * before
     net.set_norm(ts.x_mean, ts.x_std, np.where(mask, ts.y_mean, 0.0), np.where(mask, ts.y_std, 0.0))
     return mask
 
@@ -370,7 +382,7 @@ def compare(fine_cfg: Config, factor: int, net: Network | None, coarse_steps: in
         _lib.check(_lib.lib().cssx_mhd_set_closure(closed._h, net.handle), "cssx_mhd_set_closure")
     y = y_fine if y_fine is not None else torch.as_tensor(fine.model.initial_state(), device="cuda").contiguous()
     dtf = fine.cfl(y)
-    yc0 = box_filter(y, factor).double().contiguous()
+    yc0 = filter_state(y, fine, plain, factor)
     dtc = min(factor * dtf, plain.cfl(yc0))
     dtf = dtc / factor
     yp = yc0.clone()
@@ -379,7 +391,7 @@ def compare(fine_cfg: Config, factor: int, net: Network | None, coarse_steps: in
     t_fine = t_plain = t_closed = 0.0
 
     def rec(k):
-        yf = box_filter(y, factor).double()
+        yf = filter_state(y, fine, plain, factor)
         r = {"step": k, "time": k * dtc, "filtered_fine": stats_of(yf, cmodel), "coarse": stats_of(yp, cmodel)}
         if yq is not None:
             r["coarse_closure"] = stats_of(yq, cmodel)
@@ -391,7 +403,7 @@ def compare(fine_cfg: Config, factor: int, net: Network | None, coarse_steps: in
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         fine.rk4(y, dtf, n * factor, precision)
-        box_filter(y, factor)
+        filter_state(y, fine, plain, factor)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         plain.rk4(yp, dtc, n, precision)

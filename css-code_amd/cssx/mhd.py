@@ -130,3 +130,11 @@ class Solver:
         cs = self.model.c.cfl_safety if cfl_safety is None else cfl_safety
         _lib.check(_lib.lib().cssx_cfl(self._h, y0.data_ptr(), float(cs), ctypes.byref(dt), _stream()), "cssx_cfl")
         return dt.value
+
+    def conservation(self, y0):
+        """{E, L, m, mrsint}: the totals Compute_Step_Size reports (rhs_mpi.f90:156-216) for an
+        FP64 state (synchronizes)."""
+        self._check_state(y0, torch.float64)
+        out = (ctypes.c_double * 4)()
+        _lib.check(_lib.lib().cssx_conservation(self._h, y0.data_ptr(), out, _stream()), "cssx_conservation")
+        return {"E": out[0], "L": out[1], "m": out[2], "mrsint": out[3]}

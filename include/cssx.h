@@ -176,6 +176,11 @@ int cssx_diffusion(cssx_mhd *m, int prec, const void *state, const void *b, void
  * flow and Alfven crossing times.  Synchronizes the stream; *dt_out on the host. */
 int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream);
 
+/* The conservation totals Compute_Step_Size prints (rhs_mpi.f90:156-216, the scalar.dat
+ * quantities): out4 = {E_tot, L_tot, m_tot, mrsint} of an FP64 state, E including the magnetic
+ * energy |B|^2/(8 pi).  Synchronizes the stream. */
+int cssx_conservation(cssx_mhd *m, const void *y0_f64, double *out4, void *stream);
+
 /* Per-phase GPU timing, the counterpart of the reference's Do_Timings buckets (constants.f90:82,
  * timings(1:16) around each phase of rhs_mpi.f90:324-766 and step_rk4.f90:343-491).  While
  * enabled, every kernel launch of the library is bracketed by hipEvents on its stream.

@@ -188,27 +188,29 @@ def test_attached_closure_adds_the_network_tendency(cl, precision):
 
 
 def test_closure_gains_over_the_plain_coarse_rhs(cl):
-    """Trained on filtered fine RHS residuals of a 64x128x256 run (coarse 16x32x64), the network,
-    gated per field on one held-out snapshot, removes a clear part of another held-out
-    snapshot's residual and makes no field worse (a-priori gain)."""
+    """Trained on filtered fine RHS residuals of a 32x64x128 run (coarse 8x16x32, dt 1e-4,
+    snapshots every 5 steps), the network, gated per field on one held-out snapshot, removes
+    most of another held-out snapshot's residual and makes no field worse (a-priori gain).
+    (In this early-time regime the residual is the commutation error of filtering and the
+    compact derivatives on the seeded perturbations, a function of the resolved stencil.)"""
     from cssx.mhd import Solver
     from cssx.model import Config, Model
     print("closure a-priori gain", flush=True)
-    fc = Config(nr=64, nth=128, nphi=256, sld_coef=0.1)
+    fc = Config(nr=32, nth=64, nphi=128, sld_coef=0.1)
     fine = Solver(Model(fc))
     coarse = Solver(Model(cl.coarse_config(fc, 4)))
     y = torch.as_tensor(fine.model.initial_state(), device="cuda").contiguous()
-    dt = fine.cfl(y)
+    dt = 1e-4
     snaps = []
     for k in range(8):
         if k:
-            fine.rk4(y, dt, 10, "f32")
-        snaps.append(cl.snapshot(fine, coarse, y, 4, k * 10 * dt))
+            fine.rk4(y, dt, 5, "f64")
+        snaps.append(cl.snapshot(fine, coarse, y, 4, k * 5 * dt))
     train_ids, val_id, test_id = [0, 1, 2, 4, 5, 7], 3, 6
-    ts = cl.build_training_set([snaps[i] for i in train_ids], len(train_ids) * 32768)
-    net = cl.Network(128, 128, max_rows=32768, seed=7)
-    hist = cl.train(net, ts, epochs=15, batch=4096, lr=1e-3)
-    assert hist[-1] < hist[0]
+    ts = cl.build_training_set([snaps[i] for i in train_ids], len(train_ids) * 4096)
+    net = cl.Network(128, 128, max_rows=4096, seed=7)
+    hist = cl.train(net, ts, epochs=40, batch=1024, lr=1e-3)
+    assert hist[-1] < 0.5 * hist[0], hist
     mask = cl.field_gate(net, snaps[val_id], ts)
     before, after = cl.apriori_gain(net, snaps[test_id], ts, mask)
     print("gate", mask, "residual rms before", before, "after", after, flush=True)
@@ -217,7 +219,7 @@ def test_closure_gains_over_the_plain_coarse_rhs(cl):
     assert np.all(ratio < 1.02), ratio
     ys = ts.y_std
     total = np.sum((after / ys) ** 2) / np.sum((before / ys) ** 2)
-    assert total < 0.8, (total, ratio)
+    assert total < 0.5, (total, ratio)
     net.close()
     fine.close()
     coarse.close()
