@@ -325,6 +325,146 @@ __global__ __launch_bounds__(256) void k_standardize(const float *__restrict__ x
     if (of) of[e] = v;
 }
 
+// ---- fused inference: features -> 3 layers -> tendency, activations never leave registers ------
+// Computed transposed (Z^T = W^T X^T): the samples are the MFMA's B columns, so a layer's
+// accumulators (lane l: units 16*nb + 4*(l/16) + r of sample l mod 16) are, after bias+GELU
+// and a BF16 cast, already the next layer's B operand, provided the next weights' K slots are
+// permuted to match (slot 8g+e of K step ks <-> unit 16*(2ks + e/4) + 4g + e mod 4).  The
+// weights are packed once per call into that layout and held in LDS by every workgroup.
+constexpr int kH = 128;                 // hidden width of the fused kernel
+constexpr int kKp1 = 224;               // 216 features padded to 7 K steps of 32
+constexpr int kS1 = kKp1 + 8, kS2 = kH + 8;   // LDS row strides (BF16), padded against bank conflicts
+constexpr int kPackBf = kH * kS1 + kH * kS2 + 16 * kS2;
+constexpr int kPackF = 2 * kH + 16;     // biases b1, b2, b3 (padded to 16), FP32
+constexpr size_t kPackBytes = (size_t)kPackBf * 2 + (size_t)kPackF * 4;
+
+__device__ __forceinline__ int perm_unit(int ks, int slot) {   // slot = 8g + e
+    const int g = slot >> 3, e = slot & 7;
+    return 16 * (2 * ks + (e >> 2)) + 4 * g + (e & 3);
+}
+
+__global__ __launch_bounds__(256) void k_pack_infer(const float *__restrict__ W, long oW1, long ob1, long oW2, long ob2,
+                                                    long oW3, long ob3, cssx_bf16 *__restrict__ pk) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const cssx_bf16 zero = (cssx_bf16)0.0f;
+    if (i < (long)kH * kS1) {
+        // layer-1 K slots are neighbour-major: slot 8*nb + field <-> feature field*27 + nb, so a
+        // lane's 8 slots of a K step are one neighbour of all 8 fields (one address computation)
+        const int n = (int)(i / kS1), k = (int)(i % kS1);
+        const int nbh = k >> 3, iv = k & 7;
+        pk[i] = (k < kKp1 && nbh < 27) ? (cssx_bf16)W[oW1 + (long)(iv * 27 + nbh) * kH + n] : zero;
+    } else if (i < (long)kH * kS1 + (long)kH * kS2) {
+        const long j = i - (long)kH * kS1;
+        const int o = (int)(j / kS2), k = (int)(j % kS2);
+        pk[i] = k < kH ? (cssx_bf16)W[oW2 + (long)perm_unit(k >> 5, k & 31) * kH + o] : zero;
+    } else if (i < kPackBf) {
+        const long j = i - (long)kH * kS1 - (long)kH * kS2;
+        const int o = (int)(j / kS2), k = (int)(j % kS2);
+        pk[i] = (k < kH && o < kOut) ? (cssx_bf16)W[oW3 + (long)perm_unit(k >> 5, k & 31) * kOut + o] : zero;
+    } else if (i < kPackBf + kPackF) {
+        const int j = (int)(i - kPackBf);
+        float *bf = reinterpret_cast<float *>(pk + kPackBf);
+        bf[j] = j < kH ? W[ob1 + j] : j < 2 * kH ? W[ob2 + j - kH] : (j - 2 * kH < kOut ? W[ob3 + j - 2 * kH] : 0.0f);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void k_closure_infer(const T *__restrict__ st, long N, int nr, int nphi, int nth,
+                                                        int fluct, const T *__restrict__ rbar,
+                                                        const T *__restrict__ sbar, const float *__restrict__ xmean,
+                                                        const float *__restrict__ xistd, const float *__restrict__ ymean,
+                                                        const float *__restrict__ ystd,
+                                                        const cssx_bf16 *__restrict__ pk, float *__restrict__ ext) {
+    extern __shared__ uint4 lds4[];
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(pk);
+        for (int i = threadIdx.x; i < (int)(kPackBytes / 16); i += blockDim.x) lds4[i] = src[i];
+    }
+    __syncthreads();
+    const cssx_bf16 *W1T = reinterpret_cast<const cssx_bf16 *>(lds4);
+    const cssx_bf16 *W2P = W1T + kH * kS1, *W3P = W2P + kH * kS2;
+    const float *b1 = reinterpret_cast<const float *>(W1T + kPackBf), *b2 = b1 + kH, *b3 = b2 + kH;
+    const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15;
+    const long ntiles = (N + 15) / 16;
+    const int wpb = blockDim.x >> 6;
+    const cssx_bf16 zero = (cssx_bf16)0.0f;
+    for (long t = (long)blockIdx.x * wpb + (threadIdx.x >> 6); t < ntiles; t += (long)gridDim.x * wpb) {
+        const long pt = t * 16 + li;
+        const bool ok = pt < N;
+        const long pc = ok ? pt : N - 1;
+        const int it = (int)(pc % nth);
+        const long rest = pc / nth;
+        const int ip = (int)(rest % nphi), ir = (int)(rest / nphi);
+        cssx_f32x4 acc[8];
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++) acc[nb] = cssx_f32x4{0.f, 0.f, 0.f, 0.f};
+        // layer 1: the lane's 8 features of each K step, gathered and standardized in place
+#pragma unroll 1
+        for (int ks = 0; ks < kKp1 / 32; ks++) {
+            cssx_bf16x8 b;
+            const int nbh = 4 * ks + g;   // this lane's neighbour; its 8 slots are the 8 fields
+            if (nbh < 27) {
+                const int jr = min(max(ir + nbh / 9 - 1, 0), nr - 1);
+                const int jt = min(max(it + nbh % 3 - 1, 0), nth - 1);
+                int jp = ip + (nbh / 3) % 3 - 1;
+                jp = jp < 0 ? jp + nphi : (jp >= nphi ? jp - nphi : jp);
+                const T *p0 = st + ((long)jr * nphi + jp) * nth + jt;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const T raw = p0[(long)e * N];
+                    float x;
+                    if (!fluct && e == 0) x = (float)((double)raw - (double)rbar[jr]);
+                    else if (!fluct && e == 4) x = (float)((double)raw - (double)sbar[jr]);
+                    else x = (float)raw;
+                    const int q = e * 27 + nbh;
+                    b[e] = (cssx_bf16)((x - xmean[q]) * xistd[q]);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) b[e] = zero;
+            }
+#pragma unroll
+            for (int nb = 0; nb < 8; nb++) {
+                const cssx_bf16x8 a = *reinterpret_cast<const cssx_bf16x8 *>(W1T + (nb * 16 + li) * kS1 + ks * 32 + 8 * g);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+            }
+        }
+        cssx_bf16x8 h[4];
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) h[nb >> 1][(nb & 1) * 4 + r] = (cssx_bf16)gelu(acc[nb][r] + b1[nb * 16 + 4 * g + r]);
+        // layer 2
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++) acc[nb] = cssx_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nb = 0; nb < 8; nb++) {
+                const cssx_bf16x8 a = *reinterpret_cast<const cssx_bf16x8 *>(W2P + (nb * 16 + li) * kS2 + ks * 32 + 8 * g);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], acc[nb], 0, 0, 0);
+            }
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) h[nb >> 1][(nb & 1) * 4 + r] = (cssx_bf16)gelu(acc[nb][r] + b2[nb * 16 + 4 * g + r]);
+        // layer 3 (8 outputs padded to one 16-row block)
+        cssx_f32x4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const cssx_bf16x8 a = *reinterpret_cast<const cssx_bf16x8 *>(W3P + li * kS2 + ks * 32 + 8 * g);
+            y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], y, 0, 0, 0);
+        }
+        if (ok && g < 2) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int o = 4 * g + r;
+                ext[(long)o * N + pt] = (y[r] + b3[o]) * ystd[o] + ymean[o];
+            }
+        }
+    }
+}
+
 inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace cssx
@@ -343,6 +483,7 @@ struct cssx_mlp {
     float *Z1 = nullptr, *Z2 = nullptr, *Y = nullptr, *dY = nullptr, *dZ2 = nullptr, *dZ1 = nullptr, *loss = nullptr;
     cssx_bf16 *H1 = nullptr, *H2 = nullptr, *dYb = nullptr, *dZ2b = nullptr, *dZ1b = nullptr, *X = nullptr;
     float *xmean = nullptr, *xistd = nullptr, *ymean = nullptr, *ystd = nullptr;
+    cssx_bf16 *pk = nullptr;        // weights packed for the fused inference kernel (h1 = h2 = 128)
     long step = 0;
     void *blk = nullptr;
 };
@@ -438,6 +579,28 @@ namespace cssx {
 int closure_tendency(cssx_mlp *n, int prec, const void *st, int fluct, const void *rbar, const void *sbar, int nr,
                      int nphi, int nth, float *ext, hipStream_t s) {
     const long N = (long)nr * nphi * nth;
+    if (n->h1 == kH && n->h2 == kH) {
+        const long tot = kPackBf + kPackF;
+        hipLaunchKernelGGL(k_pack_infer, dim3(nblk(tot)), dim3(256), 0, s, n->W, n->oW1, n->ob1, n->oW2, n->ob2,
+                           n->oW3, n->ob3, n->pk);
+        int dev = 0, ncu = 256;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const void *fn = prec == CSSX_PREC_F64 ? (const void *)k_closure_infer<double> : (const void *)k_closure_infer<float>;
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPackBytes) != hipSuccess)
+            return CSSX_EHIP;
+        const long tiles = (N + 15) / 16;
+        const unsigned grid = (unsigned)std::min<long>((long)ncu, (tiles + 7) / 8);
+        if (prec == CSSX_PREC_F64)
+            hipLaunchKernelGGL(k_closure_infer<double>, dim3(grid), dim3(512), kPackBytes, s, (const double *)st, N, nr,
+                               nphi, nth, fluct, (const double *)rbar, (const double *)sbar, n->xmean, n->xistd,
+                               n->ymean, n->ystd, n->pk, ext);
+        else
+            hipLaunchKernelGGL(k_closure_infer<float>, dim3(grid), dim3(512), kPackBytes, s, (const float *)st, N, nr,
+                               nphi, nth, fluct, (const float *)rbar, (const float *)sbar, n->xmean, n->xistd,
+                               n->ymean, n->ystd, n->pk, ext);
+        return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+    }
     for (long r0 = 0; r0 < N; r0 += n->max_rows) {
         const long rows = min((long)n->max_rows, N - r0);
         int rc = prec == CSSX_PREC_F64
@@ -472,7 +635,7 @@ extern "C" int cssx_mlp_create(int nin, int h1, int h2, int nout, int max_rows, 
     const long fl = 4 * n->np + R * (2L * h1 + 2L * h2 + 2L * nout) + 1 + 2L * nin + 2L * nout;
     const long bf = n->np + R * (h1 + h2 + nout + h2 + h1 + nin);
     char *p = nullptr;
-    const size_t bytes = (size_t)fl * 4 + (size_t)bf * 2 + 64 * 16;
+    const size_t bytes = (size_t)fl * 4 + (size_t)bf * 2 + kPackBytes + 64 * 16;
     if (hipMalloc((void **)&p, bytes) != hipSuccess) {
         delete n;
         return CSSX_ENOMEM;
@@ -486,6 +649,7 @@ extern "C" int cssx_mlp_create(int nin, int h1, int h2, int nout, int max_rows, 
     n->xmean = takef(nin); n->xistd = takef(nin); n->ymean = takef(nout); n->ystd = takef(nout);
     n->Wb = takeb(n->np); n->H1 = takeb(R * h1); n->H2 = takeb(R * h2); n->dYb = takeb(R * nout);
     n->dZ2b = takeb(R * h2); n->dZ1b = takeb(R * h1); n->X = takeb(R * nin);
+    n->pk = takeb((long)(kPackBytes / 2));
     // PyTorch-Linear-style initialization: U(-1/sqrt(fan_in), 1/sqrt(fan_in)) for W and b
     std::vector<float> w(n->np), zeros(n->np, 0.0f), ones(nin, 1.0f);
     uint64_t s = seed;
@@ -721,4 +885,12 @@ extern "C" int cssx_standardize_rows(const float *x, long rows, int cols, const 
     hipLaunchKernelGGL(k_standardize, dim3(nblk(rows * cols)), dim3(256), 0, (hipStream_t)stream, x, rows, cols, mean,
                        stddev, (cssx_bf16 *)out_bf16, out_f32);
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
+extern "C" int cssx_closure_tendency(cssx_mlp *n, const void *state, int prec, int fluct, const void *rho_bar,
+                                     const void *s_bar, int nr, int nphi, int nth, float *ext, void *stream) {
+    if (!n || !state || !ext || nr < 1 || nphi < 1 || nth < 1 || (!fluct && (!rho_bar || !s_bar)) ||
+        (prec != CSSX_PREC_F64 && prec != CSSX_PREC_F32))
+        return CSSX_EINVAL;
+    return closure_tendency(n, prec, state, fluct, rho_bar, s_bar, nr, nphi, nth, ext, (hipStream_t)stream);
 }

@@ -101,6 +101,8 @@ SIGNATURES = {
                                ctypes.c_int, ctypes.c_long, ctypes.c_long, _vp, _vp, _vp], ctypes.c_int),
     "cssx_closure_targets": ([_vp, _vp, ctypes.c_int, ctypes.c_long, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cssx_mhd_set_closure": ([_vp, _vp], ctypes.c_int),
+    "cssx_closure_tendency": ([_vp, _vp, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                               ctypes.c_int, _vp, _vp], ctypes.c_int),
     "cssx_colstats": ([_vp, ctypes.c_long, ctypes.c_int, _dp, _dp, _vp], ctypes.c_int),
     "cssx_field_stats": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                           _vp, _vp, _dp, _dp, _vp], ctypes.c_int),

@@ -175,6 +175,20 @@ def features(state: torch.Tensor, model: Model, fluct: bool = False, net: Networ
     return out
 
 
+def tendency(net: Network, state: torch.Tensor, model: Model, fluct: bool = False,
+             out: torch.Tensor | None = None) -> torch.Tensor:
+    """The network's closure tendency (8, nr, nphi, nth) FP32 of an 8-field state."""
+    _cuda(state)
+    _, nr, nphi, nth = state.shape
+    rb, sb = _background(model, state.dtype, state.device)
+    if out is None:
+        out = torch.empty(state.shape, dtype=torch.float32, device=state.device)
+    _lib.check(_lib.lib().cssx_closure_tendency(net.handle, state.data_ptr(), _prec(state), int(fluct), rb.data_ptr(),
+                                                sb.data_ptr(), nr, nphi, nth, out.data_ptr(), _stream()),
+               "cssx_closure_tendency")
+    return out
+
+
 def residual_rows(fine_filtered: torch.Tensor, coarse_rhs: torch.Tensor) -> torch.Tensor:
     """Row-major (N, 8) FP32 filtered-fine minus coarse tendency."""
     _cuda(fine_filtered, torch.float32)

@@ -295,6 +295,11 @@ int cssx_standardize_rows(const float *x, long rows, int cols, const float *mean
                           void *out_bf16, float *out_f32, void *stream);
 /* dst row i = src row idx[i] (idx: device int64), rows of row_bytes bytes (a multiple of 4) */
 int cssx_gather_rows(const void *src, long row_bytes, const long *idx, long n, void *dst, void *stream);
+/* the closure tendency (8 fields, FP32, field-major like the state) of a whole 8-field state;
+ * h1 = h2 = 128 networks run one fused kernel (features, three layers and the scaling with the
+ * activations held in registers and the weights in LDS), other widths layer by layer */
+int cssx_closure_tendency(cssx_mlp *net, const void *state, int prec, int fluct, const void *rho_bar,
+                          const void *s_bar, int nr, int nphi, int nth, float *ext, void *stream);
 /* attach (net != NULL) or detach a closure: every later RHS evaluation of m (cssx_rhs and the
  * steppers' stages) adds the network's tendency of its input state.  net is not owned. */
 int cssx_mhd_set_closure(cssx_mhd *m, cssx_mlp *net);
