@@ -220,3 +220,32 @@ def test_conservation_totals():
     vol = (c.r2 ** 3 - c.r1 ** 3) / 3 * (np.cos(c.th1) - np.cos(c.th2)) * (c.phi2 - c.phi1)
     assert abs(sol.conservation(y1)["m"] - vol) < 0.06 * vol   # end-point quadrature: ~3.5%
     sol.close()
+
+
+def test_scalar_quantities():
+    """Scalar_Output's sums (report_mpi.f90:195-244) against numpy on the same state and B."""
+    from cssx import diagnostics as dg
+    from cssx.mhd import Solver
+    from cssx.model import Config, Model
+    print("scalar quantities", flush=True)
+    m = Model(Config(nr=16, nth=24, nphi=16))
+    sol = Solver(m)
+    st = m.initial_state()
+    rng = np.random.default_rng(5)
+    st[1:4] = 1e-2 * rng.standard_normal(st[1:4].shape)
+    y = torch.as_tensor(st, device="cuda")
+    q = dg.scalar_quantities(sol, y)
+    _, B = sol.rhs(y, want_b=True)
+    B = B.cpu().numpy()
+    rho, w, v, u, s = st[:5]
+    N = rho.size
+    usq = w * w + v * v + u * u
+    bsq = (B ** 2).sum(0)
+    tsq = rho ** m.gamm1 * np.exp(s / m.Cv) * m.invtconst * m.gamm1 * m.c.Cp
+    ref = {"kinener": (rho * usq).sum() / (2 * N), "cke": (rho * u * u).sum() / (2 * N),
+           "mcke": ((rho * u * u).sum() + (rho * w * w).sum()) / (2 * N), "drke": (rho * v * v).sum() / (2 * N),
+           "magener": bsq.sum() / (8 * np.pi * N), "bmax": np.sqrt(bsq.max()) * 2 * np.sqrt(2 * np.pi),
+           "umach": np.sqrt((usq / (2 * bsq / rho + usq + tsq)).max())}
+    for k, r in ref.items():
+        assert abs(q[k] - r) <= 1e-12 * abs(r), (k, q[k], r)
+    sol.close()
