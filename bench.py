@@ -140,7 +140,7 @@ def run_c5(args):
     for k in range(args.c5_snapshots):
         if k:
             fine.rk4(y, dt, args.c5_every, "f32")
-        snaps.append(CL.snapshot(fine, coarse, y, factor, k * args.c5_every * dt))
+        snaps.append(CL.snapshot(fine, coarse, y, factor, k * args.c5_every * dt, include_sld=args.c5_sld))
         print(f"c5: snapshot {k}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
@@ -195,7 +195,8 @@ def run_c5(args):
         "config": {"workload": f"closure_{fc.nr}x{fc.nth}x{fc.nphi}_to_{fc.nr // 4}x{fc.nth // 4}x{fc.nphi // 4}",
                    "factor": factor, "snapshots": args.c5_snapshots, "snapshot_every_fine_steps": args.c5_every,
                    "samples": n, "epochs": len(hist), "batch": 8192, "lr": 1e-3, "network": "216-128-128-8 GELU",
-                   "coarse_steps": args.c5_steps, "dt_coarse": times["dt_coarse"], "dt_fine": times["dt_fine"]},
+                   "coarse_steps": args.c5_steps, "dt_coarse": times["dt_coarse"], "dt_fine": times["dt_fine"],
+                   "targets_include_sld": bool(args.c5_sld)},
         "wall_s": {"fine_run_and_filter": round(times["fine_s"], 3), "coarse_plain": round(times["coarse_plain_s"], 3),
                    "coarse_closure": round(times["coarse_closure_s"], 3), "training_data": round(t_data, 3),
                    "training": round(t_train, 3)},
@@ -328,6 +329,8 @@ def main():
     ap.add_argument("--c5-epochs", type=int, default=10)
     ap.add_argument("--c5-steps", type=int, default=100, help="closure: coarse steps of the comparison runs")
     ap.add_argument("--c5-stat-every", type=int, default=10)
+    ap.add_argument("--c5-sld", type=int, default=1,
+                    help="closure targets include each grid's SLD diffusion (1) or the RHS alone (0)")
     ap.add_argument("--c3-patches", type=int, default=4096)
     ap.add_argument("--c3-chunk", type=int, default=64)
     ap.add_argument("--c3-members", type=int, default=64)

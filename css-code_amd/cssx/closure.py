@@ -274,13 +274,21 @@ def filter_state(y: torch.Tensor, fine: Solver, coarse: Solver, factor: int) -> 
     return (yf + coarse.background(y.device)).contiguous()
 
 
-def snapshot(fine: Solver, coarse: Solver, y: torch.Tensor, factor: int, t: float) -> Snapshot:
-    """Filtered fine RHS minus coarse RHS of the filtered FP64 state y (one time)."""
+def snapshot(fine: Solver, coarse: Solver, y: torch.Tensor, factor: int, t: float,
+             include_sld: bool = False) -> Snapshot:
+    """Filtered fine tendency minus coarse tendency of the filtered FP64 state y (one time): the
+    RHS, plus with include_sld the SLD diffusion each grid applies (the coarse grid's SLD acts
+    on scales the fine grid resolves, so its surplus damping is part of what the closure can
+    learn to undo)."""
     F = fine.rhs(y)
+    if include_sld:
+        F += fine.diffusion(y)
     Fb = box_filter(F, factor)
     del F
     Uc = filter_state(y, fine, coarse, factor)
     Rc = coarse.rhs(Uc)
+    if include_sld:
+        Rc += coarse.diffusion(Uc)
     return Snapshot(features(Uc, coarse.model), residual_rows(Fb, Rc), t)
 
 
