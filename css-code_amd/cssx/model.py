@@ -108,6 +108,15 @@ class Config:
     Top_Perfect_Cond: bool = False
     Bottom_Perfect_Cond: bool = False
     Theta_Stress_Free: bool = True
+    # explicit diffusion (constants.f90:21,66; mhdsph_mpi.f90:349-354): Laplacian = viscosity
+    # mur = sld_pr kappa rho_bar, thermal diffusion ks = kappa rho_bar T_bar (rho_bar/rho_bar(r2))^g0,
+    # resistivity etar = ks; SLaplacian = the thermal diffusion alone.  Either replaces SLD.
+    Laplacian: bool = False
+    SLaplacian: bool = False
+    kappa_value: float = 1e-5
+    g0: float = 0.0
+    lumr1: float = 0.0              # constant-flux wall luminosities under Laplacian/SLaplacian
+    lumr2: float = 0.0
 
     @property
     def nv(self):
@@ -179,15 +188,19 @@ class Model:
         a, b = 0.5 * (c.r1 + c.r2), 0.5 * (c.r2 - c.r1)
         if k > 0:
             tk = np.tanh(k)
-            self.r = a + b * np.tanh(k * (2 * x - 1)) / tk
-            rp = b * 2 * k * (1 - np.tanh(k * (2 * x - 1)) ** 2) / tk       # dr/dx
+            tz = np.tanh(k * (2 * x - 1))
+            self.r = a + b * tz / tk
+            rp = b * 2 * k * (1 - tz ** 2) / tk                              # dr/dx
+            rpp = -8 * k * k * b * (1 - tz ** 2) * tz / tk                    # d2r/dx2
             xm = -1.0 / (nr - 1)
             r_below = a + b * np.tanh(k * (2 * xm - 1)) / tk
         else:
             self.r = a + b * (2 * x - 1)
             rp = np.full(nr, 2 * b)
+            rpp = np.zeros(nr)
             r_below = self.r[0] - (self.r[1] - self.r[0])
         self.dxdr = 1.0 / rp
+        self.d2xdr2 = -rpp / rp ** 3
         self.dr = np.empty(nr)                     # dr_ind as mhdsph_mpi.f90:331-334
         self.dr[0] = self.r[0] - r_below
         self.dr[1:] = np.diff(self.r)
@@ -218,6 +231,14 @@ class Model:
         self.dsdr1 = np.zeros(nth)     # dsdrm = 0 (isentropic), mhdsph_mpi.f90:351,573
         self.dsdr2 = np.zeros(nth)
         self.ibcr, self.ibct = boundary_codes(c)
+        # explicit-diffusion profiles (mhdsph_mpi.f90:349-354): exact log-derivatives of the
+        # polytrope's rho_bar and T_bar
+        self.ks = c.kappa_value * self.rho_bar * self.T_bar * (self.rho_bar / self.rho_bar[-1]) ** c.g0
+        self.etar = self.ks.copy()
+        self.mur = c.sld_pr * c.kappa_value * self.rho_bar
+        dzeta = -self.alp * c.r2 / self.r ** 2 / zeta                          # d ln zeta / dr
+        self.dlnks = (g + c.g0) / self.gamm1 * dzeta
+        self.dlnmu = dzeta / self.gamm1
         # SLD diffusion speed^2 (diffusion.f90:48-58), flux cutoff off (tanh -> 1)
         ds = c.sld_speedh * (self.rho_bar / self.rho_bar[-1]) ** c.sld_exp + c.sld_speedr
         sp = np.empty(nr + 2)
@@ -264,20 +285,22 @@ class Model:
         for i in range(13):
             p.ibcr[i] = self.ibcr[i]
             p.ibct[i] = self.ibct[i]
-        p.sld_on = 1
+        p.sld_on = 0 if (c.Laplacian or c.SLaplacian) else 1
         p.sld_coef = c.sld_coef
         p.mass_diffusion, p.magnetic_diffusion = int(c.Mass_Diffusion), int(c.Magnetic_Diffusion)
         p.viscous_heating, p.ohmic_heating = int(c.Do_Viscous_Heating), int(c.Do_Ohmic_Heating)
         p.dstype = c.dstype
         p.sld_pr, p.sld_prm = c.sld_pr, c.sld_prm
         p.bottom_perf_cond, p.top_perf_cond = int(c.Bottom_Perfect_Cond), int(c.Top_Perfect_Cond)
+        p.laplacian, p.slaplacian = int(c.Laplacian), int(c.SLaplacian)
+        p.lumr1, p.lumr2 = c.lumr1, c.lumr2
         return p
 
     def profiles_struct(self):
         """cssx_mhd_profiles pointing at this model's arrays (keep the Model alive)."""
         self._keep = {k: np.ascontiguousarray(getattr(self, k), dtype=np.float64) for k in
                       ("r", "dr", "dxdr", "theta", "gravity", "eps", "rho_bar", "s_bar", "dsdr1", "dsdr2",
-                       "diffusion_speed")}
+                       "diffusion_speed", "mur", "dlnmu", "ks", "dlnks", "etar", "d2xdr2")}
         sr1 = np.full(self.c.nth, self.s_bar[0])
         sr2 = np.full(self.c.nth, self.s_bar[-1])
         self._keep["sr1"], self._keep["sr2"] = sr1, sr2
@@ -306,10 +329,13 @@ class MhdParams(ctypes.Structure):
                 ("mass_diffusion", ctypes.c_int), ("magnetic_diffusion", ctypes.c_int),
                 ("viscous_heating", ctypes.c_int), ("ohmic_heating", ctypes.c_int),
                 ("dstype", ctypes.c_int), ("sld_pr", ctypes.c_double), ("sld_prm", ctypes.c_double),
-                ("bottom_perf_cond", ctypes.c_int), ("top_perf_cond", ctypes.c_int)]
+                ("bottom_perf_cond", ctypes.c_int), ("top_perf_cond", ctypes.c_int),
+                ("laplacian", ctypes.c_int), ("slaplacian", ctypes.c_int),
+                ("lumr1", ctypes.c_double), ("lumr2", ctypes.c_double)]
 
 
 class MhdProfiles(ctypes.Structure):
     """cssx_mhd_profiles (include/cssx.h)."""
     _fields_ = [(k, _dp) for k in ("r", "dr", "dxdr", "theta", "gravity", "eps", "rho_bar", "s_bar",
-                                   "dsdr1", "dsdr2", "sr1", "sr2", "diffusion_speed")]
+                                   "dsdr1", "dsdr2", "sr1", "sr2", "diffusion_speed", "mur", "dlnmu", "ks",
+                                   "dlnks", "etar", "d2xdr2")]

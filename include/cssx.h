@@ -88,8 +88,9 @@ int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const
 
 /* Physics constants, switches and boundary codes.  Field names follow the reference's
  * namelist / module variables (constants.f90:17-72, rhs_mpi.f90:10-13, diffusion.f90:8-11).
- * Only the stratified (not Unstratified), Laplacian=.False., do_radheat=.False. physics is
- * supported; the flags below are the ones that path reads. */
+ * The stratified (not Unstratified), do_radheat=.False. physics is supported, with either SLD
+ * diffusion or the explicit Laplacian / SLaplacian diffusion; the flags below are the ones that
+ * path reads. */
 typedef struct cssx_mhd_params {
     double gamma, Cp;            /* Cv=Cp/gamma, pconst=(gamma-1)*Cp (rhs_mpi.f90:59-73) */
     double omega0;               /* frame rotation rate omega_0 */
@@ -111,6 +112,13 @@ typedef struct cssx_mhd_params {
     double sld_pr, sld_prm;      /* flux multipliers (1 = off): diffusion.f90:846-854 */
     int bottom_perf_cond, top_perf_cond;   /* A_theta, A_phi ghosts 0 at that r wall in the SLD
                                               fluxes (diffusion.f90:628-643, 691-706) */
+    /* explicit diffusion (rhs_mpi.f90:844-1474, :2070-2366).  laplacian: viscosity mur, thermal
+     * diffusion ks and resistivity etar; slaplacian: the thermal diffusion alone.  Either one
+     * needs the profiles mur..d2xdr2 and excludes sld_on. */
+    int laplacian, slaplacian;
+    double lumr1, lumr2;         /* luminosities of constant-flux entropy walls under laplacian /
+                                    slaplacian: ds/dr = -lumr/(4 pi r^2 ks) (rhs_mpi.f90:586-588,
+                                    :618-620) */
 } cssx_mhd_params;
 
 /* Radial / latitudinal profiles (host arrays, copied at creation).  Lengths: nr or nth. */
@@ -126,6 +134,11 @@ typedef struct cssx_mhd_profiles {
     const double *dsdr1, *dsdr2; /* wall entropy gradients (nth) for constant-flux walls */
     const double *sr1, *sr2;     /* wall entropies (nth) for constant-temperature walls */
     const double *diffusion_speed;/* SLD speed^2 per radius, nr+2 entries (0..nr+1) */
+    /* explicit-diffusion profiles (nr), read when laplacian or slaplacian: viscosity mur and
+     * d ln mur/dr, thermal diffusivity ks and d ln ks/dr, resistivity etar
+     * (mhdsph_mpi.f90:350-354), and d2x/dr2 of the radial map (rhs_mpi.f90:1277-1283; zeros on a
+     * uniform grid) */
+    const double *mur, *dlnmu, *ks, *dlnks, *etar, *d2xdr2;
 } cssx_mhd_profiles;
 
 typedef struct cssx_mhd cssx_mhd;
@@ -173,7 +186,9 @@ int cssx_rk2_tvd(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *s
 int cssx_diffusion(cssx_mhd *m, int prec, const void *state, const void *b, void *dif, void *stream);
 
 /* Compute_Step_Size (rhs_mpi.f90:92-275): dt = cfl_safety * min over points of the sound,
- * flow and Alfven crossing times.  Synchronizes the stream; *dt_out on the host. */
+ * flow and Alfven crossing times, and with laplacian the diffusive limit
+ * min(dr, r1 min(min(sin th1, sin thN) dphi, dth))^2 / max(ks, mur/rho_bar, etar) (:194-198).
+ * Synchronizes the stream; *dt_out on the host. */
 int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream);
 
 /* The conservation totals Compute_Step_Size prints (rhs_mpi.f90:156-216, the scalar.dat

@@ -6,7 +6,10 @@
  * header include/cssx.h (data layout only).
  *
  * Restated (reference file:line):
- *   RHS                    rhs_mpi.f90:279-783  (stratified, Laplacian=F, do_radheat=F)
+ *   RHS                    rhs_mpi.f90:279-783  (stratified, do_radheat=F; Laplacian and
+ *                          SLaplacian branches included)
+ *   Laplacian_Cross_Terms  rhs_mpi.f90:2070-2296
+ *   VectorCurl             derivatives.f90:1894-2104
  *   Compute_B/VectorCurlA  derivatives.f90:1830-1892, :2106-2294
  *   Source_Theta_Terms     rhs_mpi.f90:786-1029
  *   Phi_Terms              rhs_mpi.f90:1032-1261
@@ -51,6 +54,8 @@ typedef struct {
     cssx_mhd_params p;
     double gamm1, Cv, pconst, invpconst, invtconst, invCv;
     const double *r, *dr, *dxdr, *theta, *gravity, *eps, *dsdr1, *dsdr2, *sr1, *sr2;
+    const double *mur, *dlnmu, *ks, *dlnks, *etar, *d2xdr2;
+    int lap, slap;            /* Laplacian; Laplacian or SLaplacian (the thermal diffusion) */
     double *sines, *cosines;
 } orc_mhd;
 
@@ -71,6 +76,10 @@ static void setup(orc_mhd *m, const cssx_grid *g, const cssx_mhd_params *p, cons
     m->r = pr->r; m->dr = pr->dr; m->dxdr = pr->dxdr; m->theta = pr->theta;
     m->gravity = pr->gravity; m->eps = pr->eps;
     m->dsdr1 = pr->dsdr1; m->dsdr2 = pr->dsdr2; m->sr1 = pr->sr1; m->sr2 = pr->sr2;
+    m->mur = pr->mur; m->dlnmu = pr->dlnmu; m->ks = pr->ks; m->dlnks = pr->dlnks; m->etar = pr->etar;
+    m->d2xdr2 = pr->d2xdr2;
+    m->lap = p->laplacian != 0;
+    m->slap = p->laplacian || p->slaplacian;
     m->sines = (double *)malloc(sizeof(double) * 2 * g->nth);
     m->cosines = m->sines + g->nth;
     for (it = 0; it < g->nth; it++) { m->sines[it] = sin(pr->theta[it]); m->cosines[it] = cos(pr->theta[it]); }
@@ -105,6 +114,36 @@ static void dr_(const orc_mhd *m, const double *x, double *out, int code, const 
         ghi[k] = whi ? whi[k] : 0.0;
     }
     orc_deriv_field(0, 1, m->nth, m->nphi, m->nr, x, out, 1, 1, code, glo, ghi, m->dri, m->dri, m->dxdr);
+}
+
+/* second derivatives (D2bydt2 / D2bydp2 / D2bydr2): scale the inverse spacing squared; a
+ * prescribed wall derivative enters divided by the radial dri on theta as well (derivatives.f90
+ * :1322, :1338).  d2r_ returns the x-space derivative (callers multiply by dxdr^2). */
+static void d2th(const orc_mhd *m, const double *x, double *out, int code, const double *wlo, const double *whi,
+                 double *glo, double *ghi)
+{
+    long k, np = (long)m->nphi * m->nr;
+    for (k = 0; k < np; k++) {
+        glo[k * 4] = wlo ? wlo[k] : 0.0;
+        ghi[k * 4] = whi ? whi[k] : 0.0;
+    }
+    orc_deriv_field(1, 2, m->nth, m->nphi, m->nr, x, out, 1, 1, code, glo, ghi, m->dti * m->dti, m->dri, NULL);
+}
+
+static void d2ph(const orc_mhd *m, const double *x, double *out)
+{
+    orc_deriv_field(2, 2, m->nth, m->nphi, m->nr, x, out, 0, 0, 0, NULL, NULL, m->dpi * m->dpi, m->dri, NULL);
+}
+
+static void d2r_(const orc_mhd *m, const double *x, double *out, int code, const double *wlo, const double *whi,
+                 double *glo, double *ghi)
+{
+    long k, np = (long)m->nphi * m->nth;
+    for (k = 0; k < np; k++) {
+        glo[k] = wlo ? wlo[k] : 0.0;
+        ghi[k] = whi ? whi[k] : 0.0;
+    }
+    orc_deriv_field(0, 2, m->nth, m->nphi, m->nr, x, out, 1, 1, code, glo, ghi, m->dri * m->dri, m->dri, NULL);
 }
 
 /* Compute_B: B = curl A with wall zeroing; a, b point to 3 fields (theta, phi, r). */
@@ -177,6 +216,58 @@ static void compute_b(const orc_mhd *m, const double *a, double *b, double *work
         }
 }
 
+#define ORC_C_LIGHT 2.99792458e10   /* c_light, constants.f90:7 */
+
+/* VectorCurl (derivatives.f90:1894-2104): J = curl B into jt, jp, jr.  theta sweeps of sin*B_phi
+ * and B_r with code 1 under Theta_Normal_Field, else 0 (their wall values then go unused);
+ * phi sweeps periodic; r sweeps of r*B_theta, r*B_phi with codes ibcr(7), ibcr(8) and zero wall
+ * values (the radial-field walls zero those halos, rhs_mpi.f90:596-601). */
+static void vector_curl(const orc_mhd *m, const double *bt, const double *bp, const double *br, double *jt,
+                        double *jp, double *jr, double *work, double *work1, double *work2, double *wlo,
+                        double *whi, double *glo, double *ghi)
+{
+    const int nr = m->nr, nth = m->nth, nphi = m->nphi;
+    const int code = m->p.theta_normal_field ? 1 : 0;
+    int it, ip, ir;
+    long i, kk;
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++) {
+            kk = (long)ir * nphi + ip;
+            /* sin_wall * binfo(2,*)(7) = sin * (-cot B_phi) with Theta_Normal_Field */
+            wlo[kk] = m->p.theta_normal_field ? -m->cosines[0] * bp[IDX(m, 0, ip, ir)] : 0.0;
+            whi[kk] = m->p.theta_normal_field ? -m->cosines[nth - 1] * bp[IDX(m, nth - 1, ip, ir)] : 0.0;
+            for (it = 0; it < nth; it++) work[IDX(m, it, ip, ir)] = m->sines[it] * bp[IDX(m, it, ip, ir)];
+        }
+    dth(m, work, work1, code, wlo, whi, glo, ghi);
+    dth(m, br, work2, code, NULL, NULL, glo, ghi);
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                i = IDX(m, it, ip, ir);
+                jr[i] = work1[i] * (1.0 / m->sines[it]) * (1.0 / m->r[ir]);
+                jp[i] = -work2[i] * (1.0 / m->r[ir]);
+            }
+    dph(m, br, work1);
+    dph(m, bt, work2);
+    for (ir = 0; ir < nr; ir++)
+        for (ip = 0; ip < nphi; ip++)
+            for (it = 0; it < nth; it++) {
+                i = IDX(m, it, ip, ir);
+                jt[i] = (1.0 / m->r[ir]) * work1[i] * (1.0 / m->sines[it]);
+                jr[i] = jr[i] - (1.0 / m->r[ir]) * work2[i] * (1.0 / m->sines[it]);
+            }
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) work[i] = m->r[ir] * bt[i];
+    dr_(m, work, work1, m->p.ibcr[7], NULL, NULL, glo, ghi);
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) jp[i] = jp[i] + work1[i] * (1.0 / m->r[ir]);
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) work[i] = m->r[ir] * bp[i];
+    dr_(m, work, work1, m->p.ibcr[8], NULL, NULL, glo, ghi);
+    for (ir = 0; ir < nr; ir++)
+        for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) jt[i] = jt[i] - work1[i] * (1.0 / m->r[ir]);
+}
+
 /* RHS on state v (nv fields with A in 6-8) -> dv (nv fields); bout (3 fields) optional. */
 int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, const double *v,
             double *dv, double *bout)
@@ -187,7 +278,10 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
     double *B, *T, *ir_, *work, *work1, *work2, *wlo, *whi, *glo, *ghi;
     const double *rho, *w, *vv, *u, *s, *bt, *bp, *br;
     double *drho, *dw, *dvv, *du, *ds;
+    double *v2[5] = {NULL, NULL, NULL, NULL, NULL};   /* the Laplacian branches' vars2 */
     setup(m, g, p, pr);
+    if (m->slap && (!pr->ks || !pr->dlnks || !pr->d2xdr2)) return 22;
+    if (m->lap && (!pr->mur || !pr->dlnmu || !pr->etar)) return 22;
     N = m->N; nr = m->nr; nth = m->nth; nphi = m->nphi;
     nv = p->magnetic ? 8 : 5;
     np = (long)nphi * (nr > nth ? nr : nth) * 4 + 16;
@@ -202,6 +296,9 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
     glo = (double *)calloc(np * 4 > 4L * nth * nphi ? np * 4 : 4L * nth * nphi, sizeof(double));
     ghi = (double *)calloc(np * 4 > 4L * nth * nphi ? np * 4 : 4L * nth * nphi, sizeof(double));
     if (!B || !T || !ir_ || !work || !work1 || !work2 || !wlo || !whi || !glo || !ghi) return 12;
+    if (m->lap)
+        for (i = 0; i < 5; i++)
+            if (!(v2[i] = (double *)calloc(N, sizeof(double)))) return 12;
     rho = v; w = v + N; vv = v + 2 * N; u = v + 3 * N; s = v + 4 * N;
     drho = dv; dw = dv + N; dvv = dv + 2 * N; du = dv + 3 * N; ds = dv + 4 * N;
     memset(dv, 0, sizeof(double) * nv * N);
@@ -226,7 +323,6 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
     /* ---- Source_Theta_Terms (rhs_mpi.f90:786-1029) ---- */
     {
         const double om2 = 2.0 * p->omega0;
-        double *tw = work1;       /* theta derivative of the current field */
         long kk;
         /* theta wall values of v: cot(theta)*v (rhs_mpi.f90:414, :441) */
         for (ir = 0; ir < nr; ir++)
@@ -235,15 +331,29 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                 wlo[kk] = m->cosines[0] / m->sines[0] * vv[IDX(m, 0, ip, ir)];
                 whi[kk] = m->cosines[nth - 1] / m->sines[nth - 1] * vv[IDX(m, nth - 1, ip, ir)];
             }
-        /* the six theta derivatives first (each is a whole-field call here) */
+        /* the theta derivatives first (each is a whole-field call here); the second
+         * derivatives of the Laplacian branches use the same wall values */
         {
             double *d_w = (double *)malloc(sizeof(double) * N), *d_v = (double *)malloc(sizeof(double) * N);
             double *d_u = (double *)malloc(sizeof(double) * N), *d_s = (double *)malloc(sizeof(double) * N);
             double *d_bp = NULL, *d_br = NULL;
+            double *e_w = NULL, *e_v = NULL, *e_u = NULL, *e_s = NULL;
             dth(m, w, d_w, p->ibct[2], NULL, NULL, glo, ghi);
             dth(m, vv, d_v, p->ibct[3], wlo, whi, glo, ghi);
             dth(m, u, d_u, p->ibct[4], NULL, NULL, glo, ghi);
             dth(m, s, d_s, p->ibct[5], NULL, NULL, glo, ghi);
+            if (m->lap) {
+                e_w = (double *)malloc(sizeof(double) * N);
+                e_v = (double *)malloc(sizeof(double) * N);
+                e_u = (double *)malloc(sizeof(double) * N);
+                d2th(m, w, e_w, p->ibct[2], NULL, NULL, glo, ghi);
+                d2th(m, vv, e_v, p->ibct[3], wlo, whi, glo, ghi);
+                d2th(m, u, e_u, p->ibct[4], NULL, NULL, glo, ghi);
+            }
+            if (m->slap) {
+                e_s = (double *)malloc(sizeof(double) * N);
+                d2th(m, s, e_s, p->ibct[5], NULL, NULL, glo, ghi);
+            }
             if (p->magnetic) {
                 /* Theta_Normal_Field wall values: bp -> -cot*bp, br -> 0 (rhs_mpi.f90:418-422) */
                 for (ir = 0; ir < nr; ir++)
@@ -260,12 +370,14 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
             for (ir = 0; ir < nr; ir++) {
                 const double ri = 1.0 / m->r[ir], rv = m->r[ir], epsr = m->eps ? m->eps[ir] : 0.0;
                 const double omsq = rv * p->omega0 * p->omega0;
+                const double dlm = m->lap ? m->dlnmu[ir] : 0.0;
                 for (ip = 0; ip < nphi; ip++)
                     for (it = 0; it < nth; it++) {
                         const long q = IDX(m, it, ip, ir);
                         const double sn = m->sines[it], cs = m->cosines[it], cot = cs / sn;
                         double Rho = rho[q], W = w[q], V = vv[q], U = u[q], t = T[q];
-                        double xdw, xdv, xdu, xdrho, xds, wk;
+                        double xdw, xdv, xdu, xds, wk, work6 = 0.0, ddiff;
+                        double xdrho;
                         xdw = omsq * sn * cs;
                         xdu = -m->gravity[ir] + omsq * sn * sn;
                         xdrho = -Rho * W * cot * ri;
@@ -274,23 +386,55 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                         wk = d_w[q] * ri;
                         xdrho = xdrho - Rho * wk;
                         xdw = xdw - W * wk;
+                        if (m->lap) {   /* :844-864 */
+                            work6 = m->mur[ir] * ir_[q];
+                            ddiff = -(4.0 / 3.0) * ri * ri;
+                            xdw = xdw + (ddiff * (1.0 / (sn * sn)) - ri * dlm) * work6 * W;
+                            v2[0][q] = wk;
+                            wk = wk * work6;
+                            ddiff = -(1.0 / 3.0) * (7.0 * ri + 2.0 * dlm);
+                            xdu = xdu + ddiff * (ri * work6 * cot * W + wk);
+                            ddiff = (4.0 / 3.0) * ri;
+                            xdw = xdw + ddiff * cot * wk;
+                            xdw = xdw + ddiff * ri * work6 * e_w[q];
+                        }
                         xdw = xdw + V * (ri * cot * V + om2 * cs);
                         xdv = xdv - ri * V * W * cot;
                         xdu = xdu + V * (ri * V + om2 * sn);
                         wk = d_v[q] * ri;
                         xdv = xdv - W * wk;
+                        if (m->lap) {   /* :883-891 */
+                            xdv = xdv + ri * work6 * (cot * wk - (ri * (1.0 / (sn * sn)) + dlm) * V);
+                            v2[1][q] = wk;
+                            xdv = xdv + ri * ri * work6 * e_v[q];
+                        }
                         drho[q] = xdrho - 2.0 * ri * Rho * U;
                         xdv = xdv - U * (ri * V + om2 * sn);
                         xdw = xdw - ri * U * W;
                         if (!p->magnetic) dvv[q] = xdv;
                         wk = d_u[q] * ri;
                         xdu = xdu - W * wk;
+                        if (m->lap) {   /* :914-926 */
+                            v2[2][q] = wk;
+                            wk = wk * work6;
+                            xdw = xdw + (2.0 * (4.0 / 3.0) * ri + dlm) * wk;
+                            xdu = xdu + ri * cot * wk;
+                            xdu = xdu + ri * ri * work6 * e_u[q];
+                        }
                         if (!p->magnetic) du[q] = xdu;
                         xds = (fabs(epsr) > 0.0) ? epsr * ir_[q] / t : 0.0;
                         wk = d_s[q] * ri;
                         if (p->magnetic) xdw = xdw - m->gamm1 * t * wk;
                         else dw[q] = xdw - m->gamm1 * t * wk;
                         xds = xds - W * wk;
+                        if (m->slap) {   /* :959-975: the second-derivative term carries ri^3 */
+                            const double oorho = ir_[q] / t;
+                            ddiff = ri * m->ks[ir];
+                            wk = ddiff * wk * oorho;
+                            xds = xds + cot * wk;
+                            ddiff = ddiff * ri * ri;
+                            xds = xds + ddiff * e_s[q] * oorho;
+                        }
                         ds[q] = xds;
                         if (p->magnetic) {
                             double oorho = ir_[q] * ONE_OVER_4PI * ri;
@@ -310,8 +454,8 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                     }
             }
             free(d_w); free(d_v); free(d_u); free(d_s); free(d_bp); free(d_br);
+            free(e_w); free(e_v); free(e_u); free(e_s);
         }
-        (void)tw;
     }
 
     /* ---- Phi_Terms (rhs_mpi.f90:1032-1261) ---- */
@@ -319,7 +463,18 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
         double *d_rho = (double *)malloc(sizeof(double) * N), *d_v = (double *)malloc(sizeof(double) * N);
         double *d_w = (double *)malloc(sizeof(double) * N), *d_u = (double *)malloc(sizeof(double) * N);
         double *d_s = (double *)malloc(sizeof(double) * N), *d_bt = NULL, *d_br = NULL;
+        double *e_v = NULL, *e_w = NULL, *e_u = NULL, *e_s = NULL;
         dph(m, rho, d_rho); dph(m, vv, d_v); dph(m, w, d_w); dph(m, u, d_u); dph(m, s, d_s);
+        if (m->lap) {
+            e_v = (double *)malloc(sizeof(double) * N);
+            e_w = (double *)malloc(sizeof(double) * N);
+            e_u = (double *)malloc(sizeof(double) * N);
+            d2ph(m, vv, e_v); d2ph(m, w, e_w); d2ph(m, u, e_u);
+        }
+        if (m->slap) {
+            e_s = (double *)malloc(sizeof(double) * N);
+            d2ph(m, s, e_s);
+        }
         if (p->magnetic) {
             d_bt = (double *)malloc(sizeof(double) * N);
             d_br = (double *)malloc(sizeof(double) * N);
@@ -327,12 +482,14 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
         }
         for (ir = 0; ir < nr; ir++) {
             const double ri = 1.0 / m->r[ir];
+            const double dlm = m->lap ? m->dlnmu[ir] : 0.0;
             for (ip = 0; ip < nphi; ip++)
                 for (it = 0; it < nth; it++) {
                     const long q = IDX(m, it, ip, ir);
                     const double oors = ri * (1.0 / m->sines[it]);
-                    double Rho = rho[q], V = vv[q], t = T[q], oorho = ir_[q];
-                    double xdrho, xdv, xdw, xdu, xds, wk;
+                    const double cot = m->cosines[it] / m->sines[it];
+                    double Rho = rho[q], V = vv[q], W = w[q], U = u[q], t = T[q], oorho = ir_[q];
+                    double xdrho, xdv, xdw, xdu, xds, wk, work2 = 0.0, ddiff, ddiff2;
                     wk = d_rho[q] * oors;
                     xdrho = drho[q];
                     xdv = dvv[q];
@@ -343,17 +500,46 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                     xdv = xdv - V * wk;
                     drho[q] = xdrho - Rho * wk;
                     xdw = dw[q];
+                    xdu = du[q];
+                    if (m->lap) {   /* :1123-1141 */
+                        work2 = oorho * m->mur[ir];
+                        v2[3][q] = wk;
+                        /* viscous heating theta-theta, phi-phi */
+                        xds = xds + (4.0 / 3.0) * work2 * (v2[0][q] + U * ri) *
+                                        (v2[0][q] + 2.0 * U * ri - wk - ri * cot * W) / t;
+                        wk = wk * work2;
+                        ddiff = -(7.0 / 3.0) * ri;
+                        ddiff2 = ddiff - (2.0 / 3.0) * dlm;
+                        xdw = xdw + ddiff * cot * wk;
+                        xdu = xdu + ddiff2 * wk;
+                        xdv = xdv + (4.0 / 3.0) * (e_v[q] * oors * oors) * work2;
+                    }
                     wk = d_w[q] * oors;
                     xdw = xdw - V * wk;
-                    xdu = du[q];
+                    if (m->lap) {   /* :1156-1169: viscous heating theta-phi */
+                        xds = xds + work2 * (v2[1][q] + wk - ri * cot * V) * (v2[1][q] + wk - ri * cot * V) / t;
+                        v2[1][q] = wk;
+                        ddiff = (7.0 / 3.0) * ri;
+                        wk = wk * work2;
+                        xdv = xdv + ddiff * cot * wk;
+                        xdw = xdw + (e_w[q] * oors * oors) * work2;
+                    }
                     if (!p->magnetic) dw[q] = xdw;
                     wk = d_u[q] * oors;
                     xdu = xdu - V * wk;
+                    if (m->lap) {   /* :1189-1198 */
+                        v2[4][q] = wk;
+                        ddiff = 2.0 * (4.0 / 3.0) * ri + dlm;
+                        xdv = xdv + ddiff * wk * work2;
+                        xdu = xdu + (e_u[q] * oors * oors) * work2;
+                    }
                     if (!p->magnetic) du[q] = xdu;
                     wk = d_s[q] * oors;
                     xds = xds - V * wk;
                     xdv = xdv - m->gamm1 * t * wk;
                     if (!p->magnetic) dvv[q] = xdv;
+                    if (m->slap)   /* :1221-1228 */
+                        xds = xds + m->ks[ir] * (e_s[q] * (oors * oors)) * (oorho / t);
                     ds[q] = xds;
                     if (p->magnetic) {
                         double Bt = bt[q], Bp = bp[q], Br = br[q];
@@ -368,6 +554,7 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                 }
         }
         free(d_rho); free(d_v); free(d_w); free(d_u); free(d_s); free(d_bt); free(d_br);
+        free(e_v); free(e_w); free(e_u); free(e_s);
     }
 
     /* ---- radial wall values (rhs_mpi.f90:578-637) and Radial_Terms (:1264-1519) ---- */
@@ -375,8 +562,15 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
         double *d_w = (double *)malloc(sizeof(double) * N), *d_v = (double *)malloc(sizeof(double) * N);
         double *d_u = (double *)malloc(sizeof(double) * N), *d_s = (double *)malloc(sizeof(double) * N);
         double *d_bt = NULL, *d_bp = NULL;
+        double *e_w = NULL, *e_v = NULL, *e_u = NULL, *e_s = NULL;
         const double r1 = m->r[0], r2 = m->r[nr - 1], x1 = m->dxdr[0], x2 = m->dxdr[nr - 1];
         long kk;
+        if (m->lap) {
+            e_w = (double *)malloc(sizeof(double) * N);
+            e_v = (double *)malloc(sizeof(double) * N);
+            e_u = (double *)malloc(sizeof(double) * N);
+        }
+        if (m->slap) e_s = (double *)malloc(sizeof(double) * N);
         for (ip = 0; ip < nphi; ip++)
             for (it = 0; it < nth; it++) {
                 kk = (long)ip * nth + it;
@@ -384,6 +578,7 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                 whi[kk] = p->top_stress_free ? w[IDX(m, it, ip, nr - 1)] / r2 / x2 : w[IDX(m, it, ip, nr - 1)];
             }
         dr_(m, w, d_w, p->ibcr[2], wlo, whi, glo, ghi);
+        if (m->lap) d2r_(m, w, e_w, p->ibcr[2], wlo, whi, glo, ghi);
         for (ip = 0; ip < nphi; ip++)
             for (it = 0; it < nth; it++) {
                 kk = (long)ip * nth + it;
@@ -391,39 +586,85 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                 whi[kk] = p->top_stress_free ? vv[IDX(m, it, ip, nr - 1)] / r2 / x2 : vv[IDX(m, it, ip, nr - 1)];
             }
         dr_(m, vv, d_v, p->ibcr[3], wlo, whi, glo, ghi);
+        if (m->lap) d2r_(m, vv, e_v, p->ibcr[3], wlo, whi, glo, ghi);
         dr_(m, u, d_u, p->ibcr[4], NULL, NULL, glo, ghi);
+        if (m->lap) d2r_(m, u, e_u, p->ibcr[4], NULL, NULL, glo, ghi);
+        /* constant-flux entropy walls: dsdr/dxdr, or -lumr/(4 pi r^2 ks dxdr) under the explicit
+         * thermal diffusion (:584-590, :616-622) */
         for (ip = 0; ip < nphi; ip++)
             for (it = 0; it < nth; it++) {
                 kk = (long)ip * nth + it;
-                wlo[kk] = p->bottom_const_flux ? m->dsdr1[it] / x1 : 0.0;
-                whi[kk] = p->top_const_flux ? m->dsdr2[it] / x2 : 0.0;
+                wlo[kk] = !p->bottom_const_flux ? 0.0
+                          : m->slap ? -p->lumr1 * ONE_OVER_4PI / (r1 * r1 * m->ks[0] * x1) : m->dsdr1[it] / x1;
+                whi[kk] = !p->top_const_flux ? 0.0
+                          : m->slap ? -p->lumr2 * ONE_OVER_4PI / (r2 * r2 * m->ks[nr - 1] * x2) : m->dsdr2[it] / x2;
             }
         dr_(m, s, d_s, p->ibcr[5], wlo, whi, glo, ghi);
+        if (m->slap) d2r_(m, s, e_s, p->ibcr[5], wlo, whi, glo, ghi);
         if (p->magnetic) {
             d_bt = (double *)malloc(sizeof(double) * N);
             d_bp = (double *)malloc(sizeof(double) * N);
             dr_(m, bt, d_bt, p->ibcr[7], NULL, NULL, glo, ghi);
             dr_(m, bp, d_bp, p->ibcr[8], NULL, NULL, glo, ghi);
         }
-        for (ir = 0; ir < nr; ir++)
+        for (ir = 0; ir < nr; ir++) {
+            const double x = m->dxdr[ir], ri = 1.0 / m->r[ir];
+            /* Radial_Terms' work9 = d2xdr2/dxdr, work5 = 2/r + dlnmu, work6 = (2/r + dlnks) ks */
+            const double w9 = m->slap ? m->d2xdr2[ir] / x : 0.0;
+            const double w5 = m->lap ? 2.0 * ri + m->dlnmu[ir] : 0.0;
+            const double w6 = m->slap ? (2.0 * ri + m->dlnks[ir]) * m->ks[ir] : 0.0;
             for (ip = 0; ip < nphi; ip++)
                 for (it = 0; it < nth; it++) {
                     const long q = IDX(m, it, ip, ir);
-                    double U = u[q], t = T[q], xdw, xdv, xdu, xds, wk;
+                    const double cot = m->cosines[it] / m->sines[it];
+                    double U = u[q], W = w[q], V = vv[q], t = T[q], xdw, xdv, xdu, xds, wk, work2 = 0.0, pw;
                     wk = d_w[q];
                     xdw = dw[q] - U * wk;
+                    xdu = du[q];
+                    xds = ds[q];
+                    if (m->lap) {   /* :1317-1343 */
+                        work2 = m->mur[ir] * ir_[q];
+                        xdu = xdu + (1.0 / 3.0) * cot * ri * wk;   /* no mur/rho factor here (:1322) */
+                        xdw = xdw + work2 * w5 * wk;
+                        /* viscous heating r-theta */
+                        xds = xds + work2 * (wk + v2[2][q] - W * ri) * (wk + v2[2][q] - W * ri) / t;
+                        xdw = xdw + w9 * wk * work2;
+                        xdw = xdw + (e_w[q] * x * x) * work2;
+                    }
                     if (!p->magnetic) dw[q] = xdw;
                     wk = d_v[q];
                     xdv = dvv[q] - U * wk;
+                    if (m->lap) {   /* :1364-1380 */
+                        /* viscous heating r-phi (listed as r-theta in the reference) */
+                        xds = xds + work2 * (wk + v2[4][q] - V * ri) * (wk + v2[4][q] - V * ri) / t;
+                        xdv = xdv + wk * w5 * work2;
+                        xdv = xdv + w9 * wk * work2;
+                        xdv = xdv + (e_v[q] * x * x) * work2;
+                    }
                     if (!p->magnetic) dvv[q] = xdv;
-                    xdu = du[q];
                     wk = d_u[q];
                     xdu = xdu - U * wk;
                     drho[q] = drho[q] - rho[q] * wk;
-                    xds = ds[q];
+                    if (m->lap) {   /* :1409-1431; the r-phi heating term appears again with du/dr in
+                                       place of dv/dr (:1418) */
+                        xdu = xdu + (4.0 / 3.0) * work2 * w5 * (wk - U * ri);
+                        xdu = xdu + (4.0 / 3.0) * w9 * wk * work2;
+                        xds = xds + (4.0 / 3.0) * work2 * wk * (wk - v2[0][q] - U * ri) / t;
+                        xds = xds + work2 * (wk + v2[4][q] - V * ri) * (wk + v2[4][q] - V * ri) / t;
+                        pw = v2[3][q] + (cot * W + U) * ri;
+                        xds = xds + (4.0 / 3.0) * work2 * pw * (pw - wk) / t;
+                        xdu = xdu + (4.0 / 3.0) * (e_u[q] * x * x) * work2;
+                    }
                     wk = d_s[q];
                     xdu = xdu - m->gamm1 * t * wk;
                     xds = xds - U * wk;
+                    if (m->slap) {   /* :1456-1472 */
+                        const double oorho = ir_[q] / t;
+                        wk = wk * oorho;
+                        xds = xds + wk * w6;
+                        xds = xds + w9 * wk * m->ks[ir];
+                        xds = xds + m->ks[ir] * (e_s[q] * x * x) * oorho;
+                    }
                     if (!p->magnetic) du[q] = xdu;
                     ds[q] = xds;
                     if (p->magnetic) {
@@ -436,7 +677,85 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                         du[q] = xdu - bp[q] * wk;
                     }
                 }
+        }
         free(d_w); free(d_v); free(d_u); free(d_s); free(d_bt); free(d_bp);
+        free(e_w); free(e_v); free(e_u); free(e_s);
+    }
+
+    /* ---- Laplacian_Cross_Terms_Radial / _Theta (rhs_mpi.f90:2070-2296) ---- */
+    if (m->lap) {
+        double *X = (double *)malloc(sizeof(double) * N), *dX = (double *)malloc(sizeof(double) * N);
+        double *tmp;
+        long kk;
+        const double r1 = m->r[0], r2 = m->r[nr - 1];
+        /* vars2 slots 2 and 5 trade places (:2103-2105): 1 dw/dth/r, 2 du/dph/(r sin),
+         * 3 du/dth/r, 4 dv/dph/(r sin), 5 dw/dph/(r sin) */
+        tmp = v2[1]; v2[1] = v2[4]; v2[4] = tmp;
+        /* radial: d/dr of r*vars2(k) with wall values r_wall*dbinfo; only dbinfo(1,1)(1) and
+         * dbinfo(1,2)(4) are set, both under Bottom_Stress_Free (:2113-2125), the rest stay 0.
+         * The mur/r/rho factor takes invrho at the point itself (the reference indexes
+         * invrho(it+ir-1,...) with it unset, :2133). */
+        {
+            static const int slot[4] = {0, 1, 2, 3};
+            const int code[4] = {p->ibcr[2], p->ibcr[4], p->ibcr[4], p->ibcr[3]};
+            int k;
+            for (k = 0; k < 4; k++) {
+                for (ir = 0; ir < nr; ir++)
+                    for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) X[i] = m->r[ir] * v2[slot[k]][i];
+                for (ip = 0; ip < nphi; ip++)
+                    for (it = 0; it < nth; it++) {
+                        kk = (long)ip * nth + it;
+                        wlo[kk] = (k == 0 && p->bottom_stress_free)
+                                      ? r1 * (v2[0][IDX(m, it, ip, 0)] / r1 / m->dxdr[0]) : 0.0;
+                        whi[kk] = (k == 3 && p->bottom_stress_free)
+                                      ? r2 * (v2[3][IDX(m, it, ip, nr - 1)] / r2 / m->dxdr[nr - 1]) : 0.0;
+                    }
+                dr_(m, X, dX, code[k], wlo, whi, glo, ghi);
+                for (ir = 0; ir < nr; ir++) {
+                    const double c6 = (1.0 / 3.0) * m->mur[ir] / m->r[ir];
+                    for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) {
+                        const double add = c6 * ir_[i] * dX[i];
+                        if (k == 0 || k == 3) du[i] += add;
+                        else if (k == 1) dvv[i] += add;
+                        else dw[i] += add;
+                    }
+                }
+            }
+        }
+        /* theta: d/dth of sin*vars2(4) and sin*vars2(5), wall values sin_wall*dbinfo with
+         * dbinfo(2,*)(4) = vars2(4) cot at the walls (:2243-2250) and (5) = 0 */
+        {
+            int k;
+            for (k = 0; k < 2; k++) {
+                const double *src = v2[3 + k];
+                for (ir = 0; ir < nr; ir++)
+                    for (ip = 0; ip < nphi; ip++)
+                        for (it = 0; it < nth; it++) {
+                            const long q = IDX(m, it, ip, ir);
+                            X[q] = m->sines[it] * src[q];
+                        }
+                for (ir = 0; ir < nr; ir++)
+                    for (ip = 0; ip < nphi; ip++) {
+                        kk = (long)ir * nphi + ip;
+                        wlo[kk] = k == 0 ? m->sines[0] * (src[IDX(m, 0, ip, ir)] * m->cosines[0] / m->sines[0]) : 0.0;
+                        whi[kk] = k == 0 ? m->sines[nth - 1] * (src[IDX(m, nth - 1, ip, ir)] * m->cosines[nth - 1] /
+                                                                 m->sines[nth - 1])
+                                         : 0.0;
+                    }
+                dth(m, X, dX, k == 0 ? p->ibct[3] : p->ibct[2], wlo, whi, glo, ghi);
+                for (ir = 0; ir < nr; ir++) {
+                    const double rm = m->mur[ir] / m->r[ir];
+                    for (ip = 0; ip < nphi; ip++)
+                        for (it = 0; it < nth; it++) {
+                            const long q = IDX(m, it, ip, ir);
+                            const double add = (1.0 / 3.0) * rm * ir_[q] / m->sines[it] * dX[q];
+                            if (k == 0) dw[q] += add;
+                            else dvv[q] += add;
+                        }
+                }
+            }
+        }
+        free(X); free(dX);
     }
 
     /* ---- Density_Terms_Theta (rhs_mpi.f90:1522-1605) ---- */
@@ -495,20 +814,41 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
         }
     }
 
-    /* ---- Induction_Terms: u x B (rhs_mpi.f90:2342-2346) ---- */
+    /* ---- Induction_Terms: u x B (rhs_mpi.f90:2342-2346); with Laplacian the resistive term
+     * -etar curl B and the Ohmic heating 4 pi etar |curl B|^2 / c^2 / (rho T) (:2303-2322) ---- */
     if (p->magnetic) {
         double *dat = dv + 5 * N, *dap = dv + 6 * N, *dar = dv + 7 * N;
+        if (m->lap) {
+            const double jconst = 4.0 * PI_ / (ORC_C_LIGHT * ORC_C_LIGHT);
+            vector_curl(m, bt, bp, br, dat, dap, dar, work, work1, work2, wlo, whi, glo, ghi);
+            for (ir = 0; ir < nr; ir++) {
+                const double eta = m->etar[ir];
+                for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) {
+                    v2[0][i] = -eta * dat[i];
+                    v2[1][i] = -eta * dap[i];
+                    v2[2][i] = -eta * dar[i];
+                    ds[i] = ds[i] + (jconst * eta) * (dat[i] * dat[i] + dap[i] * dap[i] + dar[i] * dar[i]) * ir_[i] / T[i];
+                }
+            }
+        }
         for (i = 0; i < N; i++) {
             dat[i] = vv[i] * br[i] - u[i] * bp[i];
             dap[i] = u[i] * bt[i] - w[i] * br[i];
             dar[i] = w[i] * bp[i] - vv[i] * bt[i];
         }
+        if (m->lap)
+            for (i = 0; i < N; i++) {
+                dat[i] += v2[0][i];
+                dap[i] += v2[1][i];
+                dar[i] += v2[2][i];
+            }
     }
     if (p->xi_cs < 1.0)
         for (i = 0; i < N; i++) drho[i] = (p->xi_cs * p->xi_cs) * drho[i];
 
     free(B); free(T); free(ir_); free(work); free(work1); free(work2);
     free(wlo); free(whi); free(glo); free(ghi); free(m->sines);
+    for (i = 0; i < 5; i++) free(v2[i]);
     return 0;
 }
 
