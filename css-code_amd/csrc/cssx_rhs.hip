@@ -1,20 +1,25 @@
 // MHD right-hand side, Bound and RK4 stages on gfx950.
 //
-// RHS(call_num) of rhs_mpi.f90:279-783 (stratified, magnetic, Laplacian=F, do_radheat=F) is
-// evaluated as:
+// RHS(call_num) of rhs_mpi.f90:279-783 (stratified, magnetic, do_radheat=F; with or without the
+// Laplacian / SLaplacian explicit diffusion) is evaluated as:
 //   1. curl A: six batched derivative sweeps + k_curl_combine (Compute_B, derivatives.f90:1830-1892)
 //   2. k_walls_state: wall values of the closures that depend on the state (rhs_mpi.f90:410-455,
 //      :578-637)
 //   3. the 19 first-derivative sweeps of Source_Theta_Terms / Phi_Terms / Radial_Terms, batched
-//      per axis (one launch per axis, one job per field)
-//   4. k_rhs_main: every pointwise term of those three routines in the reference's order
+//      per axis (one launch per axis, one job per field); with explicit diffusion also the 12
+//      second derivatives, the 6 cross-term sweeps of first-derivative slots and the 6 sweeps
+//      of curl B (Laplacian_Cross_Terms_*, VectorCurl)
+//   4. k_rhs_main: every pointwise term of those three routines in the reference's order;
+//      k_rhs_viscous adds the explicit-diffusion branches
 //   5. k_walls_density: the density wall derivatives that zero the normal momentum tendency
 //      (rhs_mpi.f90:1553-1564, :1750-1773) - they need step 4's result, hence two phases
-//   6. two density sweeps + k_rhs_final: density terms, top relaxation, u x B, xi_cs
+//   6. two density sweeps + k_rhs_final: density terms, top relaxation, u x B (and -etar curl B
+//      with its Ohmic heating), xi_cs
 // Temperature T = rho^(gamma-1) exp(s/Cv)/((gamma-1)Cv) and 1/rho are recomputed where used
 // instead of being stored (rhs_mpi.f90:377-387).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -36,6 +41,8 @@
 namespace cssx {
 
 constexpr double kOneOver4Pi = 1.0 / 3.1415926535897932384626433832795028841972 / 4.0;
+// 4 pi / c_light^2 of the Ohmic heating (c_light = 2.99792458d10, constants.f90:7)
+constexpr double kJconst = 4.0 * 3.1415926535897932384626433832795028841972 / (2.99792458e10 * 2.99792458e10);
 
 template <typename T>
 struct Prof {
@@ -55,6 +62,11 @@ struct Prof {
     // 1/(cos th_l - cos th_h) ct (nth), (th_h-th_l)/(cos th_l - cos th_h) gt (nth), and the
     // SLD speed^2 of each radius vsp (nr)
     T *rfc = nullptr, *rv = nullptr, *sph = nullptr, *rfac = nullptr, *ct = nullptr, *gt = nullptr, *vsp = nullptr;
+    // explicit diffusion (nr): viscosity mur, d ln mur/dr, thermal diffusivity ks, d ln ks/dr,
+    // resistivity etar, d2x/dr2 / dx/dr (Radial_Terms' work9), and the x-space second
+    // derivative of s_bar (the background part of d2s/dx2 in the fluctuation form)
+    T *mur = nullptr, *dlnmu = nullptr, *ks = nullptr, *dlnks = nullptr, *etar = nullptr, *w9 = nullptr,
+      *d2sbar = nullptr;
     T *buf = nullptr;
 };
 
@@ -64,6 +76,18 @@ enum Slot {
     D_PRHO, D_PV, D_PW, D_PU, D_PS, D_PBT, D_PBR,             // phi:   rho v w u s bt br
     D_RW, D_RV, D_RU, D_RS, D_RBT, D_RBP,                     // r:     w v u s bt bp
     D_NSLOT
+};
+
+// explicit-diffusion scratch slots (Laplacian / SLaplacian)
+enum LapSlot {
+    L_TW2 = 0, L_TV2, L_TU2, L_TS2,          // theta second derivatives: w v u s
+    L_PV2, L_PW2, L_PU2, L_PS2,              // phi second derivatives:   v w u s
+    L_RW2, L_RV2, L_RU2, L_RS2,              // r second derivatives (x-space): w v u s
+    L_XR1, L_XR2, L_XR3, L_XR4,              // cross terms: d/dr of dw/dth, du/dph, du/dth, dv/dph
+    L_XT1, L_XT2,                            // cross terms: d/dth of dv/dph, dw/dph
+    L_JT1, L_JT2, L_JP1, L_JP2, L_JRT, L_JRP,  // curl B: d/dth (sin B_phi), d/dth B_r, d/dph B_r,
+                                             // d/dph B_theta, d/dr (r B_theta), d/dr (r B_phi)
+    L_NSLOT
 };
 
 template <typename T>
@@ -76,6 +100,8 @@ struct Scratch {
     T *dif = nullptr;    // SLD diffusion tendency of stage 1 (nv fields, SLDiffusion runs)
     T *acc = nullptr;    // FP32 runs: RK4 tendency accumulator (nv fields)
     T *y0f = nullptr;    // FP32 runs: the FP64 state y0 in stage precision, fluctuation form
+    T *L = nullptr;      // explicit diffusion: L_NSLOT fields
+    T *lwall = nullptr;  // explicit diffusion: cross-term wall values (2 r, 2 theta buffers)
     bool ready = false;
 };
 
@@ -92,6 +118,8 @@ struct cssx_mhd {
         h_s_bar, h_diff_speed;
     // background: T_bar, compact radial derivatives of rho_bar / s_bar, radial momentum residual
     std::vector<double> h_tbar, h_drbar, h_dsbar, h_hres;
+    // explicit diffusion: profiles and the compact second derivative of s_bar (x-space)
+    std::vector<double> h_mur, h_dlnmu, h_ks, h_dlnks, h_etar, h_d2xdr2, h_d2sbar;
     double wbar_lo = 0.0, wbar_hi = 0.0;   // d rho_bar/dr imposed at Neumann density walls
     cssx::Prof<double> p64;
     cssx::Prof<float> p32;
@@ -116,6 +144,8 @@ struct Consts {
     T gamm1, invtconst, invCv, pconst, invpconst, Cv, omega0, om2, xi2, gamma, T0, damp;
     int magnetic, top_relax, xi_on;
     int fluct;   // state slots 1 and 5 hold rho - rho_bar(r) and s - s_bar(r) (FP32 path)
+    int lap, slap;        // Laplacian; Laplacian or SLaplacian (thermal diffusion)
+    T s_lap_lo, s_lap_hi; // constant-flux entropy wall values under slap: -lumr/(4 pi r^2 ks dxdr)
     T wbar_lo, wbar_hi;   // d rho_bar/dr prescribed at the Neumann density walls (0 elsewhere)
     T dT0top;             // T0 - T_bar at the top radius (FP64 difference)
 };
@@ -280,8 +310,8 @@ __global__ __launch_bounds__(256) void k_walls_state(Consts<T> c, Prof<T> P, Wal
         wall[W.off_rw_hi + i] = top_sf ? w[q1] / r2 / x2 : w[q1];
         wall[W.off_rv_lo + i] = bot_sf ? v[q0] / r1 / x1 : T(0);
         wall[W.off_rv_hi + i] = top_sf ? v[q1] / r2 / x2 : v[q1];
-        wall[W.off_rs_lo + i] = bot_flux ? P.dsdr1[it] / x1 : T(0);
-        wall[W.off_rs_hi + i] = top_flux ? P.dsdr2[it] / x2 : T(0);
+        wall[W.off_rs_lo + i] = bot_flux ? (c.slap ? c.s_lap_lo : P.dsdr1[it] / x1) : T(0);
+        wall[W.off_rs_hi + i] = top_flux ? (c.slap ? c.s_lap_hi : P.dsdr2[it] / x2) : T(0);
     }
 }
 
@@ -424,6 +454,131 @@ __global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const 
 }
 
 // ---------------------------------------------------------------------------------------------
+// 4b. explicit diffusion (Laplacian / SLaplacian branches of Source_Theta_Terms, Phi_Terms,
+//     Radial_Terms and Laplacian_Cross_Terms_Radial/_Theta, rhs_mpi.f90:844-1474, :2070-2296).
+//     Every branch adds to the tendencies without reading them, so one kernel adds the lot after
+//     k_rhs_main and before the density wall closure, which must see the viscous momentum.
+// ---------------------------------------------------------------------------------------------
+// Wall values of the cross-term sweeps.  The reference differentiates r*vars2(k) (r sweeps) and
+// sin*vars2(k) (theta sweeps), vars2 holding the first derivatives over r or r sin; here the
+// first-derivative slots themselves are swept (r * dw/dth / r = dw/dth; the 1/sin and 1/r
+// factors are constant along the pencil and applied afterwards), so the wall values are the
+// reference's divided by the same factors:
+//   r, dw/dth at r1:    r1 * vars2(1)/r1/dxdr(1)  -> D_TW/(r1 dxdr(1))      (Bottom_Stress_Free)
+//   r, dv/dph at r2:    r2 * vars2(4)/r2/dxdr(nr) -> D_PV/(r2 dxdr(nr))     (Bottom_Stress_Free,
+//                       as written at rhs_mpi.f90:2120 for the top wall)
+//   theta, dv/dph:      sin * vars2(4) cot        -> cot D_PV on both theta walls (:2243-2250)
+// every other wall value is zero (the reference's dbinfo walls are never written).
+// Layouts: r buffers (nphi, nth), theta buffers (nr, nphi).
+template <typename T>
+__global__ __launch_bounds__(256) void k_lap_walls(Consts<T> c, Prof<T> P, const T *__restrict__ D,
+                                                   T *__restrict__ lw, int bot_sf) {
+    const long nt = (long)c.nr * c.nphi, nrr = (long)c.nth * c.nphi, N = c.N;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrr) {
+        const long q0 = i, q1 = (long)(c.nr - 1) * nrr + i;
+        lw[i] = bot_sf ? D[D_TW * N + q0] * P.rinv[0] / P.dxdr[0] : T(0);
+        lw[nrr + i] = bot_sf ? D[D_PV * N + q1] * P.rinv[c.nr - 1] / P.dxdr[c.nr - 1] : T(0);
+    }
+    if (i < nt) {
+        const int ip = (int)(i % c.nphi), ir = (int)(i / c.nphi);
+        const long q0 = ((long)ir * c.nphi + ip) * c.nth, q1 = q0 + c.nth - 1;
+        lw[2 * nrr + i] = P.cot[0] * D[D_PV * N + q0];
+        lw[2 * nrr + nt + i] = P.cot[c.nth - 1] * D[D_PV * N + q1];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_viscous(Consts<T> c, Prof<T> P, const T *__restrict__ st,
+                                                     const T *__restrict__ D, const T *__restrict__ L,
+                                                     T *__restrict__ dv) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    const long N = c.N;
+    int it, ip, ir;
+    coords(i, c, it, ip, ir);
+    const T ri = P.rinv[ir], csc = P.csc[it], cot = P.cot[it];
+    const T W = st[N + i], V = st[2 * N + i], U = st[3 * N + i];
+    const Thermo<T> th = thermo(st[i], st[4 * N + i], ir, c, P);
+    const T invrho = T(1) / th.rho, t = th.t, invt = T(1) / th.t;
+    const T x = P.dxdr[ir], w9 = P.w9[ir], oors = ri * csc;
+    T ddw = T(0), ddv = T(0), ddu = T(0), dds = T(0);
+    if (c.lap) {
+        const T nu = P.mur[ir] * invrho, dlm = P.dlnmu[ir], w5 = T(2) * ri + dlm, csc2 = csc * csc;
+        const T tw = D[D_TW * N + i] * ri, tv = D[D_TV * N + i] * ri, tu = D[D_TU * N + i] * ri;
+        const T pv = D[D_PV * N + i] * oors, pw = D[D_PW * N + i] * oors, pu = D[D_PU * N + i] * oors;
+        const T rw = D[D_RW * N + i], rv = D[D_RV * N + i], ru = D[D_RU * N + i];
+        const T c43 = T(4) / T(3), c13 = T(1) / T(3), c73 = T(7) / T(3), c23 = T(2) / T(3);
+        // Source_Theta_Terms (rhs_mpi.f90:844-926)
+        ddw += (-c43 * ri * ri * csc2 - ri * dlm) * nu * W;
+        ddu += -c13 * (T(7) * ri + T(2) * dlm) * (ri * nu * cot * W + tw * nu);
+        ddw += c43 * ri * cot * tw * nu;
+        ddw += c43 * ri * ri * nu * L[L_TW2 * N + i];
+        ddv += ri * nu * (cot * tv - (ri * csc2 + dlm) * V);
+        ddv += ri * ri * nu * L[L_TV2 * N + i];
+        ddw += (T(2) * c43 * ri + dlm) * tu * nu;
+        ddu += ri * cot * tu * nu;
+        ddu += ri * ri * nu * L[L_TU2 * N + i];
+        // Phi_Terms (:1123-1198); heating theta-theta / phi-phi and theta-phi
+        dds += c43 * nu * (tw + U * ri) * (tw + T(2) * U * ri - pv - ri * cot * W) * invt;
+        ddw += -c73 * ri * cot * pv * nu;
+        ddu += (-c73 * ri - c23 * dlm) * pv * nu;
+        ddv += c43 * L[L_PV2 * N + i] * oors * oors * nu;
+        T g = tv + pw - ri * cot * V;
+        dds += nu * g * g * invt;
+        ddv += c73 * ri * cot * pw * nu;
+        ddw += L[L_PW2 * N + i] * oors * oors * nu;
+        ddv += (T(2) * c43 * ri + dlm) * pu * nu;
+        ddu += L[L_PU2 * N + i] * oors * oors * nu;
+        // Radial_Terms (:1317-1431); the cot/r dw/dr term has no mu/rho factor (:1322) and the
+        // r-phi heating term is added a second time with du/dr in place of dv/dr (:1418)
+        ddu += c13 * cot * ri * rw;
+        ddw += nu * w5 * rw + w9 * rw * nu + L[L_RW2 * N + i] * x * x * nu;
+        g = rw + tu - W * ri;
+        dds += nu * g * g * invt;
+        g = rv + pu - V * ri;
+        dds += nu * g * g * invt;
+        ddv += rv * w5 * nu + w9 * rv * nu + L[L_RV2 * N + i] * x * x * nu;
+        ddu += c43 * nu * w5 * (ru - U * ri) + c43 * w9 * ru * nu;
+        dds += c43 * nu * ru * (ru - tw - U * ri) * invt;
+        g = ru + pu - V * ri;
+        dds += nu * g * g * invt;
+        const T pp = pv + (cot * W + U) * ri;
+        dds += c43 * nu * pp * (pp - ru) * invt;
+        ddu += c43 * L[L_RU2 * N + i] * x * x * nu;
+        // Laplacian_Cross_Terms_Radial / _Theta (:2126-2295): mur/(3 r rho) times d/dr of
+        // r*vars2 (swept slots carry dx/dr), and mur/(3 r rho sin) times d/dth of sin*vars2
+        const T c6 = c13 * nu * ri;
+        ddu += c6 * L[L_XR1 * N + i];
+        ddv += c6 * L[L_XR2 * N + i] * csc;
+        ddw += c6 * L[L_XR3 * N + i];
+        ddu += c6 * L[L_XR4 * N + i] * csc;
+        ddw += c6 * csc * ri * L[L_XT1 * N + i];
+        ddv += c6 * csc * ri * L[L_XT2 * N + i];
+    }
+    if (c.slap) {
+        // thermal diffusion (:959-975, :1221-1228, :1456-1472); the theta second-derivative term
+        // carries 1/r^3 as in the reference (:970-973)
+        const T ks = P.ks[ir], ort = invrho * invt;
+        const T ts = D[D_TS * N + i] * ri;
+        T rs = D[D_RS * N + i];
+        T d2r = L[L_RS2 * N + i];
+        if (c.fluct) {
+            rs += P.dsbar[ir];
+            d2r += P.d2sbar[ir];
+        }
+        dds += cot * ri * ks * ts * ort;
+        dds += ks * ri * ri * ri * L[L_TS2 * N + i] * ort;
+        dds += ks * L[L_PS2 * N + i] * oors * oors * ort;
+        dds += rs * ort * (T(2) * ri + P.dlnks[ir]) * ks + w9 * rs * ort * ks + ks * d2r * x * x * ort;
+    }
+    dv[N + i] += ddw;
+    dv[2 * N + i] += ddv;
+    dv[3 * N + i] += ddu;
+    dv[4 * N + i] += dds;
+}
+
+// ---------------------------------------------------------------------------------------------
 // 5. density wall derivatives from the momentum tendency (rhs_mpi.f90:1553-1564, :1750-1773)
 // ---------------------------------------------------------------------------------------------
 template <typename T>
@@ -468,7 +623,7 @@ __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, W
 //    :2342-2346, :769)
 // ---------------------------------------------------------------------------------------------
 // RK4 stage update fused into the last RHS kernel (step_rk4.f90:392, 418-419, 444-445, 470-471):
-//   stage 1: acc = Dif + k/6   y1 = y0 + (dt/2) k
+//   stage 1: acc = Dif + k/6   y1 = y0 + (dt/2) k    (SLDiffusion; without it acc = k, :387)
 //   stage 2: acc += k/3        y1 = y0 + (dt/2) k
 //   stage 3: acc += k/3        y1 = y0 + dt k
 //   stage 4: acc += k/6        y0 = y0 + dt acc
@@ -501,7 +656,9 @@ __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<
     if constexpr (sizeof(T) == sizeof(double)) {
         const double kk = (double)kk_t;
         double s;
-        if (st == 1) s = (F.dif ? (double)F.dif[j] : 0.0) + kk / 6.0;
+        // stage 1: vars_save = D + k1/6 with SLDiffusion, vars_save = k1 without it
+        // (step_rk4.f90:369-389: k1 then carries weight 1, as in the reference)
+        if (st == 1) s = F.dif ? (double)F.dif[j] + kk / 6.0 : kk;
         else s = F.acc64[j] + kk / (st == 4 ? 6.0 : 3.0);
         const double y = st == 4 ? F.y0[j] + F.dt * s : F.y0[j] + (st == 3 ? F.dt : F.dt / 2.0) * kk;
         const double yb = bound_value<double>(iv, it, ip, ir, y, F.f, c.nth, c.nr, F.sr1_64);
@@ -514,7 +671,7 @@ __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<
     } else {
         const float kk = (float)kk_t;
         if (st < 4) {
-            const float a = st == 1 ? (F.dif ? (float)F.dif[j] : 0.0f) + kk / 6.0f : F.acc32[j] + kk / 3.0f;
+            const float a = st == 1 ? (F.dif ? (float)F.dif[j] + kk / 6.0f : kk) : F.acc32[j] + kk / 3.0f;
             const float h = (float)(st == 3 ? F.dt : 0.5 * F.dt);
             const float y = (float)F.y0f[j] + h * kk;
             F.acc32[j] = a;
@@ -534,7 +691,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const T *st,
                                                    const T *__restrict__ B, const T *__restrict__ dth_rho,
                                                    const T *__restrict__ dr_rho, T *dv, StageFuse<T> F,
-                                                   const float *__restrict__ ext) {
+                                                   const float *__restrict__ ext, const T *__restrict__ L) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.N) return;
     const long N = c.N;
@@ -575,6 +732,18 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
         da0 = V * Br - U * Bp;
         da1 = U * Bt - W * Br;
         da2 = W * Bp - V * Bt;
+        if (c.lap) {
+            // Induction_Terms with Laplacian (rhs_mpi.f90:2303-2322, VectorCurl derivatives.f90
+            // :1894-2104): dA/dt -= etar curl B, ds/dt += (4 pi/c^2) etar |curl B|^2 / (rho T)
+            const T ri = P.rinv[ir], csc = P.csc[it], eta = P.etar[ir];
+            const T jt = ri * L[L_JP1 * N + i] * csc - L[L_JRP * N + i] * ri;
+            const T jp = -L[L_JT2 * N + i] * ri + L[L_JRT * N + i] * ri;
+            const T jr = L[L_JT1 * N + i] * csc * ri - ri * L[L_JP2 * N + i] * csc;
+            da0 -= eta * jt;
+            da1 -= eta * jp;
+            da2 -= eta * jr;
+            ds += (T(kJconst) * eta) * (jt * jt + jp * jp + jr * jr) * invrho / t;
+        }
     }
     T dvv = (F.stage != 0 || ext) ? dv[2 * N + i] : T(0);
     if (ext) {   // learned closure tendency
@@ -754,6 +923,15 @@ static Consts<T> make_consts(const cssx_mhd *m) {
     c.wbar_lo = (T)m->wbar_lo;
     c.wbar_hi = (T)m->wbar_hi;
     c.dT0top = m->h_tbar.empty() ? T(0) : (T)(m->p.T0 - m->h_tbar[m->nr - 1]);
+    c.lap = m->p.laplacian != 0;
+    c.slap = m->p.laplacian || m->p.slaplacian;
+    c.s_lap_lo = c.s_lap_hi = T(0);
+    if (c.slap) {
+        const int nr = m->nr;
+        const double r1 = m->h_r[0], r2 = m->h_r[nr - 1];
+        c.s_lap_lo = (T)(-m->p.lumr1 * kOneOver4Pi / (r1 * r1 * m->h_ks[0] * m->h_dxdr[0]));
+        c.s_lap_hi = (T)(-m->p.lumr2 * kOneOver4Pi / (r2 * r2 * m->h_ks[nr - 1] * m->h_dxdr[nr - 1]));
+    }
     return c;
 }
 
@@ -769,7 +947,7 @@ template <typename T>
 static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const int nr = m->nr, nth = m->nth;
     const std::vector<double> &drbar = m->h_drbar, &dsbar = m->h_dsbar, &hres = m->h_hres;
-    const size_t tot = 16 * (size_t)nr + 10 * (size_t)nth + 2;
+    const size_t tot = 23 * (size_t)nr + 10 * (size_t)nth + 2;
     std::vector<T> h(tot);
     size_t o = 0;
     auto put = [&](size_t n, auto gen) {
@@ -822,6 +1000,14 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
         return (t_hi(i) - t_lo(i)) / (std::cos(t_lo(i)) - std::cos(t_hi(i)));
     });
     const size_t o_vsp = put(nr, [&](size_t i) { return m->h_diff_speed.empty() ? 0.0 : m->h_diff_speed[i + 1]; });
+    auto opt = [](const std::vector<double> &v, size_t i) { return v.empty() ? 0.0 : v[i]; };
+    const size_t o_mur = put(nr, [&](size_t i) { return opt(m->h_mur, i); });
+    const size_t o_dlm = put(nr, [&](size_t i) { return opt(m->h_dlnmu, i); });
+    const size_t o_ks = put(nr, [&](size_t i) { return opt(m->h_ks, i); });
+    const size_t o_dlk = put(nr, [&](size_t i) { return opt(m->h_dlnks, i); });
+    const size_t o_eta = put(nr, [&](size_t i) { return opt(m->h_etar, i); });
+    const size_t o_w9 = put(nr, [&](size_t i) { return opt(m->h_d2xdr2, i) / m->h_dxdr[i]; });
+    const size_t o_d2s = put(nr, [&](size_t i) { return opt(m->h_d2sbar, i); });
     T *d = nullptr;
     if (P.buf) {
         d = P.buf;
@@ -836,13 +1022,15 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     P.dsdr1 = d + o_d1; P.dsdr2 = d + o_d2; P.sr1 = d + o_s1;
     P.rfc = d + o_rfc; P.rv = d + o_rv; P.sph = d + o_sph; P.rfac = d + o_rfac; P.ct = d + o_ct; P.gt = d + o_gt;
     P.vsp = d + o_vsp;
+    P.mur = d + o_mur; P.dlnmu = d + o_dlm; P.ks = d + o_ks; P.dlnks = d + o_dlk; P.etar = d + o_eta;
+    P.w9 = d + o_w9; P.d2sbar = d + o_d2s;
     return CSSX_OK;
 }
 
 // Physical radial derivative (times dx/dr) of a radial profile by the FP64 compact operator
 // with zero wall values: the background part of d/dr in the FP32 fluctuation form.
 static int background_derivative(cssx_mhd *m, const double *prof_dev, int code, double wlo, double whi,
-                                 std::vector<double> &out) {
+                                 std::vector<double> &out, int order = 1) {
     double *d = nullptr;
     if (hipMalloc(&d, sizeof(double) * (m->nr + 2)) != hipSuccess) return CSSX_ENOMEM;
     // prescribed physical wall derivatives, in x units (the sweep multiplies by dx/dr)
@@ -855,14 +1043,15 @@ static int background_derivative(cssx_mhd *m, const double *prof_dev, int code, 
     g.n = m->nr; g.na = 1; g.nb = 1; g.sa = 1; g.sb = 0; g.srow = 1;
     g.g_k = 0; g.g_a = 0; g.g_b = 0; g.phase = CSSX_PHASE_OTHER;
     DerivSpec s;
-    s.order = 1;
+    s.order = order;
     s.in = prof_dev; s.out = d;
     s.glo = d + m->nr; s.ghi = d + m->nr + 1;
     s.lo_kind = s.hi_kind = END_WALL;
     s.bdry = code;
-    s.scale = m->grid.dri;
+    // order 2 stays in x-space (the caller applies dx/dr^2)
+    s.scale = order == 1 ? m->grid.dri : m->grid.dri * m->grid.dri;
     s.neu_mul = 1.0 / m->grid.dri;
-    s.post = m->p64.dxdr;
+    s.post = order == 1 ? m->p64.dxdr : nullptr;
     int rc = run_derivs(&s, 1, g, CSSX_PREC_F64, nullptr);
     out.assign(m->nr, 0.0);
     if (!rc && (hipDeviceSynchronize() != hipSuccess ||
@@ -901,6 +1090,11 @@ static int ensure_scratch(cssx_mhd *m) {
                            hipMalloc(&S.y0f, sizeof(T) * N * m->nv) != hipSuccess))
         return CSSX_ENOMEM;
     if (hipMemset(S.wall, 0, sizeof(T) * W.total) != hipSuccess) return CSSX_EHIP;
+    if (m->p.laplacian || m->p.slaplacian) {
+        const long nw = 2 * ((long)m->nth * m->nphi + (long)m->nr * m->nphi);
+        if (hipMalloc(&S.L, sizeof(T) * N * L_NSLOT) != hipSuccess) return CSSX_ENOMEM;
+        if (hipMalloc(&S.lwall, sizeof(T) * nw) != hipSuccess) return CSSX_ENOMEM;
+    }
     S.ready = true;
     return CSSX_OK;
 }
@@ -938,6 +1132,18 @@ static DerivSpec spec(cssx_mhd *m, int axis, const T *in, T *out, int code, cons
         if (axis == CSSX_AXIS_THETA) { s.g_k = 0; s.g_a = 1; s.g_b = m->nphi; }
         else { s.g_k = 0; s.g_a = 1; s.g_b = m->nth; }
     }
+    return s;
+}
+
+// second-derivative DerivSpec (D2bydr2 / D2bydt2 / D2bydp2): scale the inverse spacing squared,
+// wall derivatives divided by the radial dri on theta too (derivatives.f90:1322, :1338); r
+// results stay in x-space
+template <typename T>
+static DerivSpec spec2(cssx_mhd *m, int axis, const T *in, T *out, int code, const T *wlo, const T *whi) {
+    DerivSpec s = spec<T>(m, axis, in, out, code, wlo, whi);
+    s.order = 2;
+    s.scale = s.scale * s.scale;
+    if (axis != CSSX_AXIS_PHI) s.neu_mul = 1.0 / m->grid.dri;
     return s;
 }
 
@@ -1027,9 +1233,62 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            spec<T>(m, CSSX_AXIS_R, bp, D + D_RBP * N, ibcr[8], nullptr, nullptr, nullptr, P.dxdr)};
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, mag ? 6 : 4, stream))) return rc;
     }
+    // 3b. explicit diffusion: second derivatives (same wall values as the first), the cross-term
+    //     sweeps of first-derivative slots, and curl B
+    T *L = S.L;
+    if (c.slap) {
+        const int nd = c.lap ? 4 : 1;
+        DerivSpec th[4] = {spec2<T>(m, CSSX_AXIS_THETA, s, L + L_TS2 * N, ibct[5], nullptr, nullptr),
+                           spec2<T>(m, CSSX_AXIS_THETA, w, L + L_TW2 * N, ibct[2], nullptr, nullptr),
+                           spec2<T>(m, CSSX_AXIS_THETA, v, L + L_TV2 * N, ibct[3], wall + W.off_tv_lo, wall + W.off_tv_hi),
+                           spec2<T>(m, CSSX_AXIS_THETA, u, L + L_TU2 * N, ibct[4], nullptr, nullptr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, nd, stream))) return rc;
+        DerivSpec ph[4] = {spec2<T>(m, CSSX_AXIS_PHI, s, L + L_PS2 * N, 0, nullptr, nullptr),
+                           spec2<T>(m, CSSX_AXIS_PHI, v, L + L_PV2 * N, 0, nullptr, nullptr),
+                           spec2<T>(m, CSSX_AXIS_PHI, w, L + L_PW2 * N, 0, nullptr, nullptr),
+                           spec2<T>(m, CSSX_AXIS_PHI, u, L + L_PU2 * N, 0, nullptr, nullptr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, nd, stream))) return rc;
+        DerivSpec rr[4] = {spec2<T>(m, CSSX_AXIS_R, s, L + L_RS2 * N, ibcr[5], wall + W.off_rs_lo, wall + W.off_rs_hi),
+                           spec2<T>(m, CSSX_AXIS_R, w, L + L_RW2 * N, ibcr[2], wall + W.off_rw_lo, wall + W.off_rw_hi),
+                           spec2<T>(m, CSSX_AXIS_R, v, L + L_RV2 * N, ibcr[3], wall + W.off_rv_lo, wall + W.off_rv_hi),
+                           spec2<T>(m, CSSX_AXIS_R, u, L + L_RU2 * N, ibcr[4], nullptr, nullptr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, nd, stream))) return rc;
+    }
+    if (c.lap) {
+        const long nrr = (long)m->nth * m->nphi, nt = (long)m->nr * m->nphi;
+        T *lw = S.lwall;
+        hipLaunchKernelGGL(k_lap_walls<T>, dim3(nblocks(std::max(nrr, nt), 256)), dim3(256), 0, stream, c, P,
+                           (const T *)D, lw, m->p.bottom_stress_free);
+        DerivSpec xr[4] = {spec<T>(m, CSSX_AXIS_R, D + D_TW * N, L + L_XR1 * N, ibcr[2], lw, nullptr, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, D + D_PU * N, L + L_XR2 * N, ibcr[4], nullptr, nullptr, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, D + D_TU * N, L + L_XR3 * N, ibcr[4], nullptr, nullptr, nullptr, P.dxdr),
+                           spec<T>(m, CSSX_AXIS_R, D + D_PV * N, L + L_XR4 * N, ibcr[3], nullptr, lw + nrr, nullptr, P.dxdr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, xr, 4, stream))) return rc;
+        DerivSpec xt[2] = {spec<T>(m, CSSX_AXIS_THETA, D + D_PV * N, L + L_XT1 * N, ibct[3], lw + 2 * nrr, lw + 2 * nrr + nt),
+                           spec<T>(m, CSSX_AXIS_THETA, D + D_PW * N, L + L_XT2 * N, ibct[2], nullptr, nullptr)};
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, xt, 2, stream))) return rc;
+        if (m->p.magnetic) {
+            // VectorCurl: theta sweeps with code 1 (Theta_Normal_Field) or 0, whose wall values go
+            // unused; r sweeps with zero wall values (the radial-field walls zero those halos)
+            const int tcode = m->p.theta_normal_field ? 1 : 0;
+            DerivSpec jt[2] = {spec<T>(m, CSSX_AXIS_THETA, bp, L + L_JT1 * N, tcode, nullptr, nullptr, P.sn),
+                               spec<T>(m, CSSX_AXIS_THETA, br, L + L_JT2 * N, tcode, nullptr, nullptr)};
+            if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, jt, 2, stream))) return rc;
+            DerivSpec jp[2] = {spec<T>(m, CSSX_AXIS_PHI, br, L + L_JP1 * N, 0, nullptr, nullptr),
+                               spec<T>(m, CSSX_AXIS_PHI, bt, L + L_JP2 * N, 0, nullptr, nullptr)};
+            if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, jp, 2, stream))) return rc;
+            DerivSpec jr[2] = {spec<T>(m, CSSX_AXIS_R, bt, L + L_JRT * N, ibcr[7], nullptr, nullptr, P.r, P.dxdr),
+                               spec<T>(m, CSSX_AXIS_R, bp, L + L_JRP * N, ibcr[8], nullptr, nullptr, P.r, P.dxdr)};
+            if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, jr, 2, stream))) return rc;
+        }
+    }
     // 4. pointwise momentum / entropy / continuity terms
     CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (m->p.magnetic ? 32.0 : 24.0) * N * sizeof(T),
                hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv));
+    if (c.slap)
+        CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (c.lap ? 46.0 : 12.0) * N * sizeof(T),
+                   hipLaunchKernelGGL(k_rhs_viscous<T>, dim3(nb), dim3(256), 0, stream, c, P, st, (const T *)D,
+                                      (const T *)L, dv));
     // SLD diffusion while the derivative scratch is free (it needs B and the stage-1 state)
     if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream))) return rc;
     // 5. density wall derivatives from the momentum tendency
@@ -1060,7 +1319,7 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                                            : (FF.stage == 1 ? 12.0 + (FF.dif ? 4.0 : 0.0) : FF.stage < 4 ? 16.0 : 24.0) * m->nv;
         CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T) + fb * N,
                    hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N,
-                                      D + 1 * N, dv, FF, ext));
+                                      D + 1 * N, dv, FF, ext, (const T *)L));
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
@@ -1092,6 +1351,16 @@ static SldParams<T> make_sld(const cssx_mhd *m) {
     sp.top_pc = p.top_perf_cond;
     sp.th_normal = p.theta_normal_field;
     sp.th_pc = p.theta_perf_cond;
+    const bool slap = p.laplacian || p.slaplacian;
+    sp.sflux_lo = slap && p.bottom_const_flux;
+    sp.sflux_hi = slap && p.top_const_flux;
+    sp.soff_lo = sp.soff_hi = T(0);
+    if (slap) {
+        const int nr = m->nr;
+        const double r1 = m->h_r[0], r2 = m->h_r[nr - 1], d1 = m->h_dr[0];
+        sp.soff_lo = (T)(d1 * p.lumr1 * kOneOver4Pi / (r1 * r1 * m->h_ks[0] * m->h_dxdr[0]));
+        sp.soff_hi = (T)(d1 * p.lumr2 * kOneOver4Pi / (r2 * r2 * m->h_ks[nr - 1] * m->h_dxdr[nr - 1]));
+    }
     return sp;
 }
 
@@ -1233,6 +1502,10 @@ extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *par
     for (int i = 1; i <= 12; i++)
         if (params->ibcr[i] < 0 || params->ibcr[i] > 4 || params->ibct[i] < 0 || params->ibct[i] > 4)
             return CSSX_EINVAL;
+    // explicit diffusion needs its profiles
+    const bool slap = params->laplacian || params->slaplacian;
+    if (slap && (!pr->ks || !pr->dlnks || !pr->d2xdr2)) return CSSX_EINVAL;
+    if (params->laplacian && (!pr->mur || !pr->dlnmu || !pr->etar)) return CSSX_EINVAL;
     cssx_mhd *m = new (std::nothrow) cssx_mhd;
     if (!m) return CSSX_ENOMEM;
     m->grid = *grid;
@@ -1260,6 +1533,16 @@ extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *par
     m->h_rho_bar.assign(pr->rho_bar, pr->rho_bar + nr);
     m->h_s_bar.assign(pr->s_bar, pr->s_bar + nr);
     if (pr->diffusion_speed) m->h_diff_speed.assign(pr->diffusion_speed, pr->diffusion_speed + nr + 2);
+    if (slap) {
+        m->h_ks.assign(pr->ks, pr->ks + nr);
+        m->h_dlnks.assign(pr->dlnks, pr->dlnks + nr);
+        m->h_d2xdr2.assign(pr->d2xdr2, pr->d2xdr2 + nr);
+    }
+    if (params->laplacian) {
+        m->h_mur.assign(pr->mur, pr->mur + nr);
+        m->h_dlnmu.assign(pr->dlnmu, pr->dlnmu + nr);
+        m->h_etar.assign(pr->etar, pr->etar + nr);
+    }
     m->dth = (m->h_theta[nth - 1] - m->h_theta[0]) / (nth - 1);
     m->dphi = 1.0 / grid->dpi;
     m->h_tbar.resize(nr);
@@ -1276,6 +1559,7 @@ extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *par
     // background's radial momentum balance exactly on Neumann density walls (the density wall
     // rule of Density_Terms_Radial, rhs_mpi.f90:1750-1773, applied to the background)
     if (!rc) rc = background_derivative(m, m->p64.sbar, params->ibcr[5], 0.0, 0.0, m->h_dsbar);
+    if (!rc && slap) rc = background_derivative(m, m->p64.sbar, params->ibcr[5], 0.0, 0.0, m->h_d2sbar, 2);
     if (!rc) {
         const int code = params->ibcr[1];
         auto du_bar = [&](int i) { return -m->h_grav[i] - m->gamm1 * m->h_tbar[i] * m->h_dsbar[i]; };
@@ -1303,7 +1587,8 @@ extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
     for (void *p : {(void *)m->p64.buf, (void *)m->p32.buf, (void *)m->s64.D, (void *)m->s64.B, (void *)m->s64.wall,
                     (void *)m->s64.y1, (void *)m->s64.k, (void *)m->s32.D, (void *)m->s32.B, (void *)m->s32.wall,
                     (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red, (void *)m->s64.dif,
-                    (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f, (void *)m->ext})
+                    (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f, (void *)m->ext, (void *)m->s64.L,
+                    (void *)m->s64.lwall, (void *)m->s32.L, (void *)m->s32.lwall})
         if (p) (void)hipFree(p);
     delete m;
 }
@@ -1421,7 +1706,18 @@ extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, doub
         dtc = std::min(dtc, h[6 * b]);
         dtb = std::min(dtb, h[6 * b + 1]);
     }
-    const double dtd = 1e10;   // no explicit diffusion limit without Laplacian (rhs_mpi.f90:187-191)
+    // explicit diffusion limit (rhs_mpi.f90:194-198)
+    double dtd = 1e10;
+    if (m->p.laplacian) {
+        const int nr = m->nr, nth = m->nth;
+        const double drmin = *std::min_element(m->h_dr.begin(), m->h_dr.end());
+        const double sinmin = std::min(std::sin(m->h_theta[0]), std::sin(m->h_theta[nth - 1]));
+        const double dx = std::min(drmin, m->h_r[0] * std::min(sinmin * m->dphi, m->dth));
+        double dmax = 0.0;
+        for (int i = 0; i < nr; i++)
+            dmax = std::max({dmax, m->h_ks[i], m->h_mur[i] / m->h_rho_bar[i], m->h_etar[i]});
+        dtd = dx * dx / dmax;
+    }
     *dt_out = std::min(std::min(dtb, dtc), dtd) * cfl_safety;
     return std::isfinite(*dt_out) ? CSSX_OK : CSSX_ENAN;
 }

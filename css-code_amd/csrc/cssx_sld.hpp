@@ -28,6 +28,10 @@ struct SldParams {
     int first, last;        // diffused fields, 0-based (varlist of :86-112)
     int dstype, magnetic, visc, mass, ohmic;
     int bot_pc, top_pc, th_normal, th_pc;
+    // entropy ghost of a constant-flux radial wall under explicit thermal diffusion
+    // (diffusion.f90:614-618, :688-692): wall value +- dr_ind(1) lumr/(4 pi r^2 ks dxdr)
+    int sflux_lo, sflux_hi;
+    T soff_lo, soff_hi;
 };
 
 // 1/x: IEEE division in FP64 (the reference's arithmetic), the hardware reciprocal (1 ulp) in
@@ -120,7 +124,13 @@ __device__ T sld_ghost_r(const Consts<T> &c, const Prof<T> &P, const SldParams<T
                          int f, long col, long plane, bool hi) {
     const int j0 = hi ? c.nr - 1 : 0, dj = hi ? -1 : 1;
     switch (f) {
-    case 0: case 4: return T(0);
+    case 0: return T(0);
+    case 4:
+        if (hi ? sp.sflux_hi : sp.sflux_lo) {
+            const T t0 = sld_val(c, P, fp, f, col + (long)j0 * plane, j0);
+            return hi ? t0 - sp.soff_hi : t0 + sp.soff_lo;
+        }
+        return T(0);
     case 1: case 2: {
         const T t0 = sld_val(c, P, fp, f, col + (long)j0 * plane, j0);
         const T t1 = sld_val(c, P, fp, f, col + (long)(j0 + dj) * plane, j0 + dj);

@@ -83,6 +83,7 @@ class Config:
     xi_cs: float = 1.0
     damping_time: float = 1e9
     cfl_safety: float = 0.4
+    SLDiffusion: bool = True        # SLD diffusion folded into RK4 stage 1 (sld_coef 0 gives none)
     sld_coef: float = 0.0
     sld_speedh: float = 1.0
     sld_speedr: float = 0.0
@@ -110,7 +111,7 @@ class Config:
     Theta_Stress_Free: bool = True
     # explicit diffusion (constants.f90:21,66; mhdsph_mpi.f90:349-354): Laplacian = viscosity
     # mur = sld_pr kappa rho_bar, thermal diffusion ks = kappa rho_bar T_bar (rho_bar/rho_bar(r2))^g0,
-    # resistivity etar = ks; SLaplacian = the thermal diffusion alone.  Either replaces SLD.
+    # resistivity etar = ks; SLaplacian = the thermal diffusion alone.
     Laplacian: bool = False
     SLaplacian: bool = False
     kappa_value: float = 1e-5
@@ -285,7 +286,7 @@ class Model:
         for i in range(13):
             p.ibcr[i] = self.ibcr[i]
             p.ibct[i] = self.ibct[i]
-        p.sld_on = 0 if (c.Laplacian or c.SLaplacian) else 1
+        p.sld_on = int(c.SLDiffusion)
         p.sld_coef = c.sld_coef
         p.mass_diffusion, p.magnetic_diffusion = int(c.Mass_Diffusion), int(c.Magnetic_Diffusion)
         p.viscous_heating, p.ohmic_heating = int(c.Do_Viscous_Heating), int(c.Do_Ohmic_Heating)

@@ -88,9 +88,9 @@ int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const
 
 /* Physics constants, switches and boundary codes.  Field names follow the reference's
  * namelist / module variables (constants.f90:17-72, rhs_mpi.f90:10-13, diffusion.f90:8-11).
- * The stratified (not Unstratified), do_radheat=.False. physics is supported, with either SLD
- * diffusion or the explicit Laplacian / SLaplacian diffusion; the flags below are the ones that
- * path reads. */
+ * The stratified (not Unstratified), do_radheat=.False. physics is supported, with SLD
+ * diffusion, the explicit Laplacian / SLaplacian diffusion, or both; the flags below are the
+ * ones that path reads. */
 typedef struct cssx_mhd_params {
     double gamma, Cp;            /* Cv=Cp/gamma, pconst=(gamma-1)*Cp (rhs_mpi.f90:59-73) */
     double omega0;               /* frame rotation rate omega_0 */
@@ -114,7 +114,7 @@ typedef struct cssx_mhd_params {
                                               fluxes (diffusion.f90:628-643, 691-706) */
     /* explicit diffusion (rhs_mpi.f90:844-1474, :2070-2366).  laplacian: viscosity mur, thermal
      * diffusion ks and resistivity etar; slaplacian: the thermal diffusion alone.  Either one
-     * needs the profiles mur..d2xdr2 and excludes sld_on. */
+     * needs the profiles mur..d2xdr2; both combine with sld_on. */
     int laplacian, slaplacian;
     double lumr1, lumr2;         /* luminosities of constant-flux entropy walls under laplacian /
                                     slaplacian: ds/dr = -lumr/(4 pi r^2 ks) (rhs_mpi.f90:586-588,
@@ -162,8 +162,10 @@ int cssx_rhs(cssx_mhd *m, int prec, const void *state, void *rhs, void *b_out, v
 /* Bound (rhs_mpi.f90:2395-2491) applied in place to a state (FP32: fluctuation form). */
 int cssx_bound(cssx_mhd *m, int prec, void *state, void *stream);
 
-/* nsteps classical RK4 steps of Step_RK4 (step_rk4.f90:339-493, SLDiffusion branch) on the
- * FP64 state y0 (nv fields, updated in place).  prec = CSSX_PREC_F64 runs everything in FP64;
+/* nsteps RK4 steps of Step_RK4 (step_rk4.f90:339-493) on the FP64 state y0 (nv fields, updated
+ * in place).  With sld_on the weights are the classical 1/6, 1/3, 1/3, 1/6 plus the Euler SLD
+ * tendency; without it k1 carries weight 1 as in the reference (vars_save = dvars1, :387), so
+ * sld_on with sld_coef = 0 is the plain classical RK4.  prec = CSSX_PREC_F64 runs everything in FP64;
  * CSSX_PREC_F32 evaluates stages, RHS and diffusion in FP32 while the RK4 accumulation and y0
  * stay FP64. */
 int cssx_rk4(cssx_mhd *m, int prec, void *y0, double dt, int nsteps, void *stream);

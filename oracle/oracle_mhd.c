@@ -945,6 +945,7 @@ int orc_diffusion(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_p
     double *vcc, *f, *invrho, *itemp, *tmp, *delue, *work, *work2, *glo, *ghi, *curl;
     double tinyvar[9], dth, dphi;
     int varlist[8], nsld = 0, isl, iv, it, ip, ir, k;
+    const int slap = p->laplacian || p->slaplacian;
     long i;
     setup(m, g, p, pr);
     memset(dv, 0, sizeof(double) * nv * N);
@@ -1026,7 +1027,19 @@ int orc_diffusion(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_p
                 double gm, gp;
                 for (ir = 0; ir < nr; ir++) tmp[ir] = fv[IDX(m, it, ip, ir)];
                 switch (iv) {           /* b1(:,4) and b2(:,1) at the walls (:586-711) */
-                case 1: case 5: gm = 0.0; gp = 0.0; break;
+                case 1: gm = 0.0; gp = 0.0; break;
+                case 5:
+                    /* constant-flux walls under explicit thermal diffusion (:614-618, :688-692;
+                     * dr_ind(1) at both walls) */
+                    gm = (slap && p->bottom_const_flux)
+                             ? tmp[0] + pr->dr[0] * p->lumr1 * ONE_OVER_4PI /
+                                            (pr->r[0] * pr->r[0] * pr->ks[0] * pr->dxdr[0])
+                             : 0.0;
+                    gp = (slap && p->top_const_flux)
+                             ? tmp[nr - 1] - pr->dr[0] * p->lumr2 * ONE_OVER_4PI /
+                                                 (pr->r[nr - 1] * pr->r[nr - 1] * pr->ks[nr - 1] * pr->dxdr[nr - 1])
+                             : 0.0;
+                    break;
                 case 2: case 3:
                     gm = -10.0 / 3.0 * tmp[0] / pr->r[0] + 6.0 * tmp[1] / pr->r[1] - 2.0 * tmp[2] / pr->r[2] +
                          1.0 / 3.0 * tmp[3] / pr->r[3];

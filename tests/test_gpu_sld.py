@@ -49,6 +49,11 @@ VARIANTS = {
                   Top_Perfect_Cond=True, Bottom_Perfect_Cond=True, Theta_Perf_Conductor=True,
                   Theta_Normal_Field=False, Do_Viscous_Heating=False),
     "hydro": dict(sld_coef=0.4, magnetic=False, Top_Stress_Free=False, Bottom_Stress_Free=False),
+    # SLD beside the explicit diffusion: constant-flux entropy ghosts from the wall luminosity
+    # (diffusion.f90:614-618, :688-692)
+    "explicit": dict(sld_coef=0.3, Laplacian=True, kappa_value=1e-3, lumr1=0.3, lumr2=0.2),
+    "explicit_top": dict(sld_coef=0.3, SLaplacian=True, kappa_value=1e-3, lumr1=0.3, lumr2=0.2,
+                         Top_Constant_Temp=False, Top_Constant_Flux=True),
 }
 SHAPES = [(24, 40, 32), (40, 36, 72)]      # (nr, nth, nphi)
 
