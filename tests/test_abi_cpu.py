@@ -196,3 +196,30 @@ def test_contig_pcr_solve_equals_dense(n, order, lo, hi, bdry):
     x = _contig_solve(plan, nl, n, MR, d)
     ref = np.linalg.solve(_dense_lhs(order, n, lo, hi, bdry), d)
     assert np.abs(x - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+def _c_layout(struct, fields, tmp_path):
+    """sizeof and offsetof of a header struct, compiled with the host C compiler."""
+    src = tmp_path / "layout.c"
+    body = "\n".join(f'    printf("%zu\\n", offsetof({struct}, {f}));' for f in fields)
+    src.write_text(f'#include <stddef.h>\n#include <stdio.h>\n#include "{ROOT}/include/cssx.h"\n'
+                   f'int main(void) {{\n    printf("%zu\\n", sizeof({struct}));\n{body}\n    return 0;\n}}\n')
+    exe = tmp_path / "layout"
+    import subprocess
+    subprocess.run(["gcc", "-std=c99", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    return int(out[0]), [int(x) for x in out[1:]]
+
+
+@pytest.mark.parametrize("struct,cls", [("cssx_grid", "Grid"), ("cssx_patch", "Patch"),
+                                        ("cssx_mhd_params", "MhdParams"), ("cssx_mhd_profiles", "MhdProfiles")])
+def test_struct_layouts_match_header(struct, cls, tmp_path):
+    """The ctypes mirrors have the header's size and field offsets (a field added to one side
+    only would shift every later field the library reads)."""
+    from cssx import model
+    C = getattr(_lib, cls, None) or getattr(model, cls)
+    names = [f[0] for f in C._fields_]
+    cnames = ["in" if n == "in_" else n for n in names]
+    size, offs = _c_layout(struct, cnames, tmp_path)
+    assert ctypes.sizeof(C) == size
+    assert [getattr(C, n).offset for n in names] == offs
