@@ -343,6 +343,10 @@ def main():
     ap.add_argument("--sld-coef", type=float, default=0.1,
                     help="SLD diffusion coefficient (SLDiffusion is on; 0 keeps its cost with a zero tendency)")
     ap.add_argument("--paths", default="f64,f32", help="precisions to run: f64,f32 | f32 | f64")
+    ap.add_argument("--laplacian", type=int, default=0,
+                    help="1: the explicit-diffusion physics (Laplacian viscosity, thermal diffusion and "
+                         "resistivity) in place of SLD")
+    ap.add_argument("--kappa", type=float, default=1e-5, help="kappa_value of the explicit diffusion")
     ap.add_argument("--study-steps", type=int, default=100,
                     help="FP32-vs-FP64 precision study length (untimed; errors every 10 steps; 0 = off)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -378,7 +382,9 @@ def main():
         print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}))
         return
     seeds = {"c2": (2, 3), "c4": (5, 6)}[args.config]     # entropy / vector-potential seeds
-    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=args.sld_coef, seed_s=seeds[0], seed_a=seeds[1])
+    lap = bool(args.laplacian)
+    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=0.0 if lap else args.sld_coef, seed_s=seeds[0],
+               seed_a=seeds[1], Laplacian=lap, kappa_value=args.kappa, SLDiffusion=not lap)
     m = Model(c)
     y0 = torch.as_tensor(m.initial_state(), device="cuda").contiguous()
     sol = Solver(m)
@@ -454,7 +460,7 @@ def main():
 
     val32 = npts / (ms32 * 1e-3) / 1e6
     val64 = npts / (ms64 * 1e-3) / 1e6
-    workload = f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field"
+    workload = f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field" + ("_laplacian" if lap else "")
     tr = measured_traffic(workload)
     roof32 = roofline(tab32)
     if roof32 is not None and tr is not None and roof32["kernel"] in tr.get("families", {}):
@@ -475,7 +481,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": workload, "nr": c.nr, "nth": c.nth,
                    "nphi": c.nphi, "fields": c.nv, "accumulate": "f64", "dt": dt, "sld_coef": c.sld_coef,
-                   "parallelism": "single-gpu"},
+                   "parallelism": "single-gpu",
+                   **({"laplacian": True, "kappa_value": c.kappa_value, "sld": False} if lap else {})},
         "fp64": {"value": round(val64, 2), "unit": "Mpt-steps/s", "ms_per_step": round(ms64, 4)},
         "fp32_speedup_vs_fp64": round(ms64 / ms32, 3) if len(paths) == 2 else None,
         "fp32_vs_fp64": cmp,

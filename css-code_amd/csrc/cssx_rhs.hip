@@ -1286,7 +1286,9 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (m->p.magnetic ? 32.0 : 24.0) * N * sizeof(T),
                hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv));
     if (c.slap)
-        CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (c.lap ? 46.0 : 12.0) * N * sizeof(T),
+        // reads 5 state, 11 (2) first-derivative and 18 (3) explicit-diffusion fields, reads
+        // and writes 4 tendencies (Laplacian; SLaplacian alone in brackets)
+        CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (c.lap ? 42.0 : 18.0) * N * sizeof(T),
                    hipLaunchKernelGGL(k_rhs_viscous<T>, dim3(nb), dim3(256), 0, stream, c, P, st, (const T *)D,
                                       (const T *)L, dv));
     // SLD diffusion while the derivative scratch is free (it needs B and the stage-1 state)
