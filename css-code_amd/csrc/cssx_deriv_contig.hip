@@ -20,6 +20,24 @@ __device__ __forceinline__ T shfl(T v, int src) {
     return __shfl(v, src, 64);
 }
 
+// the value of lane-1 (lane 0 gets 0) and of lane+1 (lane 63 gets 0) by DPP wave shifts
+// (wave_shr:1 / wave_shl:1) instead of an LDS permute
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ T from_prev(T v) { return dpp_shift<0x138>(v); }
+template <typename T>
+__device__ __forceinline__ T from_next(T v) { return dpp_shift<0x130>(v); }
+
 // MR consecutive values as 16-byte vector accesses (p 16-byte aligned)
 template <typename T, int MR>
 __device__ __forceinline__ void load_run(const T *__restrict__ p, T (&v)[MR]) {
@@ -39,20 +57,33 @@ __device__ __forceinline__ void store_run(T *__restrict__ p, const T (&v)[MR]) {
     for (int c = 0; c < NC; c++) reinterpret_cast<uint4 *>(p)[c] = t[c];
 }
 
+// the MR coefficients of lane j from a row table staged in 16-byte chunks (see k_deriv_contig)
+template <typename T, int MR>
+__device__ __forceinline__ void lds_rows(const T *tab, int nlT, int j, T (&v)[MR]) {
+    constexpr int E = 16 / (int)sizeof(T);
+#pragma unroll
+    for (int h = 0; h < MR / E; h++) {
+        const uint4 t = *reinterpret_cast<const uint4 *>(tab + ((long)h * nlT + j) * E);
+        __builtin_memcpy(&v[h * E], &t, 16);
+    }
+}
+
 template <typename T, int ORDER, int MR>
 __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     extern __shared__ unsigned char smem_raw[];
     T *sp = reinterpret_cast<T *>(smem_raw);
     const DerivJob<T> &J = L.jobs[blockIdx.z];
     const int n = L.n;
-    // The five row tables are staged transposed, row j*MR+jj of lane j at [jj*nl + j], so the
-    // lanes of a wavefront read consecutive LDS words (no bank conflicts); the interface tables
-    // follow them unchanged.
+    // The five row tables are staged in 16-byte chunks of E = 16/sizeof(T) rows: rows
+    // jj = h*E..h*E+E-1 of lane j sit at [(h*nl + j)*E], so a lane reads E coefficients with one
+    // 16-byte LDS load and the 16 lanes of a load phase read consecutive chunks (no bank
+    // conflicts); the interface tables follow them unchanged.
+    constexpr int E = 16 / (int)sizeof(T);
     const int nlT = (n + MR - 1) / MR, nT = nlT * MR;
     for (int i = threadIdx.x; i < 5 * n; i += blockDim.x) {
         const int c = i / n, r = i - c * n;
         const int jl = r / MR, jr = r - jl * MR;
-        sp[c * nT + jr * nlT + jl] = J.plan[i];
+        sp[c * nT + ((jr / E) * nlT + jl) * E + jr % E] = J.plan[i];
     }
     constexpr int tail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
     for (int i = threadIdx.x; i < tail; i += blockDim.x) sp[5 * nT + i] = J.plan[5 * n + i];
@@ -102,8 +133,8 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     T w[W];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        w[k] = shfl(v[MR - 4 + k], lane - 1);      // rows st-4 .. st-1
-        w[MR + 4 + k] = shfl(v[k], lane + 1);       // rows st+MR .. st+MR+3
+        w[k] = from_prev(v[MR - 4 + k]);     // rows st-4 .. st-1
+        w[MR + 4 + k] = from_next(v[k]);      // rows st+MR .. st+MR+3
     }
 #pragma unroll
     for (int k = 0; k < MR; k++) w[4 + k] = v[k];
@@ -200,11 +231,15 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     const bool has_iface = active && j < nl - 1;
     const int mi = has_iface ? MR - 1 : m;
     const T dI = d[MR - 1];
+    T tf[MR], tl[MR], tu[MR];
+    lds_rows<T, MR>(cf, nlT, j, tf);
+    lds_rows<T, MR>(cl, nlT, j, tl);
+    lds_rows<T, MR>(cu, nlT, j, tu);
     T y = T(0);
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
         if (jj < mi) {
-            y = cf[jj * nlT + j] * d[jj] - cl[jj * nlT + j] * y;
+            y = tf[jj] * d[jj] - tl[jj] * y;
             d[jj] = y;
         }
     }
@@ -212,7 +247,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = MR - 1; jj >= 0; jj--) {
         if (jj < mi) {
-            xn = d[jj] - cu[jj * nlT + j] * xn;
+            xn = d[jj] - tu[jj] * xn;
             d[jj] = xn;
         }
     }
@@ -220,19 +255,23 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     const T xlast = d[MR - 2];   // last interior row of a full segment
 
     // ---- interface system by parallel cyclic reduction -----------------------------------------
-    const T xf_next = shfl(xfirst, lane + 1);
+    const T xf_next = from_next(xfirst);
     T R = T(0);
     if (has_iface) R = dI - rl[j] * xlast - ru[j] * xf_next;
     int steps = 0;
     while ((1 << steps) < nI) steps++;
-    for (int s = 0; s < steps; s++) {
+    if (steps > 0) {
+        const T Rm = from_prev(R), Rp = from_next(R);
+        if (has_iface) R = R - k1[j] * Rm - k2[j] * Rp;
+    }
+    for (int s = 1; s < steps; s++) {
         const int dd = 1 << s;
         const T Rm = shfl(R, lane - dd);
         const T Rp = shfl(R, lane + dd);
         if (has_iface) R = R - k1[s * kMaxLanes + j] * Rm - k2[s * kMaxLanes + j] * Rp;
     }
     const T xI = has_iface ? R * invb[j] : T(0);
-    T xl = shfl(xI, lane - 1);
+    T xl = from_prev(xI);
     if (j == 0) xl = T(0);
     const T xr = xI;
 
@@ -241,11 +280,13 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     T *__restrict__ out = J.out;
     const T *__restrict__ post = J.post;
     const T scale = J.scale, beta = J.beta;
-    T o[MR];
+    T o[MR], tvl[MR], tvr[MR];
+    lds_rows<T, MR>(vL, nlT, j, tvl);
+    lds_rows<T, MR>(vR, nlT, j, tvr);
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
         const int row = min(st + jj, n - 1);
-        T x = (jj < mi) ? d[jj] + vL[jj * nlT + j] * xl + vR[jj * nlT + j] * xr : xr;
+        T x = (jj < mi) ? d[jj] + tvl[jj] * xl + tvr[jj] * xr : xr;
         o[jj] = scale * x;
         if (post) o[jj] *= post[row];
     }
