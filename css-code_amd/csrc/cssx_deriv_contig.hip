@@ -68,7 +68,11 @@ __device__ __forceinline__ void lds_rows(const T *tab, int nlT, int j, T (&v)[MR
     }
 }
 
-template <typename T, int ORDER, int MR>
+// FULL: unit-stride pencils whose length is a multiple of MR.  Every lane then holds MR rows,
+// lane 0 alone touches the lower end and lane nl-1 alone the upper one, so the wall, halo and
+// interface special cases become per-lane tests at compile-time row positions instead of
+// per-row comparisons.
+template <typename T, int ORDER, int MR, bool FULL>
 __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     extern __shared__ unsigned char smem_raw[];
     T *sp = reinterpret_cast<T *>(smem_raw);
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     for (int i = threadIdx.x; i < tail; i += blockDim.x) sp[5 * nT + i] = J.plan[5 * n + i];
     __syncthreads();
     // unit-stride pencils whose length is a multiple of MR move as 16-byte vectors
-    const bool vec = L.srow == 1 && n % MR == 0;
+    constexpr bool vec = FULL;
     const int b = blockIdx.y;
     const T *cf = sp, *cl = cf + nT, *cu = cl + nT, *vL = cu + nT, *vR = vL + nT;
     const T *rl = sp + 5 * nT, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
@@ -127,9 +131,16 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
         for (int k = 0; k < MR; k++) v[k] *= J.pre[min(st + k, n - 1)];
     }
+    if constexpr (FULL) {
+        if (!active) {
 #pragma unroll
-    for (int k = 0; k < MR; k++)
-        if (k >= m) v[k] = T(0);
+            for (int k = 0; k < MR; k++) v[k] = T(0);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < MR; k++)
+            if (k >= m) v[k] = T(0);
+    }
     T w[W];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -140,23 +151,34 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     for (int k = 0; k < MR; k++) w[4 + k] = v[k];
     // rows outside the pencil: ghosts (halo end), the wrapped rows (periodic pencil), or zero
     // (wall end; not read by wall rows)
-#pragma unroll
-    for (int i = 0; i < W; i++) {
-        const int q = st - 4 + i;
-        if (q < 0 || q >= n) {
-            T g = T(0);
-            if (pen_ok && q >= -4 && q < n + 4) {
-                if (q < 0) {
-                    if (J.lo_kind == END_HALO) g = J.glo[gbase + (long)(q + 4) * J.g_k];
-                    else if (J.lo_kind == END_PERIODIC) g = in[pbase + (long)(q + n) * L.srow];
-                } else {
-                    if (J.hi_kind == END_HALO) g = J.ghi[gbase + (long)(q - n) * J.g_k];
-                    else if (J.hi_kind == END_PERIODIC) g = in[pbase + (long)(q - n) * L.srow];
-                }
-                if (J.pre && (q < 0 ? J.lo_kind : J.hi_kind) == END_PERIODIC)
-                    g *= J.pre[q < 0 ? q + n : q - n];
+    auto ghost = [&](int q) {
+        T g = T(0);
+        if (pen_ok && q >= -4 && q < n + 4) {
+            if (q < 0) {
+                if (J.lo_kind == END_HALO) g = J.glo[gbase + (long)(q + 4) * J.g_k];
+                else if (J.lo_kind == END_PERIODIC) g = in[pbase + (long)(q + n) * L.srow];
+            } else {
+                if (J.hi_kind == END_HALO) g = J.ghi[gbase + (long)(q - n) * J.g_k];
+                else if (J.hi_kind == END_PERIODIC) g = in[pbase + (long)(q - n) * L.srow];
             }
-            w[i] = g;
+            if (J.pre && (q < 0 ? J.lo_kind : J.hi_kind) == END_PERIODIC) g *= J.pre[q < 0 ? q + n : q - n];
+        }
+        return g;
+    };
+    if constexpr (FULL) {
+        if (j == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[i] = ghost(i - 4);
+        }
+        if (j == nl - 1) {
+#pragma unroll
+            for (int i = MR + 4; i < W; i++) w[i] = ghost(n + i - MR - 4);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            const int q = st - 4 + i;
+            if (q < 0 || q >= n) w[i] = ghost(q);
         }
     }
     T lo_val = T(0), hi_val = T(0);
@@ -174,7 +196,11 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
         else
             val = Rows<T, 2>::interior(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6]);
         const int row = st + jj;
-        if (row == 0) {
+        const bool r0 = FULL ? (jj == 0 && j == 0) : row == 0;
+        const bool r1 = FULL ? (jj == 1 && j == 0) : row == 1;
+        const bool rn1 = FULL ? (jj == MR - 1 && j == nl - 1) : row == n - 1;
+        const bool rn2 = FULL ? (jj == MR - 2 && j == nl - 1) : row == n - 2;
+        if (r0) {
             if (J.lo_kind == END_WALL) {
                 if constexpr (ORDER == 1) {
                     T r1 = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
@@ -190,11 +216,11 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
                 else
                     val = Rows<T, 2>::halo_lo0(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
             }
-        } else if (row == 1 && J.lo_kind == END_WALL) {
+        } else if (r1 && J.lo_kind == END_WALL) {
             if constexpr (ORDER == 1) val = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
             else val = Rows<T, 2>::wall1(w[4], w[5], w[6], w[7], w[8]);
         }
-        if (row == n - 1) {
+        if (rn1) {
             if (J.hi_kind == END_WALL) {
                 if constexpr (ORDER == 1) {
                     T rn2 = -Rows<T, 1>::wall1(w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj]);
@@ -214,7 +240,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
                     val = Rows<T, 2>::halo_hi0(w[jj + 2], w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7],
                                                w[jj + 8]);
             }
-        } else if (row == n - 2 && J.hi_kind == END_WALL) {
+        } else if (rn2 && J.hi_kind == END_WALL) {
             // row n-2's window sits one row lower: f[n-1] = w[jj+5]
             if constexpr (ORDER == 1) val = -Rows<T, 1>::wall1(w[jj + 5], w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1]);
             else val = Rows<T, 2>::wall1(w[jj + 5], w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1]);
@@ -235,10 +261,13 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     lds_rows<T, MR>(cf, nlT, j, tf);
     lds_rows<T, MR>(cl, nlT, j, tl);
     lds_rows<T, MR>(cu, nlT, j, tu);
+    // rows solved locally: all but an interface row (FULL: the last row of every lane but the
+    // pencil's last)
+    auto local = [&](int jj) { return FULL ? (jj < MR - 1 || !has_iface) : jj < mi; };
     T y = T(0);
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
-        if (jj < mi) {
+        if (local(jj)) {
             y = tf[jj] * d[jj] - tl[jj] * y;
             d[jj] = y;
         }
@@ -246,7 +275,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     T xn = T(0);
 #pragma unroll
     for (int jj = MR - 1; jj >= 0; jj--) {
-        if (jj < mi) {
+        if (local(jj)) {
             xn = d[jj] - tu[jj] * xn;
             d[jj] = xn;
         }
@@ -286,7 +315,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
         const int row = min(st + jj, n - 1);
-        T x = (jj < mi) ? d[jj] + tvl[jj] * xl + tvr[jj] * xr : xr;
+        T x = local(jj) ? d[jj] + tvl[jj] * xl + tvr[jj] * xr : xr;
         o[jj] = scale * x;
         if (post) o[jj] *= post[row];
     }
@@ -317,10 +346,16 @@ hipError_t launch_deriv_contig(const DerivLaunch<T> &L, int order, hipStream_t s
     const unsigned gx = (unsigned)((L.na + (long)ppw * wpb - 1) / ((long)ppw * wpb));
     dim3 grid(gx, L.nb, L.njobs), block(64 * wpb);
     const size_t shm = (size_t)(5 * MR * ((L.n + MR - 1) / MR) + 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes) * sizeof(T);
-#define CSSX_LAUNCH_C(MM)                                                                                \
-    do {                                                                                                 \
-        if (order == 1) hipLaunchKernelGGL((k_deriv_contig<T, 1, MM>), grid, block, shm, stream, L);     \
-        else hipLaunchKernelGGL((k_deriv_contig<T, 2, MM>), grid, block, shm, stream, L);                \
+    const bool full = L.srow == 1 && L.n % MR == 0;
+#define CSSX_LAUNCH_C(MM)                                                                                     \
+    do {                                                                                                      \
+        if (full) {                                                                                           \
+            if (order == 1) hipLaunchKernelGGL((k_deriv_contig<T, 1, MM, true>), grid, block, shm, stream, L);  \
+            else hipLaunchKernelGGL((k_deriv_contig<T, 2, MM, true>), grid, block, shm, stream, L);             \
+        } else {                                                                                              \
+            if (order == 1) hipLaunchKernelGGL((k_deriv_contig<T, 1, MM, false>), grid, block, shm, stream, L); \
+            else hipLaunchKernelGGL((k_deriv_contig<T, 2, MM, false>), grid, block, shm, stream, L);            \
+        }                                                                                                     \
     } while (0)
     switch (MR) {
     case 4: CSSX_LAUNCH_C(4); break;
