@@ -84,11 +84,13 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     // conflicts); the interface tables follow them unchanged.
     constexpr int E = 16 / (int)sizeof(T);
     const int nlT = (n + MR - 1) / MR, nT = nlT * MR;
-    for (int i = threadIdx.x; i < 5 * n; i += blockDim.x) {
-        const int c = i / n, r = i - c * n;
-        const int jl = r / MR, jr = r - jl * MR;
-        sp[c * nT + ((jr / E) * nlT + jl) * E + jr % E] = J.plan[i];
-    }
+    // (one pass per table: MR and E are powers of two, so no division by n)
+#pragma unroll
+    for (int c = 0; c < 5; c++)
+        for (int r = threadIdx.x; r < n; r += blockDim.x) {
+            const int jl = r / MR, jr = r % MR;
+            sp[c * nT + ((jr / E) * nlT + jl) * E + jr % E] = J.plan[c * n + r];
+        }
     constexpr int tail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
     for (int i = threadIdx.x; i < tail; i += blockDim.x) sp[5 * nT + i] = J.plan[5 * n + i];
     __syncthreads();
@@ -293,11 +295,14 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
         const T Rm = from_prev(R), Rp = from_next(R);
         if (has_iface) R = R - k1[j] * Rm - k2[j] * Rp;
     }
-    for (int s = 1; s < steps; s++) {
-        const int dd = 1 << s;
-        const T Rm = shfl(R, lane - dd);
-        const T Rp = shfl(R, lane + dd);
-        if (has_iface) R = R - k1[s * kMaxLanes + j] * Rm - k2[s * kMaxLanes + j] * Rp;
+#pragma unroll
+    for (int s = 1; s < kPcrSteps; s++) {
+        if (s < steps) {   // uniform
+            const int dd = 1 << s;
+            const T Rm = shfl(R, lane - dd);
+            const T Rp = shfl(R, lane + dd);
+            if (has_iface) R = R - k1[s * kMaxLanes + j] * Rm - k2[s * kMaxLanes + j] * Rp;
+        }
     }
     const T xI = has_iface ? R * invb[j] : T(0);
     T xl = from_prev(xI);
