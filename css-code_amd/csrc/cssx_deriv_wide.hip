@@ -29,6 +29,9 @@
 
 namespace cssx {
 
+// capacity of each interface exchange array: 1024 slots plus one padding slot per row
+constexpr int kWideShared = 1024 + 64;
+
 template <typename T, int ORDER, int M, int G>
 __global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
     constexpr int P = 64 / G;    // pencils per workgroup
@@ -46,7 +49,12 @@ __global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
         const int c = i / n, r = i - c * n;
         sp[c * pn + r + r / M] = J.plan[i];
     }
-    T *sh_xf = sp + 5 * pn, *sh_xl = sh_xf + 1024, *sh_dI = sh_xl + 1024, *sh_xI = sh_dI + 1024;
+    // exchange arrays, one padding slot per row so that the lanes of a wavefront, which walk
+    // the other index, hit different banks: [segment][pencil] with stride PS, [pencil][segment]
+    // with stride XS
+    constexpr int PS = P + 1, XS = SM + 1, SHN = kWideShared;
+    static_assert(SM * PS <= SHN && P * XS <= SHN, "exchange arrays");
+    T *sh_xf = sp + 5 * pn, *sh_xl = sh_xf + SHN, *sh_dI = sh_xl + SHN, *sh_xI = sh_dI + SHN;
     const T *cf = sp, *cl = cf + pn, *cu = cl + pn, *vL = cu + pn, *vR = vL + pn;
 
     const int lane = tid & 63;
@@ -176,9 +184,9 @@ __global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
             d[jj] = xn;
         }
     }
-    sh_xf[tid] = d[0];
-    sh_xl[tid] = d[M - 2];   // last interior row of a full segment
-    sh_dI[tid] = dI;
+    sh_xf[s * PS + p] = d[0];
+    sh_xl[s * PS + p] = d[M - 2];   // last interior row of a full segment
+    sh_dI[s * PS + p] = dI;
     __syncthreads();
 
     // ---- 4. interface system: thread t solves interface row q = t mod SM of pencil t / SM by
@@ -191,7 +199,7 @@ __global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
                 *invb = k2 + kPcrSteps * kMaxLanes;
         const int qc = q < kMaxLanes ? q : kMaxLanes - 1;
         T R = T(0);
-        if (iq) R = sh_dI[q * P + pp] - rl[qc] * sh_xl[q * P + pp] - ru[qc] * sh_xf[(q + 1) * P + pp];
+        if (iq) R = sh_dI[q * PS + pp] - rl[qc] * sh_xl[q * PS + pp] - ru[qc] * sh_xf[(q + 1) * PS + pp];
         int steps = 0;
         while ((1 << steps) < nI) steps++;
         for (int k = 0; k < steps; k++) {
@@ -202,11 +210,11 @@ __global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
             // group's (finite) value never enters
             if (iq) R = R - k1[k * kMaxLanes + qc] * Rm - k2[k * kMaxLanes + qc] * Rp;
         }
-        sh_xI[tid] = iq ? R * invb[qc] : T(0);   // sh_xI[pp*SM + q]
+        sh_xI[pp * XS + q] = iq ? R * invb[qc] : T(0);
     }
     __syncthreads();
-    const T xl = (s > 0 && seg_ok) ? sh_xI[p * SM + s - 1] : T(0);
-    const T xr = has_iface ? sh_xI[p * SM + s] : T(0);
+    const T xl = (s > 0 && seg_ok) ? sh_xI[p * XS + s - 1] : T(0);
+    const T xr = has_iface ? sh_xI[p * XS + s] : T(0);
 
     // ---- 5. correct and store ------------------------------------------------------------------
     if (!active) return;
@@ -241,7 +249,7 @@ void wide_shape(int n, bool f64, int &M, int &G) {
 
 size_t wide_lds_bytes(int n, int M, size_t elem) {
     const int nl = (n + M - 1) / M;
-    return ((size_t)5 * (n + nl) + 4 * 1024) * elem;
+    return ((size_t)5 * (n + nl) + 4 * kWideShared) * elem;
 }
 
 template <typename T>
