@@ -51,7 +51,9 @@ __device__ __forceinline__ void hi_closure(const T *w, int jj, const DerivJob<T>
 }
 
 template <typename T, int ORDER, int M>
-__global__ __launch_bounds__(1024) void k_deriv(const DerivLaunch<T> L) {
+// FP32 segments of up to 16 rows fit 8 waves per SIMD (two 1024-thread workgroups per CU) once
+// the compiler is held to that occupancy; the default register split stops at 7 (SGPRs)
+__global__ __launch_bounds__(1024, (sizeof(T) == 4 && M <= 16) ? 8 : 1) void k_deriv(const DerivLaunch<T> L) {
     constexpr int W = M + 8;   // window w[i] = row st-4+i
     const DerivJob<T> &J = L.jobs[blockIdx.z];
     const int lane = threadIdx.x & 63;
