@@ -168,11 +168,13 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
         return g;
     };
     if constexpr (FULL) {
-        if (j == 0) {
+        // a wall end's closure rows read no row beyond the pencil: only halo and periodic ends
+        // need the ghosts
+        if (j == 0 && J.lo_kind != END_WALL) {
 #pragma unroll
             for (int i = 0; i < 4; i++) w[i] = ghost(i - 4);
         }
-        if (j == nl - 1) {
+        if (j == nl - 1 && J.hi_kind != END_WALL) {
 #pragma unroll
             for (int i = MR + 4; i < W; i++) w[i] = ghost(n + i - MR - 4);
         }
