@@ -9,12 +9,14 @@
 //      per axis (one launch per axis, one job per field); with explicit diffusion also the 12
 //      second derivatives, the 6 cross-term sweeps of first-derivative slots and the 6 sweeps
 //      of curl B (Laplacian_Cross_Terms_*, VectorCurl)
-//   4. k_rhs_main: every pointwise term of those three routines in the reference's order;
-//      k_rhs_viscous adds the explicit-diffusion branches
+//   4. main_terms / viscous_terms: every pointwise term of those three routines in the
+//      reference's order, and the explicit-diffusion branches
 //   5. k_walls_density: the density wall derivatives that zero the normal momentum tendency
-//      (rhs_mpi.f90:1553-1564, :1750-1773) - they need step 4's result, hence two phases
-//   6. two density sweeps + k_rhs_final: density terms, top relaxation, u x B (and -etar curl B
-//      with its Ohmic heating), xi_cs
+//      (rhs_mpi.f90:1553-1564, :1750-1773); they need the terms of step 4 at the wall points,
+//      which it evaluates there
+//   6. two density sweeps + k_rhs_final: the terms of step 4 at every point, the density terms,
+//      top relaxation, u x B (and -etar curl B with its Ohmic heating), xi_cs and the fused RK4
+//      stage; no tendency field is written and re-read in between
 // Temperature T = rho^(gamma-1) exp(s/Cv)/((gamma-1)Cv) and 1/rho are recomputed where used
 // instead of being stored (rhs_mpi.f90:377-387).
 #include <hip/hip_runtime.h>
@@ -75,6 +77,7 @@ enum Slot {
     D_TW = 0, D_TV, D_TU, D_TS, D_TBP, D_TBR,                 // theta: w v u s bp br
     D_PRHO, D_PV, D_PW, D_PU, D_PS, D_PBT, D_PBR,             // phi:   rho v w u s bt br
     D_RW, D_RV, D_RU, D_RS, D_RBT, D_RBP,                     // r:     w v u s bt bp
+    D_THRHO, D_RRHO,                                          // density: theta, r
     D_NSLOT
 };
 
@@ -318,15 +321,12 @@ __global__ __launch_bounds__(256) void k_walls_state(Consts<T> c, Prof<T> P, Wal
 // ---------------------------------------------------------------------------------------------
 // 4. pointwise terms of Source_Theta_Terms, Phi_Terms, Radial_Terms (reference order)
 // ---------------------------------------------------------------------------------------------
+// The terms of point i (theta index it, r index ir) in out[0..4] = drho, dw, dv, du, ds.
 template <typename T>
-__global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const T *__restrict__ st,
-                                                  const T *__restrict__ B, const T *__restrict__ D,
-                                                  T *__restrict__ dv) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N) return;
+__device__ __forceinline__ void main_terms(const Consts<T> &c, const Prof<T> &P, const T *__restrict__ st,
+                                           const T *__restrict__ B, const T *__restrict__ D, long i, int it,
+                                           int ir, T (&out)[5]) {
     const long N = c.N;
-    int it, ip, ir;
-    coords(i, c, it, ip, ir);
     const T ri = P.rinv[ir], rv = P.r[ir], epsr = P.eps[ir];
     const T omsq = rv * c.omega0 * c.omega0, om2 = c.om2;
     const T sn = P.sn[it], cs = P.cs[it], cot = P.cot[it];
@@ -446,18 +446,18 @@ __global__ __launch_bounds__(256) void k_rhs_main(Consts<T> c, Prof<T> P, const 
             du_out = xdu - Bp * wk;
         }
     }
-    dv[i] = drho_out;
-    dv[N + i] = dw_out;
-    dv[2 * N + i] = dv_out;
-    dv[3 * N + i] = du_out;
-    dv[4 * N + i] = xds;
+    out[0] = drho_out;
+    out[1] = dw_out;
+    out[2] = dv_out;
+    out[3] = du_out;
+    out[4] = xds;
 }
 
 // ---------------------------------------------------------------------------------------------
 // 4b. explicit diffusion (Laplacian / SLaplacian branches of Source_Theta_Terms, Phi_Terms,
 //     Radial_Terms and Laplacian_Cross_Terms_Radial/_Theta, rhs_mpi.f90:844-1474, :2070-2296).
-//     Every branch adds to the tendencies without reading them, so one kernel adds the lot after
-//     k_rhs_main and before the density wall closure, which must see the viscous momentum.
+//     Every branch adds to the tendencies without reading them, so viscous_terms adds the lot
+//     to main_terms' sum; the density wall closure sees the viscous momentum through it.
 // ---------------------------------------------------------------------------------------------
 // Wall values of the cross-term sweeps.  The reference differentiates r*vars2(k) (r sweeps) and
 // sin*vars2(k) (theta sweeps), vars2 holding the first derivatives over r or r sin; here the
@@ -488,15 +488,12 @@ __global__ __launch_bounds__(256) void k_lap_walls(Consts<T> c, Prof<T> P, const
     }
 }
 
+// adds the explicit-diffusion terms of point i to out[1..4] (dw, dv, du, ds)
 template <typename T>
-__global__ __launch_bounds__(256) void k_rhs_viscous(Consts<T> c, Prof<T> P, const T *__restrict__ st,
-                                                     const T *__restrict__ D, const T *__restrict__ L,
-                                                     T *__restrict__ dv) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N) return;
+__device__ __forceinline__ void viscous_terms(const Consts<T> &c, const Prof<T> &P, const T *__restrict__ st,
+                                              const T *__restrict__ D, const T *__restrict__ L, long i, int it,
+                                              int ir, T (&out)[5]) {
     const long N = c.N;
-    int it, ip, ir;
-    coords(i, c, it, ip, ir);
     const T ri = P.rinv[ir], csc = P.csc[it], cot = P.cot[it];
     const T W = st[N + i], V = st[2 * N + i], U = st[3 * N + i];
     const Thermo<T> th = thermo(st[i], st[4 * N + i], ir, c, P);
@@ -572,10 +569,20 @@ __global__ __launch_bounds__(256) void k_rhs_viscous(Consts<T> c, Prof<T> P, con
         dds += ks * L[L_PS2 * N + i] * oors * oors * ort;
         dds += rs * ort * (T(2) * ri + P.dlnks[ir]) * ks + w9 * rs * ort * ks + ks * d2r * x * x * ort;
     }
-    dv[N + i] += ddw;
-    dv[2 * N + i] += ddv;
-    dv[3 * N + i] += ddu;
-    dv[4 * N + i] += dds;
+    out[1] += ddw;
+    out[2] += ddv;
+    out[3] += ddu;
+    out[4] += dds;
+}
+
+// every RHS term before the density terms at point i: main_terms plus, with explicit
+// diffusion, viscous_terms (rhs_mpi.f90:786-1519, :2070-2296)
+template <typename T>
+__device__ __forceinline__ void pre_density_terms(const Consts<T> &c, const Prof<T> &P, const T *__restrict__ st,
+                                                  const T *__restrict__ B, const T *__restrict__ D,
+                                                  const T *__restrict__ L, long i, int it, int ir, T (&out)[5]) {
+    main_terms(c, P, st, B, D, i, it, ir, out);
+    if (c.slap) viscous_terms(c, P, st, D, L, i, it, ir, out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -583,35 +590,49 @@ __global__ __launch_bounds__(256) void k_rhs_viscous(Consts<T> c, Prof<T> P, con
 // ---------------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, WallPtrs W, T *__restrict__ wall,
-                                                       const T *__restrict__ st, const T *__restrict__ dv,
+                                                       const T *__restrict__ st, const T *__restrict__ B,
+                                                       const T *__restrict__ D, const T *__restrict__ L,
                                                        int lbcr_rho) {
     const long nt = (long)c.nr * c.nphi, nrr = (long)c.nth * c.nphi, N = c.N;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // the momentum tendency before the density terms, evaluated at the wall points themselves
+    T o0[5], o1[5];
     if (i < nt) {
         const int ir = (int)(i / c.nphi), ip = (int)(i % c.nphi);
         const T rv = P.r[ir] * c.invpconst;
         const long q0 = ((long)ir * c.nphi + ip) * c.nth, q1 = q0 + c.nth - 1;
         const Thermo<T> a0 = thermo(st[q0], st[4 * N + q0], ir, c, P), a1 = thermo(st[q1], st[4 * N + q1], ir, c, P);
         const T rho0 = a0.rho, rho1 = a1.rho, t0 = a0.t, t1 = a1.t;
-        wall[W.off_trho_lo + i] = rv * dv[N + q0] * rho0 / t0;
-        wall[W.off_trho_hi + i] = rv * dv[N + q1] * rho1 / t1;
+        pre_density_terms(c, P, st, B, D, L, q0, 0, ir, o0);
+        pre_density_terms(c, P, st, B, D, L, q1, c.nth - 1, ir, o1);
+        wall[W.off_trho_lo + i] = rv * o0[1] * rho0 / t0;
+        wall[W.off_trho_hi + i] = rv * o1[1] * rho1 / t1;
     }
     if (i < nrr) {
         const long q0 = i, q1 = (long)(c.nr - 1) * nrr + i;
+        const int it = (int)(i % c.nth);
         const Thermo<T> a0 = thermo(st[q0], st[4 * N + q0], 0, c, P);
         const Thermo<T> a1 = thermo(st[q1], st[4 * N + q1], c.nr - 1, c, P);
         const T rho0 = a0.rho, rho1 = a1.rho, t0 = a0.t, t1 = a1.t;
-        T lo = (lbcr_rho == 2 || lbcr_rho == 4) ? dv[3 * N + q0] * rho0 * c.invpconst / t0 : rho0;
-        T hi = (lbcr_rho == 3 || lbcr_rho == 4) ? dv[3 * N + q1] * rho1 * c.invpconst / t1 : rho1;
+        const bool nlo = lbcr_rho == 2 || lbcr_rho == 4, nhi = lbcr_rho == 3 || lbcr_rho == 4;
+        T du0 = T(0), du1 = T(0);
+        if (nlo) {
+            pre_density_terms(c, P, st, B, D, L, q0, it, 0, o0);
+            du0 = o0[3];
+        }
+        if (nhi) {
+            pre_density_terms(c, P, st, B, D, L, q1, it, c.nr - 1, o1);
+            du1 = o1[3];
+        }
+        T lo = nlo ? du0 * rho0 * c.invpconst / t0 : rho0;
+        T hi = nhi ? du1 * rho1 * c.invpconst / t1 : rho1;
         if (c.fluct) {
             // the sweep differentiates rho - rho_bar, whose background derivative drbar already
             // carries wbar at these walls: prescribe the departure
             //   (du/dt without density terms + pconst (T_bar/rho_bar) wbar) rho/(pconst T) - wbar
             //   = du_partial (rho/T)/pconst - wbar a/(1+a)      (du_partial: hres-based, small)
-            if (lbcr_rho == 2 || lbcr_rho == 4)
-                lo = dv[3 * N + q0] * (rho0 / t0) * c.invpconst - c.wbar_lo * a0.a / (T(1) + a0.a);
-            if (lbcr_rho == 3 || lbcr_rho == 4)
-                hi = dv[3 * N + q1] * (rho1 / t1) * c.invpconst - c.wbar_hi * a1.a / (T(1) + a1.a);
+            if (nlo) lo = du0 * (rho0 / t0) * c.invpconst - c.wbar_lo * a0.a / (T(1) + a0.a);
+            if (nhi) hi = du1 * (rho1 / t1) * c.invpconst - c.wbar_hi * a1.a / (T(1) + a1.a);
         }
         wall[W.off_rrho_lo + i] = lo / P.dxdr[0];
         wall[W.off_rrho_hi + i] = hi / P.dxdr[c.nr - 1];
@@ -688,10 +709,11 @@ __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const T *st,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 7) void k_rhs_final(Consts<T> c, Prof<T> P, const T *st,
                                                    const T *__restrict__ B, const T *__restrict__ dth_rho,
                                                    const T *__restrict__ dr_rho, T *dv, StageFuse<T> F,
-                                                   const float *__restrict__ ext, const T *__restrict__ L) {
+                                                   const float *__restrict__ ext, const T *__restrict__ D,
+                                                   const T *__restrict__ L) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.N) return;
     const long N = c.N;
@@ -701,7 +723,9 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
     const Thermo<T> th = thermo(st[i], st[4 * N + i], ir, c, P);
     const T Rho = th.rho, t = th.t;
     const T invrho = T(1) / Rho;
-    T drho = dv[i], dw = dv[N + i], du = dv[3 * N + i], ds = dv[4 * N + i];
+    T o[5];
+    pre_density_terms(c, P, st, B, D, L, i, it, ir, o);
+    T drho = o[0], dw = o[1], du = o[3], ds = o[4];
     // Density_Terms_Theta
     T wk = P.rinv[ir] * dth_rho[i];
     drho = drho - W * wk;
@@ -745,7 +769,7 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
             ds += (T(kJconst) * eta) * (jt * jt + jp * jp + jr * jr) * invrho / t;
         }
     }
-    T dvv = (F.stage != 0 || ext) ? dv[2 * N + i] : T(0);
+    T dvv = o[2];
     if (ext) {   // learned closure tendency
         drho += (T)ext[i];
         dw += (T)ext[N + i];
@@ -760,7 +784,7 @@ __global__ __launch_bounds__(256) void k_rhs_final(Consts<T> c, Prof<T> P, const
     }
     if (F.stage == 0) {
         dv[i] = drho;
-        if (ext) dv[2 * N + i] = dvv;
+        dv[2 * N + i] = dvv;
         dv[N + i] = dw;
         dv[3 * N + i] = du;
         dv[4 * N + i] = ds;
@@ -1207,6 +1231,9 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            m->p.bottom_stress_free, m->p.top_stress_free, m->p.bottom_const_flux,
                            m->p.top_const_flux, m->p.theta_normal_field);
     }
+    // SLD diffusion of the stage-1 state (it needs only B and the state) while the derivative
+    // scratch is still free
+    if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream))) return rc;
     // 3. derivative sweeps
     {
         const int mag = m->p.magnetic;
@@ -1282,29 +1309,20 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
             if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, jr, 2, stream))) return rc;
         }
     }
-    // 4. pointwise momentum / entropy / continuity terms
-    CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (m->p.magnetic ? 32.0 : 24.0) * N * sizeof(T),
-               hipLaunchKernelGGL(k_rhs_main<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D, dv));
-    if (c.slap)
-        // reads 5 state, 11 (2) first-derivative and 18 (3) explicit-diffusion fields, reads
-        // and writes 4 tendencies (Laplacian; SLaplacian alone in brackets)
-        CSSX_TIMED(CSSX_PHASE_RHS_MAIN, stream, (c.lap ? 42.0 : 18.0) * N * sizeof(T),
-                   hipLaunchKernelGGL(k_rhs_viscous<T>, dim3(nb), dim3(256), 0, stream, c, P, st, (const T *)D,
-                                      (const T *)L, dv));
-    // SLD diffusion while the derivative scratch is free (it needs B and the stage-1 state)
-    if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream))) return rc;
-    // 5. density wall derivatives from the momentum tendency
+    // 4-5. density wall derivatives from the momentum tendency, which k_walls_density evaluates
+    //      (every term before the density terms) at the wall points themselves
     {
         long nw = std::max((long)m->nr * m->nphi, (long)m->nth * m->nphi);
-        hipLaunchKernelGGL(k_walls_density<T>, dim3(nblocks(nw, 256)), dim3(256), 0, stream, c, P, W, wall, st, dv,
-                           ibcr[1]);
+        hipLaunchKernelGGL(k_walls_density<T>, dim3(nblocks(nw, 256)), dim3(256), 0, stream, c, P, W, wall, st,
+                           (const T *)B, (const T *)D, (const T *)L, ibcr[1]);
     }
-    // 6. density sweeps + final terms (reuse theta slots 0/1)
+    // 6. density sweeps + every pointwise term, the density terms and the stage in one kernel
     {
-        DerivSpec th = spec<T>(m, CSSX_AXIS_THETA, rho, D + 0 * N, ibct[1], wall + W.off_trho_lo, wall + W.off_trho_hi);
+        DerivSpec th = spec<T>(m, CSSX_AXIS_THETA, rho, D + D_THRHO * N, ibct[1], wall + W.off_trho_lo,
+                               wall + W.off_trho_hi);
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, &th, 1, stream))) return rc;
-        DerivSpec rr = spec<T>(m, CSSX_AXIS_R, rho, D + 1 * N, ibcr[1], wall + W.off_rrho_lo, wall + W.off_rrho_hi,
-                               nullptr, P.dxdr);
+        DerivSpec rr = spec<T>(m, CSSX_AXIS_R, rho, D + D_RRHO * N, ibcr[1], wall + W.off_rrho_lo,
+                               wall + W.off_rrho_hi, nullptr, P.dxdr);
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, &rr, 1, stream))) return rc;
         StageFuse<T> F0{};
         const StageFuse<T> &FF = F ? *F : F0;
@@ -1315,13 +1333,18 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
             if (rc) return rc;
             ext = m->ext;
         }
-        // plain: 15/21 values per point; a fused stage adds the stage traffic of every field
-        const double fb = !F ? 0.0
-                          : sizeof(T) == 8 ? (FF.stage < 4 ? 40.0 - 8.0 : 32.0 - 8.0) * m->nv
+        // reads: 5 state, 3 B, 19 (13) first-derivative and 2 density-derivative fields, 18 (3)
+        // explicit-diffusion fields with Laplacian (SLaplacian); the plain RHS writes 8 (5)
+        // tendencies, a fused stage moves the stage traffic of every field instead
+        const bool mag = m->p.magnetic;
+        const double reads = (mag ? 29.0 : 20.0) + (c.lap ? 18.0 : c.slap ? 3.0 : 0.0);
+        const double fb = !F ? (mag ? 8.0 : 5.0) * sizeof(T)
+                          : sizeof(T) == 8 ? (FF.stage == 1 ? 24.0 + (FF.dif ? 8.0 : 0.0) : FF.stage < 4 ? 32.0 : 24.0) * m->nv
                                            : (FF.stage == 1 ? 12.0 + (FF.dif ? 4.0 : 0.0) : FF.stage < 4 ? 16.0 : 24.0) * m->nv;
-        CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, (m->p.magnetic ? 21.0 : 15.0) * N * sizeof(T) + fb * N,
-                   hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, D + 0 * N,
-                                      D + 1 * N, dv, FF, ext, (const T *)L));
+        CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, reads * N * sizeof(T) + fb * N + (ext ? 32.0 * N : 0.0),
+                   hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B,
+                                      (const T *)(D + D_THRHO * N), (const T *)(D + D_RRHO * N), dv, FF, ext,
+                                      (const T *)D, (const T *)L));
     }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
