@@ -423,9 +423,8 @@ def main():
     tab32 = profile_phases(sol, y0, dt, prof_steps, "f32") if "f32" in paths else None
     tab64 = profile_phases(sol, y0, dt, prof_steps, "f64") if "f64" in paths else None
 
-    families = {"k_deriv": ("deriv_r", "deriv_theta", "deriv_phi"), "k_rhs_main": ("rhs_main",),
-                "k_rhs_final": ("rhs_final",), "k_sld": ("sld",), "pointwise_other": ("pointwise",),
-                "rk_stage": ("rk_update",)}
+    families = {"k_deriv": ("deriv_r", "deriv_theta", "deriv_phi"), "k_rhs_final": ("rhs_final",),
+                "k_sld": ("sld",), "pointwise_other": ("pointwise",), "rk_stage": ("rk_update",)}
 
     def family_rows(tab):
         rows = {}

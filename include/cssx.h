@@ -212,8 +212,10 @@ int cssx_conservation(cssx_mhd *m, const void *y0_f64, double *out4, void *strea
 #define CSSX_PHASE_RK_UPDATE 4     /* stage update + Bound */
 #define CSSX_PHASE_OTHER 5
 #define CSSX_PHASE_SLD 6           /* SLD diffusion: speed, fluxes, corrections (not its curl) */
-#define CSSX_PHASE_RHS_MAIN 7      /* the momentum / entropy / continuity term kernel */
-#define CSSX_PHASE_RHS_FINAL 8     /* density terms, u x B, and the fused RK4 stage update */
+#define CSSX_PHASE_RHS_MAIN 7      /* unused: the momentum / entropy / continuity terms are evaluated
+                                      inside the RHS_FINAL kernel */
+#define CSSX_PHASE_RHS_FINAL 8     /* every pointwise RHS term, the density terms, u x B, and the fused
+                                      RK4 stage update */
 #define CSSX_NPHASE 9
 int cssx_timing_enable(int on);
 int cssx_timing_read(double *ms, long *count, double *bytes);
