@@ -40,6 +40,9 @@ struct PatchJob {
 };
 
 constexpr int kPatchTail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
+// capacity of each padded interface exchange array: SM*(P+1) or P*(SM+1) = 1024 + max(SM, P)
+// slots with P = 1024/SM
+constexpr int kPatchShared = 2048;
 
 // first-derivative right-hand side of row `row` with walls at both ends (prescribed wall
 // derivatives zero); w[jj..jj+8] holds rows row-4 .. row+4
@@ -72,7 +75,12 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
         const int c = i / n, r = i - c * n;
         sp[c * pn + r + r / M] = J.plan[i];
     }
-    T *sh_xf = sp + 5 * pn, *sh_xl = sh_xf + 1024, *sh_dI = sh_xl + 1024, *sh_xI = sh_dI + 1024;
+    // exchange arrays with one padding slot per row ([segment][pencil] stride PS, [pencil]
+    // [segment] stride XS), so that the lanes of a wavefront, which walk the other index, hit
+    // different banks
+    const int PS = P + 1, XS = SM + 1;
+    T *sh_xf = sp + 5 * pn, *sh_xl = sh_xf + kPatchShared, *sh_dI = sh_xl + kPatchShared,
+      *sh_xI = sh_dI + kPatchShared;
     const T *cf = sp, *cl = cf + pn, *cu = cl + pn, *vL = cu + pn, *vR = vL + pn;
     const int s = tid / P, p = tid - s * P;
     const long a = (long)ablk * P + p;
@@ -112,9 +120,9 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
             xn = d[jj] - cu[pst + jj] * xn;
             d[jj] = xn;
         }
-    sh_xf[tid] = d[0];
-    sh_xl[tid] = d[M - 2];
-    sh_dI[tid] = dI;
+    sh_xf[s * PS + p] = d[0];
+    sh_xl[s * PS + p] = d[M - 2];
+    sh_dI[s * PS + p] = dI;
     __syncthreads();
     {
         const int q = tid & (SM - 1), pp = tid / SM;
@@ -123,7 +131,7 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
                 *invb = k2 + kPcrSteps * kMaxLanes;
         const int qc = q < kMaxLanes ? q : kMaxLanes - 1;
         T R = T(0);
-        if (iq) R = sh_dI[q * P + pp] - rl[qc] * sh_xl[q * P + pp] - ru[qc] * sh_xf[(q + 1) * P + pp];
+        if (iq) R = sh_dI[q * PS + pp] - rl[qc] * sh_xl[q * PS + pp] - ru[qc] * sh_xf[(q + 1) * PS + pp];
         int steps = 0;
         while ((1 << steps) < nI) steps++;
         for (int k = 0; k < steps; k++) {
@@ -132,11 +140,11 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
             const T Rp = __shfl(R, lane + dd, 64);
             if (iq) R = R - k1[k * kMaxLanes + qc] * Rm - k2[k * kMaxLanes + qc] * Rp;
         }
-        sh_xI[tid] = iq ? R * invb[qc] : T(0);
+        sh_xI[pp * XS + q] = iq ? R * invb[qc] : T(0);
     }
     __syncthreads();
-    const T xl = (s > 0 && seg_ok) ? sh_xI[p * SM + s - 1] : T(0);
-    const T xr = has_iface ? sh_xI[p * SM + s] : T(0);
+    const T xl = (s > 0 && seg_ok) ? sh_xI[p * XS + s - 1] : T(0);
+    const T xr = has_iface ? sh_xI[p * XS + s] : T(0);
     if (active) {
         const __amdgpu_buffer_rsrc_t rout = field_rsrc(J.out + bofs);
 #pragma unroll
@@ -251,7 +259,7 @@ __global__ __launch_bounds__(1024) void k_patch_derivs(const PatchJob<T> *__rest
 // LDS bytes an item of pencil length n needs (the larger of the two schemes)
 inline size_t patch_lds(int nmax, int M, int MR, size_t elem) {
     const int nl = (nmax + M - 1) / M;
-    const size_t a = (size_t)5 * (nmax + nl) + 4 * 1024;
+    const size_t a = (size_t)5 * (nmax + nl) + 4 * kPatchShared;
     const size_t b = (size_t)5 * MR * ((nmax + MR - 1) / MR) + kPatchTail;
     return std::max(a, b) * elem;
 }
