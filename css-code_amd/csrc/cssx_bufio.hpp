@@ -35,6 +35,26 @@ __device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsig
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(cssx_u32x2, v), r, vofs, sofs, 0);
 }
 
+// the value of lane-1 (lane 0 gets 0) and of lane+1 (lane 63 gets 0) by DPP wave shifts
+// (wave_shr:1 / wave_shl:1) instead of an LDS permute.  A source lane that is disabled reads as
+// zero, so a shift must not feed a select on the receiving side (the compiler can turn that
+// select into an exec-masked branch around the shift): mask on the sending side instead.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shift(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_shift(double v) {
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+template <typename T>
+__device__ __forceinline__ T from_prev(T v) { return dpp_shift<0x138>(v); }
+template <typename T>
+__device__ __forceinline__ T from_next(T v) { return dpp_shift<0x130>(v); }
+
 // v from lane l of the wavefront as a wave-uniform value (v_readlane)
 __device__ __forceinline__ float lane_val(float v, int l) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));

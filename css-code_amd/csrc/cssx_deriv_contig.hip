@@ -9,6 +9,7 @@
 // tables are staged in LDS once per workgroup).
 #include <hip/hip_runtime.h>
 
+#include "cssx_bufio.hpp"
 #include "cssx_kernels.hpp"
 #include "cssx_plan.hpp"
 #include "cssx_rows.hpp"
@@ -19,24 +20,6 @@ template <typename T>
 __device__ __forceinline__ T shfl(T v, int src) {
     return __shfl(v, src, 64);
 }
-
-// the value of lane-1 (lane 0 gets 0) and of lane+1 (lane 63 gets 0) by DPP wave shifts
-// (wave_shr:1 / wave_shl:1) instead of an LDS permute
-template <int CTRL>
-__device__ __forceinline__ float dpp_shift(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_shift(double v) {
-    const long long x = __builtin_bit_cast(long long, v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(x >> 32), CTRL, 0xf, 0xf, true);
-    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
-}
-template <typename T>
-__device__ __forceinline__ T from_prev(T v) { return dpp_shift<0x138>(v); }
-template <typename T>
-__device__ __forceinline__ T from_next(T v) { return dpp_shift<0x130>(v); }
 
 // MR consecutive values as 16-byte vector accesses (p 16-byte aligned)
 template <typename T, int MR>

@@ -44,25 +44,41 @@ constexpr int kPatchTail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
 // slots with P = 1024/SM
 constexpr int kPatchShared = 2048;
 
-// first-derivative right-hand side of row `row` with walls at both ends (prescribed wall
-// derivatives zero); w[jj..jj+8] holds rows row-4 .. row+4
-template <typename T>
-__device__ __forceinline__ T patch_row(const T *w, int jj, int row, int n, int bd) {
-    T val = Rows<T, 1>::interior(w[jj + 2], w[jj + 3], w[jj + 5], w[jj + 6]);
-    if (row == 0) {
-        const T r1 = Rows<T, 1>::wall1(w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7], w[jj + 8]);
-        val = Rows<T, 1>::wall0(bd == 1 || bd == 3, T(0), w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7], r1);
-    } else if (row == 1) {
-        val = Rows<T, 1>::wall1(w[jj + 3], w[jj + 4], w[jj + 5], w[jj + 6], w[jj + 7]);
+// A raw buffer resource over bytes [0, bytes) of base: loads outside it read zero and stores
+// outside it are dropped (the range check of raw buffers), which replaces per-row bounds tests.
+// A lane with no pencil takes the offset kNoPencil; a row below zero wraps to >= 2^32 - 4*rowb.
+// The host keeps (n + 20) * rowb <= 2^31, so that neither can come back into range.
+constexpr unsigned kNoPencil = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t span_rsrc(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+
+// First-derivative right-hand side rows st .. st+MS-1 of a pencil of n rows with walls at
+// both ends (prescribed wall derivatives zero): w[i] holds row st-4+i (zero outside the
+// pencil) and end(k) returns row n-1-k.  Interior rows everywhere, then the wall closures in
+// the segment that starts the pencil (rows 0, 1) and in the one or two that end it (rows
+// n-2, n-1).
+template <typename T, int MS, class End>
+__device__ __forceinline__ void patch_rows(T (&d)[MS], const T *w, int st, int n, int bd, End end) {
+#pragma unroll
+    for (int jj = 0; jj < MS; jj++) d[jj] = Rows<T, 1>::interior(w[jj + 2], w[jj + 3], w[jj + 5], w[jj + 6]);
+    if (st == 0) {
+        const T r1 = Rows<T, 1>::wall1(w[4], w[5], w[6], w[7], w[8]);
+        d[1] = r1;
+        d[0] = Rows<T, 1>::wall0(bd == 1 || bd == 3, T(0), w[4], w[5], w[6], w[7], r1);
     }
-    if (row == n - 1) {
-        const T rn2 = -Rows<T, 1>::wall1(w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], w[jj]);
-        const T rn1 = Rows<T, 1>::wall0(bd == 1 || bd == 2, T(0), w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1], -rn2);
-        val = (bd == 1 || bd == 2) ? -rn1 : rn1;
-    } else if (row == n - 2) {
-        val = -Rows<T, 1>::wall1(w[jj + 5], w[jj + 4], w[jj + 3], w[jj + 2], w[jj + 1]);
+    const int jl = n - 1 - st;   // row n-1 at jj = jl
+    if (jl >= 0 && jl <= MS) {
+        const T e0 = end(0), e1 = end(1), e2 = end(2), e3 = end(3), e4 = end(4);
+        const T rn2 = -Rows<T, 1>::wall1(e0, e1, e2, e3, e4);
+        const T rn1 = Rows<T, 1>::wall0(bd == 1 || bd == 2, T(0), e0, e1, e2, e3, -rn2);
+        const T vn1 = (bd == 1 || bd == 2) ? -rn1 : rn1;
+#pragma unroll
+        for (int jj = 0; jj < MS; jj++) {
+            if (jj == jl) d[jj] = vn1;
+            if (jj == jl - 1) d[jj] = rn2;
+        }
     }
-    return val;
 }
 
 template <typename T, int M>
@@ -84,23 +100,23 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
     const T *cf = sp, *cl = cf + pn, *cu = cl + pn, *vL = cu + pn, *vR = vL + pn;
     const int s = tid / P, p = tid - s * P;
     const long a = (long)ablk * P + p;
-    const bool pen_ok = a < J.na, seg_ok = s < nl, active = pen_ok && seg_ok;
+    const bool pen_ok = a < J.na, seg_ok = s < nl;
     const int st = s * M;
     const int m = seg_ok ? min(M, n - st) : 0;
     const long bofs = (long)b * J.sb;
-    const unsigned lofs = (unsigned)((pen_ok ? a : 0) * J.sa) * (unsigned)sizeof(T);
     const unsigned rowb = (unsigned)J.srow * (unsigned)sizeof(T);
-    const __amdgpu_buffer_rsrc_t rin = field_rsrc(J.in + bofs);
+    // the resource spans rows [0, n) of the item's pencils: halo rows past the walls read zero
+    // and stores of rows >= n, of lanes without a pencil or a segment, are dropped
+    const unsigned lofs = pen_ok ? (unsigned)(a * J.sa) * (unsigned)sizeof(T) : kNoPencil;
+    const unsigned span = (unsigned)n * rowb;
+    const int sr = seg_ok ? st : n;
+    const __amdgpu_buffer_rsrc_t rin = span_rsrc(J.in + bofs, span);
+    const unsigned v0 = lofs + (unsigned)(sr - 4) * rowb;
     T w[W];
 #pragma unroll
-    for (int i = 0; i < W; i++) {
-        const int q = st - 4 + i;
-        const int qc = q < 0 ? 0 : (q >= n ? n - 1 : q);
-        w[i] = (q >= 0 && q < n) ? bload<T>(rin, lofs + (unsigned)qc * rowb, 0u) : T(0);
-    }
+    for (int i = 0; i < W; i++) w[i] = bload<T>(rin, v0 + (unsigned)i * rowb, 0u);
     T d[M];
-#pragma unroll
-    for (int jj = 0; jj < M; jj++) d[jj] = patch_row<T>(w, jj, st + jj, n, J.bdry);
+    patch_rows<T, M>(d, w, sr, n, J.bdry, [&](int k) { return bload<T>(rin, lofs + (unsigned)(n - 1 - k) * rowb, 0u); });
     __syncthreads();   // plan staged
     const bool has_iface = seg_ok && s < nI;
     const int mi = has_iface ? M - 1 : m;
@@ -145,14 +161,12 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
     __syncthreads();
     const T xl = (s > 0 && seg_ok) ? sh_xI[p * XS + s - 1] : T(0);
     const T xr = has_iface ? sh_xI[p * XS + s] : T(0);
-    if (active) {
-        const __amdgpu_buffer_rsrc_t rout = field_rsrc(J.out + bofs);
+    const __amdgpu_buffer_rsrc_t rout = span_rsrc(J.out + bofs, span);
+    const unsigned o0 = lofs + (unsigned)sr * rowb;
 #pragma unroll
-        for (int jj = 0; jj < M; jj++)
-            if (jj < m) {
-                const T x = (jj < mi) ? d[jj] + vL[pst + jj] * xl + vR[pst + jj] * xr : xr;
-                bstore(J.scale * x, rout, lofs + (unsigned)(st + jj) * rowb, 0u);
-            }
+    for (int jj = 0; jj < M; jj++) {
+        const T x = (jj < mi) ? d[jj] + vL[pst + jj] * xl + vR[pst + jj] * xr : xr;
+        bstore(J.scale * x, rout, o0 + (unsigned)jj * rowb, 0u);
     }
     __syncthreads();   // LDS reused by the next item
 }
@@ -177,29 +191,36 @@ __device__ __forceinline__ void contig_item(const PatchJob<T> &J, int ablk, int 
     const bool pen_ok = a < J.na, active = pen_ok && j < nl;
     const int st = j * MR;
     const int m = active ? min(MR, n - st) : 0;
-    const long pbase = (pen_ok ? a : 0) * J.sa + (long)b * J.sb;
+    // the resource spans the item's plane of pencils (contiguous rows, srow 1): a run's rows
+    // past n belong to the next pencil and are zeroed below, lanes without a pencil read zero
+    // and their stores are dropped
+    const unsigned pofs = pen_ok ? (unsigned)(a * J.sa) * (unsigned)sizeof(T) : kNoPencil;
+    const unsigned span = (unsigned)(J.na * J.sa) * (unsigned)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rin = span_rsrc(J.in + (long)b * J.sb, span);
+    const unsigned o0 = pofs + (unsigned)st * (unsigned)sizeof(T);
     T v[MR];
 #pragma unroll
-    for (int k = 0; k < MR; k++) v[k] = J.in[pbase + (long)min(st + k, n - 1) * J.srow];
+    for (int k = 0; k < MR; k++) v[k] = bload<T>(rin, o0 + (unsigned)(k * sizeof(T)), 0u);
 #pragma unroll
     for (int k = 0; k < MR; k++)
         if (k >= m) v[k] = T(0);
+    // halo rows from the neighbouring lanes by DPP wave shifts.  The first run's left halo
+    // (rows < 0) and the last run's right halo (rows >= n) must read zero: the lanes that feed
+    // them across a pencil boundary (the last lane of the previous pencil, the first lane of the
+    // next) send zero, and inactive lanes hold zero runs.  The masking sits on the sending side
+    // so that no select follows a shift: the compiler may turn such a select into an exec-masked
+    // branch around the shift, whose source lanes then read as disabled (zero).
     T w[W];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        w[k] = __shfl(v[MR - 4 + k], lane - 1, 64);
-        w[MR + 4 + k] = __shfl(v[k], lane + 1, 64);
+        w[k] = from_prev(j == Lw - 1 ? T(0) : v[MR - 4 + k]);
+        w[MR + 4 + k] = from_next(j == 0 ? T(0) : v[k]);
     }
 #pragma unroll
     for (int k = 0; k < MR; k++) w[4 + k] = v[k];
-#pragma unroll
-    for (int i = 0; i < W; i++) {
-        const int q = st - 4 + i;
-        if (q < 0 || q >= n) w[i] = T(0);
-    }
     T d[MR];
-#pragma unroll
-    for (int jj = 0; jj < MR; jj++) d[jj] = patch_row<T>(w, jj, st + jj, n, J.bdry);
+    patch_rows<T, MR>(d, w, st, n, J.bdry,
+                      [&](int k) { return bload<T>(rin, pofs + (unsigned)((n - 1 - k) * sizeof(T)), 0u); });
     __syncthreads();   // plan staged
     const bool has_iface = active && j < nI;
     const int mi = has_iface ? MR - 1 : m;
@@ -232,13 +253,12 @@ __device__ __forceinline__ void contig_item(const PatchJob<T> &J, int ablk, int 
     const T xI = has_iface ? R * invb[j] : T(0);
     T xl = __shfl(xI, lane - 1, 64);
     if (j == 0) xl = T(0);
-    if (active) {
+    // rows past n (the next pencil's) go to an offset outside the resource
+    const __amdgpu_buffer_rsrc_t rout = span_rsrc(J.out + (long)b * J.sb, span);
 #pragma unroll
-        for (int jj = 0; jj < MR; jj++)
-            if (jj < m) {
-                const T x = (jj < mi) ? d[jj] + vL[jj * nl + j] * xl + vR[jj * nl + j] * xI : xI;
-                J.out[pbase + (long)(st + jj) * J.srow] = J.scale * x;
-            }
+    for (int jj = 0; jj < MR; jj++) {
+        const T x = (jj < mi) ? d[jj] + vL[jj * nl + j] * xl + vR[jj * nl + j] * xI : xI;
+        bstore(J.scale * x, rout, jj < m ? o0 + (unsigned)(jj * sizeof(T)) : kNoPencil, 0u);
     }
     __syncthreads();   // LDS reused by the next item
 }
@@ -365,6 +385,11 @@ int build_variant(cssx_patchset *ps, const cssx_patch *pt, int var) {
             else if (ax == 1) { J.n = q.nth; J.na = q.nphi; J.nb = q.nr; J.sa = sph; J.sb = sr; J.srow = sth; J.scale = (T)q.dti; }
             else { J.n = q.nphi; J.na = q.nth; J.nb = q.nr; J.sa = sth; J.sb = sr; J.srow = sph; J.scale = (T)q.dpi; }
             J.kind = ax == 1 ? 1 : 0;
+            // the kernels' buffer ranges (span_rsrc): offsets up to kNoPencil + (n + 20) rows
+            // stay below 2^32, and contiguous items walk unit-stride rows
+            if (J.kind == 0 && (double)(J.n + 20) * (double)J.srow * sizeof(T) > 2147483648.0) return CSSX_EINVAL;
+            if (J.kind == 1 && (J.srow != 1 || (double)J.na * (double)J.sa * sizeof(T) > 2147483648.0))
+                return CSSX_EINVAL;
             PlanKey key{J.n, 1, END_WALL, END_WALL, q.bdry};
             key.variant = J.kind == 1 ? 1 : 2;
             key.seg = J.kind == 1 ? MR : M;
