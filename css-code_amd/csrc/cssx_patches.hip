@@ -3,7 +3,7 @@
 // faces (the Dbydr/Dbydt/Dbydp closures of derivatives.f90:547-796, :1010-1262 applied on
 // every axis, as Determine_Dtype's dtype 0 would ask); all three first derivatives of every
 // patch are work items of one kernel:
-//   * r and phi (strided) items: a workgroup of 1024 lanes covers P = 1024/SM pencils cut
+//   * r and phi (strided) items: a workgroup of NT lanes covers P = NT/SM pencils cut
 //     into SM >= ceil(n/M) segments of M rows (lane = segment * P + pencil), loads stay
 //     coalesced along theta, the segment solves run in registers and the interface system by
 //     parallel cyclic reduction over SM lanes (the contiguous-kernel plan cut at M);
@@ -36,13 +36,13 @@ struct PatchJob {
     int n, na, nb, lanes;   // pencil length, pencils (a fast, b slow), SM or lanes per pencil
     long sa, sb, srow;
     T scale;
-    int bdry, kind;         // kind 0 strided, 1 contiguous
+    int bdry, kind;         // kind 0 strided (M rows per segment), 2 strided (M2), 1 contiguous
 };
 
 constexpr int kPatchTail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
-// capacity of each padded interface exchange array: SM*(P+1) or P*(SM+1) = 1024 + max(SM, P)
-// slots with P = 1024/SM
-constexpr int kPatchShared = 2048;
+// capacity of each padded interface exchange array: SM*(P+1) or P*(SM+1) = NT + max(SM, P)
+// slots with P = NT/SM, at most 2*NT
+__host__ __device__ constexpr int patch_shared(int nt) { return 2 * nt; }
 
 // A raw buffer resource over bytes [0, bytes) of base: loads outside it read zero and stores
 // outside it are dropped (the range check of raw buffers), which replaces per-row bounds tests.
@@ -81,22 +81,19 @@ __device__ __forceinline__ void patch_rows(T (&d)[MS], const T *w, int st, int n
     }
 }
 
-template <typename T, int M>
+template <typename T, int M, int NT>
 __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int b, T *sp) {
     constexpr int W = M + 8;
-    const int n = J.n, nl = (n + M - 1) / M, nI = nl - 1, SM = J.lanes, P = 1024 / SM;
+    const int n = J.n, nl = (n + M - 1) / M, nI = nl - 1, SM = J.lanes, P = NT / SM;
     const int pn = n + nl;
     const int tid = threadIdx.x, lane = tid & 63;
-    for (int i = tid; i < 5 * n; i += 1024) {
-        const int c = i / n, r = i - c * n;
-        sp[c * pn + r + r / M] = J.plan[i];
-    }
     // exchange arrays with one padding slot per row ([segment][pencil] stride PS, [pencil]
     // [segment] stride XS), so that the lanes of a wavefront, which walk the other index, hit
     // different banks
     const int PS = P + 1, XS = SM + 1;
-    T *sh_xf = sp + 5 * pn, *sh_xl = sh_xf + kPatchShared, *sh_dI = sh_xl + kPatchShared,
-      *sh_xI = sh_dI + kPatchShared;
+    constexpr int XC = patch_shared(NT);
+    T *sh_xf = sp + 5 * pn, *sh_xl = sh_xf + XC, *sh_dI = sh_xl + XC, *sh_xI = sh_dI + XC;
+    T *sh_tail = sh_xI + XC;
     const T *cf = sp, *cl = cf + pn, *cu = cl + pn, *vL = cu + pn, *vR = vL + pn;
     const int s = tid / P, p = tid - s * P;
     const long a = (long)ablk * P + p;
@@ -115,6 +112,13 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
     T w[W];
 #pragma unroll
     for (int i = 0; i < W; i++) w[i] = bload<T>(rin, v0 + (unsigned)i * rowb, 0u);
+    // plan staged in LDS while the rows are in flight: segment tables with one slot of padding
+    // per segment, then the interface-solve tables
+    for (int i = tid; i < 5 * n; i += NT) {
+        const int c = i / n, r = i - c * n;
+        sp[c * pn + r + r / M] = J.plan[i];
+    }
+    for (int i = tid; i < kPatchTail; i += NT) sh_tail[i] = J.plan[5 * n + i];
     T d[M];
     patch_rows<T, M>(d, w, sr, n, J.bdry, [&](int k) { return bload<T>(rin, lofs + (unsigned)(n - 1 - k) * rowb, 0u); });
     __syncthreads();   // plan staged
@@ -143,7 +147,7 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
     {
         const int q = tid & (SM - 1), pp = tid / SM;
         const bool iq = q < nI && (long)ablk * P + pp < J.na;
-        const T *rl = J.plan + 5 * n, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
+        const T *rl = sh_tail, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
                 *invb = k2 + kPcrSteps * kMaxLanes;
         const int qc = q < kMaxLanes ? q : kMaxLanes - 1;
         T R = T(0);
@@ -171,23 +175,17 @@ __device__ __forceinline__ void strided_item(const PatchJob<T> &J, int ablk, int
     __syncthreads();   // LDS reused by the next item
 }
 
-template <typename T, int MR>
+template <typename T, int MR, int NT>
 __device__ __forceinline__ void contig_item(const PatchJob<T> &J, int ablk, int b, T *sp) {
     constexpr int W = MR + 8;
     const int n = J.n, nl = (n + MR - 1) / MR, nI = nl - 1, Lw = J.lanes, nT = nl * MR;
     const int tid = threadIdx.x, lane = tid & 63;
-    for (int i = tid; i < 5 * n; i += 1024) {
-        const int c = i / n, r = i - c * n;
-        const int jl = r / MR, jr = r - jl * MR;
-        sp[c * nT + jr * nl + jl] = J.plan[i];
-    }
-    for (int i = tid; i < kPatchTail; i += 1024) sp[5 * nT + i] = J.plan[5 * n + i];
     const T *cf = sp, *cl = cf + nT, *cu = cl + nT, *vL = cu + nT, *vR = vL + nT;
     const T *rl = sp + 5 * nT, *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
             *invb = k2 + kPcrSteps * kMaxLanes;
     const int j = lane & (Lw - 1);
     const int ppw = 64 / Lw;
-    const long a = ((long)ablk * 16 + (tid >> 6)) * ppw + lane / Lw;
+    const long a = ((long)ablk * (NT / 64) + (tid >> 6)) * ppw + lane / Lw;
     const bool pen_ok = a < J.na, active = pen_ok && j < nl;
     const int st = j * MR;
     const int m = active ? min(MR, n - st) : 0;
@@ -201,6 +199,14 @@ __device__ __forceinline__ void contig_item(const PatchJob<T> &J, int ablk, int 
     T v[MR];
 #pragma unroll
     for (int k = 0; k < MR; k++) v[k] = bload<T>(rin, o0 + (unsigned)(k * sizeof(T)), 0u);
+    // plan staged in LDS (row-major over runs, so lanes read consecutive slots) while the rows
+    // are in flight
+    for (int i = tid; i < 5 * n; i += NT) {
+        const int c = i / n, r = i - c * n;
+        const int jl = r / MR, jr = r - jl * MR;
+        sp[c * nT + jr * nl + jl] = J.plan[i];
+    }
+    for (int i = tid; i < kPatchTail; i += NT) sp[5 * nT + i] = J.plan[5 * n + i];
 #pragma unroll
     for (int k = 0; k < MR; k++)
         if (k >= m) v[k] = T(0);
@@ -263,31 +269,38 @@ __device__ __forceinline__ void contig_item(const PatchJob<T> &J, int ablk, int 
     __syncthreads();   // LDS reused by the next item
 }
 
-template <typename T, int M, int MR>
-__global__ __launch_bounds__(1024) void k_patch_derivs(const PatchJob<T> *__restrict__ jobs,
+// FP32 with runs of 8 rows fits 64 VGPRs: eight waves per SIMD (two 1024-lane or four 512-lane
+// workgroups per CU); the other variants would spill there
+template <typename T, int M, int M2, int MR, int NT>
+__global__ __launch_bounds__(NT, (sizeof(T) == 4 && MR == 8) ? 8 : 1) void k_patch_derivs(const PatchJob<T> *__restrict__ jobs,
                                                        const int4 *__restrict__ items, int nitems) {
     extern __shared__ unsigned char smem_raw[];
     T *sp = reinterpret_cast<T *>(smem_raw);
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int4 w = items[it];
         const PatchJob<T> J = jobs[w.x];
-        if (J.kind == 0) strided_item<T, M>(J, w.y, w.z, sp);
-        else contig_item<T, MR>(J, w.y, w.z, sp);
+        if (J.kind == 0) strided_item<T, M, NT>(J, w.y, w.z, sp);
+        else if (M2 != M && J.kind == 2) strided_item<T, M2, NT>(J, w.y, w.z, sp);
+        else contig_item<T, MR, NT>(J, w.y, w.z, sp);
     }
 }
 
 // LDS bytes an item of pencil length n needs (the larger of the two schemes)
-inline size_t patch_lds(int nmax, int M, int MR, size_t elem) {
-    const int nl = (nmax + M - 1) / M;
-    const size_t a = (size_t)5 * (nmax + nl) + 4 * kPatchShared;
+inline size_t patch_lds(int nmax, int M, int MR, int NT, size_t elem) {
+    const int nl = (nmax + M - 1) / M;   // M: the shorter segment length in use
+    const size_t a = (size_t)5 * (nmax + nl) + 4 * (size_t)patch_shared(NT) + kPatchTail;
     const size_t b = (size_t)5 * MR * ((nmax + MR - 1) / MR) + kPatchTail;
     return std::max(a, b) * elem;
 }
 
-// the (M, MR) variants the autotuner chooses from
+// the variants the autotuner chooses from: strided segment lengths (M, M2: each strided job
+// takes the one that wastes fewer lanes to the power-of-two segment count), contiguous run
+// length MR and workgroup size
 constexpr int kPatchVariants = 4;
-constexpr int kVarM[kPatchVariants] = {16, 8, 16, 8};
-constexpr int kVarMR[kPatchVariants] = {8, 8, 16, 16};
+constexpr int kVarM[kPatchVariants] = {16, 16, 16, 16};
+constexpr int kVarM2[kPatchVariants] = {12, 16, 12, 12};
+constexpr int kVarMR[kPatchVariants] = {8, 8, 16, 8};
+constexpr int kVarNT[kPatchVariants] = {1024, 1024, 1024, 512};
 
 // ---- synthetic patch data: N(0,1) (SplitMix64 + Box-Muller, the generator of cssx/model.py's
 //      normal()) smoothed by [1,2,1]/4 along each axis with the edge value repeated --------------
@@ -367,7 +380,12 @@ unsigned next_pow2(unsigned v) {
 
 template <typename T>
 int build_variant(cssx_patchset *ps, const cssx_patch *pt, int var) {
-    const int M = kVarM[var], MR = kVarMR[var];
+    const int M = kVarM[var], M2 = kVarM2[var], MR = kVarMR[var], NT = kVarNT[var];
+    // lanes a strided job of n rows and na pencils per plane occupies with segments of m rows
+    auto strided_cost = [&](int n, long na, int m) {
+        const long sm = (long)next_pow2((unsigned)((n + m - 1) / m)), p = NT / sm;
+        return (na + p - 1) / p * (long)NT * m;
+    };
     std::vector<PatchJob<T>> jobs;
     std::vector<int4> items;
     const int prec = sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32;
@@ -385,26 +403,29 @@ int build_variant(cssx_patchset *ps, const cssx_patch *pt, int var) {
             else if (ax == 1) { J.n = q.nth; J.na = q.nphi; J.nb = q.nr; J.sa = sph; J.sb = sr; J.srow = sth; J.scale = (T)q.dti; }
             else { J.n = q.nphi; J.na = q.nth; J.nb = q.nr; J.sa = sth; J.sb = sr; J.srow = sph; J.scale = (T)q.dpi; }
             J.kind = ax == 1 ? 1 : 0;
+            const int Mj = (J.kind == 0 && strided_cost(J.n, J.na, M2) < strided_cost(J.n, J.na, M)) ? M2 : M;
+            if (Mj != M) J.kind = 2;
             // the kernels' buffer ranges (span_rsrc): offsets up to kNoPencil + (n + 20) rows
             // stay below 2^32, and contiguous items walk unit-stride rows
-            if (J.kind == 0 && (double)(J.n + 20) * (double)J.srow * sizeof(T) > 2147483648.0) return CSSX_EINVAL;
+            if (J.kind != 1 && (double)(J.n + 20) * (double)J.srow * sizeof(T) > 2147483648.0) return CSSX_EINVAL;
             if (J.kind == 1 && (J.srow != 1 || (double)J.na * (double)J.sa * sizeof(T) > 2147483648.0))
                 return CSSX_EINVAL;
             PlanKey key{J.n, 1, END_WALL, END_WALL, q.bdry};
             key.variant = J.kind == 1 ? 1 : 2;
-            key.seg = J.kind == 1 ? MR : M;
+            key.seg = J.kind == 1 ? MR : Mj;
             const void *plan = nullptr;
             int rc = get_plan(key, prec, &plan);
             if (rc) return rc;
             J.plan = (const T *)plan;
             long per_b;
-            if (J.kind == 0) {
-                J.lanes = (int)next_pow2((unsigned)((J.n + M - 1) / M));
-                per_b = (J.na + (1024 / J.lanes) - 1) / (1024 / J.lanes);
+            if (J.kind != 1) {
+                J.lanes = (int)next_pow2((unsigned)((J.n + Mj - 1) / Mj));
+                per_b = (J.na + (NT / J.lanes) - 1) / (NT / J.lanes);
             } else {
                 J.lanes = (int)next_pow2((unsigned)((J.n + MR - 1) / MR));
                 const int ppw = 64 / J.lanes;
-                per_b = (J.na + 16L * ppw - 1) / (16L * ppw);
+                const long pw = (long)(NT / 64) * ppw;
+                per_b = (J.na + pw - 1) / pw;
             }
             const int jid = (int)jobs.size();
             jobs.push_back(J);
@@ -424,20 +445,24 @@ int build_variant(cssx_patchset *ps, const cssx_patch *pt, int var) {
     return CSSX_OK;
 }
 
-template <typename T, int M, int MR>
+template <typename T, int M, int M2, int MR, int NT>
 hipError_t launch_variant(const cssx_patchset *ps, int var, hipStream_t s) {
-    const size_t shm = patch_lds(ps->nmax, M, MR, sizeof(T));
+    const size_t shm = patch_lds(ps->nmax, std::min(M, M2), MR, NT, sizeof(T));
+    const void *fn = (const void *)k_patch_derivs<T, M, M2, MR, NT>;
     if (shm > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_patch_derivs<T, M, MR>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    int dev = 0, ncu = 256;
+    int dev = 0, ncu = 256, per_cu = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const long g = std::min<long>(ps->nitems[var], (long)ncu * 2);
+    // persistent grid: as many workgroups as can be resident at once, so none waits for another
+    // to drain the item list
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, NT, shm);
+    if (e != hipSuccess) return e;
+    const long g = std::min<long>(ps->nitems[var], (long)ncu * std::max(1, per_cu));
     if (g < 1) return hipSuccess;
-    hipLaunchKernelGGL((k_patch_derivs<T, M, MR>), dim3((unsigned)g), dim3(1024), shm, s,
+    hipLaunchKernelGGL((k_patch_derivs<T, M, M2, MR, NT>), dim3((unsigned)g), dim3(NT), shm, s,
                        (const PatchJob<T> *)ps->jobs[var], (const int4 *)ps->items[var], (int)ps->nitems[var]);
     return hipGetLastError();
 }
@@ -445,10 +470,10 @@ hipError_t launch_variant(const cssx_patchset *ps, int var, hipStream_t s) {
 template <typename T>
 hipError_t launch_any(const cssx_patchset *ps, int var, hipStream_t s) {
     switch (var) {
-    case 0: return launch_variant<T, 16, 8>(ps, var, s);
-    case 1: return launch_variant<T, 8, 8>(ps, var, s);
-    case 2: return launch_variant<T, 16, 16>(ps, var, s);
-    default: return launch_variant<T, 8, 16>(ps, var, s);
+    case 0: return launch_variant<T, 16, 12, 8, 1024>(ps, var, s);
+    case 1: return launch_variant<T, 16, 16, 8, 1024>(ps, var, s);
+    case 2: return launch_variant<T, 16, 12, 16, 1024>(ps, var, s);
+    default: return launch_variant<T, 16, 12, 8, 512>(ps, var, s);
     }
 }
 
