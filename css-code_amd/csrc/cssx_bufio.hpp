@@ -15,6 +15,7 @@ __device__ __forceinline__ T &at_row(const T *row, unsigned byte_ofs) {
 // so a sweep row costs one memory instruction and no address arithmetic per lane.  The
 // resource spans 2^32-1 bytes; the host checks that a launch's offsets fit.
 typedef unsigned int cssx_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int cssx_u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void *base) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, -1, 0x00020000);
 }

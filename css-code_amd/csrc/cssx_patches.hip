@@ -262,9 +262,19 @@ __device__ __forceinline__ void contig_item(const PatchJob<T> &J, int ablk, int 
     // rows past n (the next pencil's) go to an offset outside the resource
     const __amdgpu_buffer_rsrc_t rout = span_rsrc(J.out + (long)b * J.sb, span);
 #pragma unroll
-    for (int jj = 0; jj < MR; jj++) {
-        const T x = (jj < mi) ? d[jj] + vL[jj * nl + j] * xl + vR[jj * nl + j] * xI : xI;
-        bstore(J.scale * x, rout, jj < m ? o0 + (unsigned)(jj * sizeof(T)) : kNoPencil, 0u);
+    for (int jj = 0; jj < MR; jj++) d[jj] = J.scale * ((jj < mi) ? d[jj] + vL[jj * nl + j] * xl + vR[jj * nl + j] * xI : xI);
+    if (m == MR) {
+        // a full run as 16-byte stores (buffer stores need 4-byte alignment only)
+        constexpr int NC = MR * (int)sizeof(T) / 16;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+            cssx_u32x4 q;
+            __builtin_memcpy(&q, &d[c * (16 / (int)sizeof(T))], 16);
+            __builtin_amdgcn_raw_buffer_store_b128(q, rout, o0 + (unsigned)(c * 16), 0u, 0);
+        }
+    } else {
+#pragma unroll
+        for (int jj = 0; jj < MR; jj++) bstore(d[jj], rout, jj < m ? o0 + (unsigned)(jj * sizeof(T)) : kNoPencil, 0u);
     }
     __syncthreads();   // LDS reused by the next item
 }
