@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/cssx.h"
+#include "cssx_bufio.hpp"
 #include "cssx_internal.hpp"
 
 typedef __bf16 cssx_bf16;
@@ -35,9 +36,21 @@ namespace cssx {
 constexpr int kFeat = 216;   // 8 fields x 27 neighbours
 constexpr int kOut = 8;
 
-__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, formula 7.1.26
+// (|error| <= 1.5e-7, below what the BF16 rounding of every activation keeps):
+// erf(x) = 1 - (a1 t + a2 t^2 + a3 t^3 + a4 t^4 + a5 t^5) exp(-x^2), t = 1 / (1 + p x), x >= 0,
+// odd in x.  About a dozen instructions (one v_rcp, one v_exp) against ~30 for erff; training
+// and inference share it, so the network is trained with the activation it runs with.
+__device__ __forceinline__ float exp_neg_sq(float x) { return __builtin_amdgcn_exp2f(-1.4426950408889634f * x * x); }
+__device__ __forceinline__ float erf_as(float x) {
+    const float ax = __builtin_fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * ax);
+    const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    return __builtin_copysignf(1.0f - poly * exp_neg_sq(ax), x);
+}
+__device__ __forceinline__ float gelu(float z) { return 0.5f * z * (1.0f + erf_as(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    return 0.5f * (1.0f + erf_as(z * 0.70710678118654752f)) + z * 0.39894228040143268f * exp_neg_sq(0.70710678118654752f * z);
 }
 
 // ---- box filter -------------------------------------------------------------------------------
@@ -465,6 +478,139 @@ __global__ __launch_bounds__(512) void k_closure_infer(const T *__restrict__ st,
     }
 }
 
+// The same network with the tile's stencil footprint staged per wavefront: when nth is a
+// multiple of 16, a tile's 16 points share (ir, ip) and cover theta rows it0 .. it0+15, so its
+// 27-point stencils read 3 (r) x 3 (phi) x 18 (theta) points of each of the 8 fields.  The wave
+// loads those 1296 values with one batch of independent loads (one memory latency per tile
+// instead of one per K step), converts them to FP32 (background subtracted as above) into its
+// slot of LDS, and the K steps gather their B operands from there; the standardization tables
+// sit in LDS beside the packed weights.
+constexpr int kFoot = 8 * 3 * 3 * 18;   // footprint values per tile
+constexpr int kInferWaves = 8;
+constexpr size_t kInferRowsBytes = kPackBytes + 2 * kFeat * sizeof(float) + (size_t)kInferWaves * kFoot * sizeof(float);
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+__global__ __launch_bounds__(64 * kInferWaves) void k_closure_infer_rows(
+    const T *__restrict__ st, long N, int nr, int nphi, int nth, int fluct, const T *__restrict__ rbar,
+    const T *__restrict__ sbar, const float *__restrict__ xmean, const float *__restrict__ xistd,
+    const float *__restrict__ ymean, const float *__restrict__ ystd, const cssx_bf16 *__restrict__ pk,
+    float *__restrict__ ext) {
+    extern __shared__ uint4 lds4[];
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(pk);
+        for (int i = threadIdx.x; i < (int)(kPackBytes / 16); i += blockDim.x) lds4[i] = src[i];
+    }
+    float *xm = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(lds4) + kPackBytes);
+    float *xs = xm + kFeat;
+    for (int i = threadIdx.x; i < kFeat; i += blockDim.x) {
+        xm[i] = xmean[i];
+        xs[i] = xistd[i];
+    }
+    __syncthreads();
+    const cssx_bf16 *W1T = reinterpret_cast<const cssx_bf16 *>(lds4);
+    const cssx_bf16 *W2P = W1T + kH * kS1, *W3P = W2P + kH * kS2;
+    const float *b1 = reinterpret_cast<const float *>(W1T + kPackBf), *b2 = b1 + kH, *b3 = b2 + kH;
+    const int lane = threadIdx.x & 63, g = lane >> 4, li = lane & 15, wv = threadIdx.x >> 6;
+    float *fp = xs + kFeat + wv * kFoot;
+    // footprint slot of the lane; lanes >= 54 are idle (their loads stay in bounds, unused)
+    const int dp = min(lane / 18, 2), tt = lane - 18 * (lane / 18);
+    const __amdgpu_buffer_rsrc_t rs = field_rsrc(st);
+    const long ntiles = N / 16;
+    const cssx_bf16 zero = (cssx_bf16)0.0f;
+    for (long t = (long)blockIdx.x * kInferWaves + wv; t < ntiles; t += (long)gridDim.x * kInferWaves) {
+        const long p0 = t * 16;
+        const int it0 = (int)(p0 % nth);
+        const long rest = p0 / nth;
+        const int ip = (int)(rest % nphi), ir = (int)(rest / nphi);
+        // footprint: value f = ((e*3 + dr)*3 + dp)*18 + tt of field e at r row ir+dr-1 (clamped),
+        // phi ip+dp-1 (periodic), theta it0+tt-1 (clamped).  Lane l < 54 owns dp = l/18 and
+        // tt = l%18, and pass k = 3e + dr fills slot 54k + l: the lane's offset is fixed per tile
+        // and each pass adds a uniform (scalar) offset, so a load costs no address arithmetic.
+        const int jt = min(max(it0 + tt - 1, 0), nth - 1);
+        int jp = ip + dp - 1;
+        jp = jp < 0 ? jp + nphi : (jp >= nphi ? jp - nphi : jp);
+        const unsigned voff = (unsigned)(((long)jp * nth + jt) * (long)sizeof(T));
+        float v[24];
+#pragma unroll
+        for (int k = 0; k < 24; k++) {
+            const int e = k / 3, dr = k % 3;
+            const int jr = min(max(ir + dr - 1, 0), nr - 1);
+            const T raw = bload<T>(rs, voff, (unsigned)(((long)e * N + (long)jr * nphi * nth) * (long)sizeof(T)));
+            if (!fluct && e == 0) v[k] = (float)((double)raw - (double)rbar[jr]);
+            else if (!fluct && e == 4) v[k] = (float)((double)raw - (double)sbar[jr]);
+            else v[k] = (float)raw;
+        }
+        wave_sync();   // the previous tile's reads of fp are done
+        if (lane < 54) {
+#pragma unroll
+            for (int k = 0; k < 24; k++) fp[54 * k + lane] = v[k];
+        }
+        wave_sync();
+        cssx_f32x4 acc[8];
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++) acc[nb] = cssx_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < kKp1 / 32; ks++) {
+            cssx_bf16x8 b;
+            const int nbh = 4 * ks + g;   // this lane's neighbour; its 8 slots are the 8 fields
+            if (nbh < 27) {
+                const int dr = nbh / 9, dp = (nbh / 3) % 3, dt = nbh % 3;
+                const float *f0 = fp + (dr * 3 + dp) * 18 + li + dt;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int q = e * 27 + nbh;
+                    b[e] = (cssx_bf16)((f0[e * 162] - xm[q]) * xs[q]);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) b[e] = zero;
+            }
+#pragma unroll
+            for (int nb = 0; nb < 8; nb++) {
+                const cssx_bf16x8 a = *reinterpret_cast<const cssx_bf16x8 *>(W1T + (nb * 16 + li) * kS1 + ks * 32 + 8 * g);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[nb], 0, 0, 0);
+            }
+        }
+        cssx_bf16x8 h[4];
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) h[nb >> 1][(nb & 1) * 4 + r] = (cssx_bf16)gelu(acc[nb][r] + b1[nb * 16 + 4 * g + r]);
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++) acc[nb] = cssx_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int nb = 0; nb < 8; nb++) {
+                const cssx_bf16x8 a = *reinterpret_cast<const cssx_bf16x8 *>(W2P + (nb * 16 + li) * kS2 + ks * 32 + 8 * g);
+                acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], acc[nb], 0, 0, 0);
+            }
+#pragma unroll
+        for (int nb = 0; nb < 8; nb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) h[nb >> 1][(nb & 1) * 4 + r] = (cssx_bf16)gelu(acc[nb][r] + b2[nb * 16 + 4 * g + r]);
+        cssx_f32x4 y = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            const cssx_bf16x8 a = *reinterpret_cast<const cssx_bf16x8 *>(W3P + li * kS2 + ks * 32 + 8 * g);
+            y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, h[ks], y, 0, 0, 0);
+        }
+        if (g < 2) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int o = 4 * g + r;
+                ext[(long)o * N + p0 + li] = (y[r] + b3[o]) * ystd[o] + ymean[o];
+            }
+        }
+    }
+}
+
 inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace cssx
@@ -586,10 +732,28 @@ int closure_tendency(cssx_mlp *n, int prec, const void *st, int fluct, const voi
         int dev = 0, ncu = 256;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        const long tiles = (N + 15) / 16;
+        if (nth % 16 == 0 && 8.0 * (double)N * (prec == CSSX_PREC_F64 ? 8.0 : 4.0) < 4294967295.0) {
+            // tiles of 16 points inside one theta row (and 32-bit byte offsets over the 8 fields): the
+            // footprint-staged kernel
+            const void *fr = prec == CSSX_PREC_F64 ? (const void *)k_closure_infer_rows<double>
+                                                   : (const void *)k_closure_infer_rows<float>;
+            if (hipFuncSetAttribute(fr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInferRowsBytes) != hipSuccess)
+                return CSSX_EHIP;
+            const unsigned grid = (unsigned)std::min<long>((long)ncu, (tiles + kInferWaves - 1) / kInferWaves);
+            if (prec == CSSX_PREC_F64)
+                hipLaunchKernelGGL(k_closure_infer_rows<double>, dim3(grid), dim3(64 * kInferWaves), kInferRowsBytes, s,
+                                   (const double *)st, N, nr, nphi, nth, fluct, (const double *)rbar,
+                                   (const double *)sbar, n->xmean, n->xistd, n->ymean, n->ystd, n->pk, ext);
+            else
+                hipLaunchKernelGGL(k_closure_infer_rows<float>, dim3(grid), dim3(64 * kInferWaves), kInferRowsBytes, s,
+                                   (const float *)st, N, nr, nphi, nth, fluct, (const float *)rbar,
+                                   (const float *)sbar, n->xmean, n->xistd, n->ymean, n->ystd, n->pk, ext);
+            return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+        }
         const void *fn = prec == CSSX_PREC_F64 ? (const void *)k_closure_infer<double> : (const void *)k_closure_infer<float>;
         if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPackBytes) != hipSuccess)
             return CSSX_EHIP;
-        const long tiles = (N + 15) / 16;
         const unsigned grid = (unsigned)std::min<long>((long)ncu, (tiles + 7) / 8);
         if (prec == CSSX_PREC_F64)
             hipLaunchKernelGGL(k_closure_infer<double>, dim3(grid), dim3(512), kPackBytes, s, (const double *)st, N, nr,
