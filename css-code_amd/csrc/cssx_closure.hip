@@ -20,7 +20,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <new>
+#include <set>
+#include <utility>
 #include <vector>
 
 #include "../../include/cssx.h"
@@ -720,6 +723,30 @@ int features_impl(cssx_mlp *n, const T *st, long N, int nr, int nphi, int nth, i
 
 namespace cssx {
 
+// The LDS limit of a fused inference kernel is raised once per (kernel, device), and the CU
+// count read once per device: a closure tendency runs once per RHS evaluation, and these host
+// calls would otherwise leave the GPU idle between the weight packing and the inference.
+int fused_setup(const void *fn, size_t lds, int *ncu) {
+    static std::mutex mu;
+    static std::set<std::pair<const void *, int>> raised;
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CSSX_EHIP;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!raised.count({fn, dev})) {
+        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return CSSX_EHIP;
+        raised.insert({fn, dev});
+    }
+    int c = dev >= 0 && dev < 64 ? cus[dev] : 0;
+    if (c <= 0) {
+        if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+        if (dev >= 0 && dev < 64) cus[dev] = c;
+    }
+    *ncu = c;
+    return CSSX_OK;
+}
+
 // closure tendency of a whole state: ext (8 fields, FP32) = network(features(st)), in chunks
 // of the network's row capacity
 int closure_tendency(cssx_mlp *n, int prec, const void *st, int fluct, const void *rbar, const void *sbar, int nr,
@@ -729,17 +756,14 @@ int closure_tendency(cssx_mlp *n, int prec, const void *st, int fluct, const voi
         const long tot = kPackBf + kPackF;
         hipLaunchKernelGGL(k_pack_infer, dim3(nblk(tot)), dim3(256), 0, s, n->W, n->oW1, n->ob1, n->oW2, n->ob2,
                            n->oW3, n->ob3, n->pk);
-        int dev = 0, ncu = 256;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        int ncu = 256;
         const long tiles = (N + 15) / 16;
         if (nth % 16 == 0 && 8.0 * (double)N * (prec == CSSX_PREC_F64 ? 8.0 : 4.0) < 4294967295.0) {
             // tiles of 16 points inside one theta row (and 32-bit byte offsets over the 8 fields): the
             // footprint-staged kernel
             const void *fr = prec == CSSX_PREC_F64 ? (const void *)k_closure_infer_rows<double>
                                                    : (const void *)k_closure_infer_rows<float>;
-            if (hipFuncSetAttribute(fr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kInferRowsBytes) != hipSuccess)
-                return CSSX_EHIP;
+            if (int rc = fused_setup(fr, kInferRowsBytes, &ncu)) return rc;
             const unsigned grid = (unsigned)std::min<long>((long)ncu, (tiles + kInferWaves - 1) / kInferWaves);
             if (prec == CSSX_PREC_F64)
                 hipLaunchKernelGGL(k_closure_infer_rows<double>, dim3(grid), dim3(64 * kInferWaves), kInferRowsBytes, s,
@@ -752,8 +776,7 @@ int closure_tendency(cssx_mlp *n, int prec, const void *st, int fluct, const voi
             return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
         }
         const void *fn = prec == CSSX_PREC_F64 ? (const void *)k_closure_infer<double> : (const void *)k_closure_infer<float>;
-        if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPackBytes) != hipSuccess)
-            return CSSX_EHIP;
+        if (int rc = fused_setup(fn, kPackBytes, &ncu)) return rc;
         const unsigned grid = (unsigned)std::min<long>((long)ncu, (tiles + 7) / 8);
         if (prec == CSSX_PREC_F64)
             hipLaunchKernelGGL(k_closure_infer<double>, dim3(grid), dim3(512), kPackBytes, s, (const double *)st, N, nr,

@@ -150,8 +150,14 @@ def box_filter(x: torch.Tensor, factor: int) -> torch.Tensor:
 
 
 def _background(model: Model, dtype, device="cuda"):
-    return (torch.as_tensor(model.rho_bar, dtype=dtype, device=device).contiguous(),
-            torch.as_tensor(model.s_bar, dtype=dtype, device=device).contiguous())
+    """rho_bar, s_bar on the device, uploaded once per (model, dtype, device): a closure
+    tendency runs once per RHS evaluation, and a host-to-device copy per call would stall it."""
+    cache = model.__dict__.setdefault("_closure_background", {})
+    key = (dtype, str(device))
+    if key not in cache:
+        cache[key] = (torch.as_tensor(model.rho_bar, dtype=dtype, device=device).contiguous(),
+                      torch.as_tensor(model.s_bar, dtype=dtype, device=device).contiguous())
+    return cache[key]
 
 
 def features(state: torch.Tensor, model: Model, fluct: bool = False, net: Network | None = None,
