@@ -153,22 +153,25 @@ def test_box_filter_and_features_match_oracle(cl, prec):
     net.close()
 
 
+@pytest.mark.parametrize("nth", [16, 20])
 @pytest.mark.parametrize("precision,width", [("f64", 128), ("f32", 128), ("f64", 64), ("f32", 64)])
-def test_attached_closure_adds_the_network_tendency(cl, precision, width):
+def test_attached_closure_adds_the_network_tendency(cl, precision, width, nth):
     """RHS with the closure attached = RHS + y_std * net(features(state)) + y_mean.  Width 128
     runs the fused register-chained inference kernel, checked against the layer-by-layer GEMM
     path (the same BF16 rounding points; a hidden activation may round the other way, so the
-    bound is on the mean and a loose max); width 64 runs the layer-by-layer path itself."""
+    bound is on the mean and a loose max); width 64 runs the layer-by-layer path itself.  nth 16
+    takes the footprint-staged kernel (tiles of 16 theta points in one row), nth 20 the
+    per-K-step gather kernel (tiles that cross rows)."""
     from cssx import _lib
     from cssx.mhd import Solver
     from cssx.model import Config, Model
-    print(f"attached closure {precision} width {width}", flush=True)
-    m = Model(Config(nr=16, nth=16, nphi=32, sld_coef=0.1))
+    print(f"attached closure {precision} width {width} nth {nth}", flush=True)
+    m = Model(Config(nr=16, nth=nth, nphi=32, sld_coef=0.1))
     sol = Solver(m)
     y = torch.as_tensor(m.initial_state(), device="cuda")
     st = y if precision == "f64" else sol.to_f32(y)
     rng = np.random.default_rng(9)
-    net = cl.Network(width, width, max_rows=2048, seed=21)     # several chunks over the 8192 points
+    net = cl.Network(width, width, max_rows=2048, seed=21)     # several chunks over the points
     xm, xs = rng.standard_normal(216) * 1e-3, rng.uniform(1e-3, 1e-2, 216)
     ym, ys = rng.standard_normal(8) * 1e-2, rng.uniform(0.1, 1.0, 8)
     net.set_norm(xm, xs, ym, ys)
