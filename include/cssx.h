@@ -244,8 +244,10 @@ typedef struct cssx_patch {
     double dri, dti, dpi;
 } cssx_patch;
 typedef struct cssx_patchset cssx_patchset;
-/* builds every plan and the work-item tables of all (M, MR) variants on the current device;
- * the patch pointers are fixed from here on */
+/* builds every plan and the work-item tables of the four kernel variants (strided segment
+ * lengths, contiguous run length, workgroup size) on the current device; the patch pointers
+ * are fixed from here on.  CSSX_EINVAL also when a strided axis's (n + 20) rows exceed 2^31
+ * bytes (the kernels' 32-bit buffer offsets) */
 int cssx_patchset_create(const cssx_patch *patches, int npatch, int prec, cssx_patchset **out);
 void cssx_patchset_destroy(cssx_patchset *ps);
 /* one persistent launch over every item; variant -1 = the autotuned (or default) one */
