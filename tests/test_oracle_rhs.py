@@ -1,7 +1,7 @@
 """Pins the CPU oracle's MHD right-hand side, Bound and RK4 stepping.
 
-The reference has no test for RHS/RK4 (SURVEY: nothing pins them), so the oracle is pinned by
-what the mathematics fixes:
+The reference's own tests hold nothing for RHS/RK4 (no fixture or known answer pins them), so
+the oracle is pinned by what the mathematics fixes:
   * manufactured solution (tests/mms.py): the interior RHS converges to the continuous
     equations evaluated with sympy's spherical operators, and every physical term is far
     above the residual, so a dropped term or a flipped sign fails;
