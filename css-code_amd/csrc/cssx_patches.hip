@@ -6,11 +6,13 @@
 //   * r and phi (strided) items: a workgroup of NT lanes covers P = NT/SM pencils cut
 //     into SM >= ceil(n/M) segments of M rows (lane = segment * P + pencil), loads stay
 //     coalesced along theta, the segment solves run in registers and the interface system by
-//     parallel cyclic reduction over SM lanes (the contiguous-kernel plan cut at M);
+//     parallel cyclic reduction over SM lanes (the contiguous-kernel plan cut at M); each
+//     strided job takes M or M2 rows per segment, whichever wastes fewer lanes;
 //   * theta (contiguous) items: lanes walk the pencil in runs of MR rows (the contiguous
-//     kernel's scheme) with the halo rows exchanged by shuffles.
-// Workgroups loop over the item table (persistent), staging each item's plan in LDS.  (M, MR)
-// is a template choice that cssx_patchset_autotune times on the actual patch set.
+//     kernel's scheme) with the halo rows exchanged by DPP wave shifts.
+// Workgroups loop over the item table (persistent, as many as can be resident), staging each
+// item's plan in LDS.  (M, M2, MR, NT) are template choices; cssx_patchset_autotune times the
+// instantiated variants on the actual patch set.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
