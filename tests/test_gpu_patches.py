@@ -43,7 +43,7 @@ def _oracle_derivs(f, bdry):
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 3e-5)])
-@pytest.mark.parametrize("bdry", [1, 0, 3])
+@pytest.mark.parametrize("bdry", [1, 0, 3, 2, 4])
 def test_patch_derivatives_match_oracle(prec, tol, bdry):
     from cssx.patches import PatchChunk
     print(f"patches {prec} bdry={bdry}", flush=True)
