@@ -112,10 +112,15 @@ def ensemble_sld(ra: np.ndarray, base: float = 0.1, ra0: float = 1e5) -> np.ndar
 
 class Ensemble:
     """`members` small MHD subdomains of one shape, Rayleigh numbers log-uniform in [ra_lo, ra_hi]
-    (SplitMix64 seed), each with its own Solver, state and CFL step."""
+    (SplitMix64 seed), each with its own Solver, state and CFL step.
+
+    A member's sweeps launch fewer workgroups than the GPU has CUs (a 48x96x96 r sweep: 144), so
+    the members are dealt round-robin over `streams` side streams (default priority) that run
+    concurrently; each member owns its scratch, so members on different streams share nothing
+    they write."""
 
     def __init__(self, members: int = 64, shape=(48, 96, 96), seed: int = 4, ra_lo: float = 1e4,
-                 ra_hi: float = 1e6, precision: str = "f32"):
+                 ra_hi: float = 1e6, precision: str = "f32", streams: int = 4):
         nr, nth, nphi = shape
         u = uniform01(seed * 7919 + 1, members)
         self.ra = np.exp(np.log(ra_lo) + u * (np.log(ra_hi) - np.log(ra_lo)))
@@ -132,10 +137,20 @@ class Ensemble:
             self.dts.append(sol.cfl(y))
         self.points = members * nr * nth * nphi
         self.graph = None
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
 
     def step(self, n: int = 1):
-        for sol, y, dt in zip(self.solvers, self.states, self.dts):
-            sol.rk4(y, dt, n, self.precision)
+        """n RK4 steps of every member: forked from the current stream onto the side streams and
+        joined back, so work queued before and after stays ordered (and a graph capture records
+        the members as parallel branches)."""
+        cur = torch.cuda.current_stream()
+        for s in self.streams:
+            s.wait_stream(cur)
+        for k, (sol, y, dt) in enumerate(zip(self.solvers, self.states, self.dts)):
+            with torch.cuda.stream(self.streams[k % len(self.streams)]):
+                sol.rk4(y, dt, n, self.precision)
+        for s in self.streams:
+            cur.wait_stream(s)
 
     def capture(self):
         """Capture one ensemble step (every member's RK4 step) in a HIP graph."""

@@ -73,6 +73,8 @@ def test_patch_derivatives_match_oracle(prec, tol, bdry):
 
 
 def test_ensemble_graph_replays_match_eager():
+    """Graph replays of one ensemble step (members on concurrent side streams) and eager
+    multi-stream steps, each bit-identical to stepping every member on one stream."""
     from cssx.patches import Ensemble
     print("ensemble graph", flush=True)
     ens = Ensemble(members=3, shape=(16, 16, 32), seed=4)
@@ -86,4 +88,11 @@ def test_ensemble_graph_replays_match_eager():
         torch.cuda.synchronize()
         for a, b in zip(ens.states, ref):
             assert torch.equal(a, b), k
+    # eager stepping with the members spread over the side streams, against one stream
+    ens.step(2)
+    for sol, y, dt in zip(ens.solvers, ref, ens.dts):
+        sol.rk4(y, dt, 2, ens.precision)
+    torch.cuda.synchronize()
+    for a, b in zip(ens.states, ref):
+        assert torch.equal(a, b)
     ens.close()
