@@ -133,6 +133,11 @@ struct cssx_mhd {
     int nred = 0;
     cssx_mlp *closure = nullptr;   // attached learned closure (not owned) or null
     float *ext = nullptr;          // its tendency (8 fields, FP32)
+    // the closure tendency runs on a side stream (non-blocking, default priority), forked from
+    // the RHS's stream at its start and joined before the final kernel, so the network's
+    // matrix-core work overlaps the memory-bound derivative sweeps
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 namespace cssx {
@@ -1221,6 +1226,16 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     T *wall = S.wall;
     const T *rho = st, *w = st + N, *v = st + 2 * N, *u = st + 3 * N, *s = st + 4 * N;
     const unsigned nb = nblocks(N, 256);
+    // 0. the learned closure's tendency of the stage state, on the side stream (it reads only
+    //    the state, which nothing here writes before the final kernel)
+    if (m->closure) {
+        if (hipEventRecord(m->ev_fork, stream) != hipSuccess || hipStreamWaitEvent(m->side, m->ev_fork, 0) != hipSuccess)
+            return CSSX_EHIP;
+        if ((rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, c.fluct, P.rbar,
+                                   P.sbar, m->nr, m->nphi, m->nth, m->ext, m->side)))
+            return rc;
+        if (hipEventRecord(m->ev_join, m->side) != hipSuccess) return CSSX_EHIP;
+    }
     // 1. B = curl A
     if (m->p.magnetic && (rc = curl_impl<T>(m, st + 5 * N, B, stream))) return rc;
     const T *bt = B, *bp = B + N, *br = B + 2 * N;
@@ -1327,10 +1342,8 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         StageFuse<T> F0{};
         const StageFuse<T> &FF = F ? *F : F0;
         const float *ext = nullptr;
-        if (m->closure) {
-            rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, c.fluct, P.rbar,
-                                  P.sbar, m->nr, m->nphi, m->nth, m->ext, stream);
-            if (rc) return rc;
+        if (m->closure) {   // join the closure's side stream
+            if (hipStreamWaitEvent(stream, m->ev_join, 0) != hipSuccess) return CSSX_EHIP;
             ext = m->ext;
         }
         // reads: 5 state, 3 B, 19 (13) first-derivative and 2 density-derivative fields, 18 (3)
@@ -1615,6 +1628,12 @@ extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
                     (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f, (void *)m->ext, (void *)m->s64.L,
                     (void *)m->s64.lwall, (void *)m->s32.L, (void *)m->s32.lwall})
         if (p) (void)hipFree(p);
+    if (m->side) {
+        (void)hipStreamSynchronize(m->side);
+        (void)hipStreamDestroy(m->side);
+    }
+    if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
+    if (m->ev_join) (void)hipEventDestroy(m->ev_join);
     delete m;
 }
 
@@ -1623,6 +1642,11 @@ extern "C" int cssx_mhd_set_closure(cssx_mhd *m, cssx_mlp *net) {
     if (net && m->nv != 8) return CSSX_EINVAL;
     if (net && !m->ext && hipMalloc((void **)&m->ext, (size_t)8 * m->N * sizeof(float)) != hipSuccess)
         return CSSX_ENOMEM;
+    if (net && !m->side &&
+        (hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming) != hipSuccess))
+        return CSSX_EHIP;
     m->closure = net;
     return CSSX_OK;
 }
