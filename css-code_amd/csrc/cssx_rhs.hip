@@ -1227,8 +1227,16 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     const T *rho = st, *w = st + N, *v = st + 2 * N, *u = st + 3 * N, *s = st + 4 * N;
     const unsigned nb = nblocks(N, 256);
     // 0. the learned closure's tendency of the stage state, on the side stream (it reads only
-    //    the state, which nothing here writes before the final kernel)
+    //    the state, which nothing here writes before the final kernel).  While the caller's
+    //    stream is being captured into a graph the closure stays on that stream: the fork has
+    //    not been exercised under capture.
+    bool fork_closure = false;
     if (m->closure) {
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(stream, &cap) != hipSuccess) return CSSX_EHIP;
+        fork_closure = cap == hipStreamCaptureStatusNone;
+    }
+    if (fork_closure) {
         if (hipEventRecord(m->ev_fork, stream) != hipSuccess || hipStreamWaitEvent(m->side, m->ev_fork, 0) != hipSuccess)
             return CSSX_EHIP;
         if ((rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, c.fluct, P.rbar,
@@ -1342,8 +1350,13 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         StageFuse<T> F0{};
         const StageFuse<T> &FF = F ? *F : F0;
         const float *ext = nullptr;
-        if (m->closure) {   // join the closure's side stream
+        if (fork_closure) {   // join the closure's side stream
             if (hipStreamWaitEvent(stream, m->ev_join, 0) != hipSuccess) return CSSX_EHIP;
+            ext = m->ext;
+        } else if (m->closure) {
+            rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, c.fluct, P.rbar,
+                                  P.sbar, m->nr, m->nphi, m->nth, m->ext, stream);
+            if (rc) return rc;
             ext = m->ext;
         }
         // reads: 5 state, 3 B, 19 (13) first-derivative and 2 density-derivative fields, 18 (3)
