@@ -280,14 +280,123 @@ extern "C" int cssx_deriv(cssx_ctx *ctx, int axis, int order, int bdry, int prec
     return run_derivs(&s, 1, g, prec, (hipStream_t)stream);
 }
 
+// Pencils of 5..7 points (too short for the segmented solve, whose end closures need 8 rows):
+// one thread per pencil assembles the rows of derivatives.py:37-160 and runs the Thomas
+// recurrence of linalg.tri_solver (its=0, linalg.py:93-113) in registers.  A singular system
+// (n = 5 with Dirichlet-type closures at both walls) divides by a zero pivot and gives +-inf,
+// as the reference does.
+template <int N>
+__global__ __launch_bounds__(256) void k_compact_small(long npen, double dri, int ibc, int dtype,
+                                                       const double *__restrict__ arr, const double *__restrict__ b1,
+                                                       const double *__restrict__ b2, double *darr,
+                                                       double *__restrict__ res) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npen) return;
+    double a[N], d[N], lo[N], up[N];
+    for (int i = 0; i < N; i++) {
+        a[i] = arr[i * npen + p];
+        d[i] = darr ? darr[i * npen + p] : 0.0;
+        lo[i] = up[i] = 1.0 / 3.0;
+    }
+    double g1[4] = {0, 0, 0, 0}, g2[4] = {0, 0, 0, 0};
+    const bool lo_halo = dtype == 1 || dtype == 3, hi_halo = dtype == 2 || dtype == 3;
+    for (int k = 0; k < 4; k++) {
+        if (lo_halo) g1[k] = b1[k * npen + p];
+        if (hi_halo) g2[k] = b2[k * npen + p];
+    }
+    const double dli = ibc == 5 ? (double)N : (double)(N - 1);
+    const double alpha = 1.0 / 3.0, alpha2 = 3.0 / 4.0, gamma2 = 1.0 / 8.0, alpha1 = 6.0, beta1 = 3.0;
+    const double cst = 1.0 / (alpha2 - beta1 * gamma2), up1 = cst * (alpha1 * alpha2 - beta1);
+    double c1 = dli * 7.0 / 9.0, c2 = dli / 36.0;
+    if (lo_halo) {
+        d[0] = c1 * (a[1] - g1[3]) + c2 * (a[2] - g1[2]);
+        d[1] = c1 * (a[2] - a[0]) + c2 * (a[3] - g1[3]);
+    }
+    if (hi_halo) {
+        d[N - 2] = c1 * (a[N - 1] - a[N - 3]) + c2 * (g2[0] - a[N - 4]);
+        d[N - 1] = c1 * (g2[0] - a[N - 2]) + c2 * (g2[1] - a[N - 3]);
+    }
+    for (int i = 2; i <= N - 3; i++) d[i] = c1 * (a[i + 1] - a[i - 1]) + c2 * (a[i + 2] - a[i - 2]);
+    // the 6th-order halo row, times the Python's extra dli/60 (derivatives.py:88-91, :137-141)
+    auto halo_lo = [&]() {
+        d[0] += -alpha * (-g1[0] + 9.0 * g1[1] - 45.0 * g1[2] + 45.0 * a[0] - 9.0 * a[1] + a[2]);
+        d[0] *= dli / 60.0;
+    };
+    auto halo_hi = [&]() {
+        d[N - 1] += -alpha * (-a[N - 3] + 9.0 * a[N - 2] - 45.0 * a[N - 1] + 45.0 * g2[1] - 9.0 * g2[2] + g2[3]);
+        d[N - 1] *= dli / 60.0;
+    };
+    if (dtype < 3) {
+        const double e1 = -43.0 / 96 * dli, e2 = -5.0 / 6.0 * dli, e3 = 9.0 / 8.0 * dli, e4 = 1.0 / 6.0 * dli,
+                     e5 = -1.0 / 96.0 * dli;
+        const double f1 = -10.0 / 3.0 * dli, f2 = -3.0 * dli, f3 = 6.0 * dli, f4 = 1.0 / 3.0 * dli;
+        if (dtype == 1 || dtype == 0) {   // upper wall
+            d[N - 2] = -(e1 * a[N - 1] + e2 * a[N - 2] + e3 * a[N - 3] + e4 * a[N - 4] + e5 * a[N - 5]);
+            if (ibc == 0) d[N - 1] = 0.0;
+            if (ibc == 1 || ibc == 2) d[N - 1] = -(f1 * a[N - 1] + f2 * a[N - 2] + f3 * a[N - 3] + f4 * a[N - 4]);
+            if (dtype == 1) halo_lo();
+            if (ibc == 1 || ibc == 2) d[N - 1] = cst * (alpha2 * d[N - 1] - beta1 * d[N - 2]);
+            lo[N - 2] = alpha2;
+            up[N - 2] = gamma2;
+            lo[N - 1] = (ibc == 1 || ibc == 2) ? up1 : 0.0;
+        }
+        if (dtype == 2 || dtype == 0) {   // lower wall
+            d[1] = e1 * a[0] + e2 * a[1] + e3 * a[2] + e4 * a[3] + e5 * a[4];
+            if (ibc == 0) d[0] = 0.0;
+            if (ibc == 1 || ibc == 3) d[0] = f1 * a[0] + f2 * a[1] + f3 * a[2] + f4 * a[3];
+            if (dtype == 2) halo_hi();
+            const bool odd = (ibc % 2) != 0;
+            if (odd) d[0] = cst * (alpha2 * d[0] - beta1 * d[1]);
+            up[0] = odd ? up1 : 0.0;
+            up[1] = alpha2;
+            lo[1] = gamma2;
+        }
+    } else {
+        halo_lo();
+        halo_hi();
+    }
+    lo[0] = 0.0;
+    up[N - 1] = 0.0;
+    if (darr)
+        for (int i = 0; i < N; i++) darr[i * npen + p] = d[i];
+    // Thomas (unit diagonal)
+    double cp[N], dp[N];
+    cp[0] = up[0];
+    dp[0] = d[0];
+    for (int i = 1; i <= N - 2; i++) {
+        const double piv = 1.0 - cp[i - 1] * lo[i];
+        cp[i] = up[i] / piv;
+        dp[i] = (d[i] - dp[i - 1] * lo[i]) / piv;
+    }
+    dp[N - 1] = (d[N - 1] - dp[N - 2] * lo[N - 1]) / (1.0 - cp[N - 2] * lo[N - 1]);
+    double x = dp[N - 1];
+    res[(N - 1) * npen + p] = x * dri;
+    for (int i = N - 2; i >= 0; i--) {
+        x = dp[i] - cp[i] * x;
+        res[i * npen + p] = x * dri;
+    }
+}
+
 extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
                                 const double *b1, const double *b2, double *darr, double *result,
                                 void *stream) {
-    if (n < 8 || npencils < 1 || !arr || !result || ibc < 0 || ibc > 5 || dtype < 0 || dtype > 3 ||
-        choose_M(n) < 0 || npencils > (long)INT32_MAX)
+    if (n < 5 || npencils < 1 || !arr || !result || ibc < 0 || ibc > 5 || dtype < 0 || dtype > 3 ||
+        npencils > (long)INT32_MAX)
         return CSSX_EINVAL;
     const bool lo_halo = (dtype == 1 || dtype == 3), hi_halo = (dtype == 2 || dtype == 3);
     if ((lo_halo && !b1) || (hi_halo && !b2)) return CSSX_EINVAL;
+    if (n < 8) {
+        const dim3 grid((unsigned)((npencils + 255) / 256));
+        hipStream_t st = (hipStream_t)stream;
+        if (n == 5)
+            hipLaunchKernelGGL(k_compact_small<5>, grid, dim3(256), 0, st, npencils, dri, ibc, dtype, arr, b1, b2, darr, result);
+        else if (n == 6)
+            hipLaunchKernelGGL(k_compact_small<6>, grid, dim3(256), 0, st, npencils, dri, ibc, dtype, arr, b1, b2, darr, result);
+        else
+            hipLaunchKernelGGL(k_compact_small<7>, grid, dim3(256), 0, st, npencils, dri, ibc, dtype, arr, b1, b2, darr, result);
+        return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+    }
+    if (choose_M(n) < 0) return CSSX_EINVAL;
     const double dli = (ibc == 5) ? (double)n : (double)(n - 1);
     // ibc 5: dli = n; at an upper wall it is Neumann with the preloaded value (not 1/2,
     // derivatives.py:82-115); at a lower wall it keeps the Dirichlet-type tridiagonal row

@@ -60,8 +60,8 @@ def compact_fd6(dri, arr, b1, b2, ibc, dtype=0, darr=None, axis=-1):
     rest = tuple(a_m.shape[1:])
     npen = int(np.prod(rest)) if rest else 1
     a2 = a_m.reshape(n, npen).contiguous()
-    if n < 8:
-        raise ValueError(f"compact_fd6: pencil length {n} < 8 is not supported")
+    if n < 5:
+        raise ValueError(f"compact_fd6: pencil length {n} < 5 (the rows reach arr[4] and arr[n-5])")
     dtype = int(dtype)
     ibc = int(ibc)
 

@@ -78,7 +78,10 @@ int cssx_deriv(cssx_ctx *ctx, int axis, int order, int bdry, int prec, const voi
  * per pencil at b[k*npencils + p] (used for dtype 1/3 and 2/3; may be NULL otherwise).
  * darr (may be NULL): on entry rows 0 / n-1 carry the Neumann values f'/dri as in the Python;
  * on exit it holds the assembled right-hand side, as the Python leaves it.
- * result = the derivative (solution * dri), double precision. */
+ * result = the derivative (solution * dri), double precision.
+ * n >= 5, as in the Python, whose rows reach arr[4] and arr[n-5]; a pencil whose tridiagonal
+ * system is singular (n = 5 with Dirichlet-type closures at both walls) returns +-inf, as
+ * the reference's Thomas solve does. */
 int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
                      const double *b1, const double *b2, double *darr, double *result, void *stream);
 

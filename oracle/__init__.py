@@ -51,6 +51,8 @@ def lib():
         L.orc_diffusion.argtypes = [_vp, _vp, _vp, _dp, _dp, _dp]
         L.orc_rk2_tvd.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_int]
         L.orc_rk3_tvd.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, ctypes.c_int]
+        L.orc_step_size.argtypes = [_vp, _vp, _vp, _dp, ctypes.c_double, _dp]
+        L.orc_lap_cross.argtypes = [_vp, _vp, _vp, _dp, _dp]
     return _lib
 
 
@@ -192,3 +194,34 @@ def rk4(model, state, dt, nsteps, scheme="rk4"):
     if rc:
         raise ValueError(f"{scheme} rc={rc}")
     return st
+
+
+def step_size(model, state, cfl_safety=None):
+    """Compute_Step_Size (rhs_mpi.f90:92-275) of an FP64 state (A in slots 6-8).
+
+    Returns a dict: dt (with the safety factor), the crossing-time minima dtmaxc (sound and
+    flow), dtmaxb (with the Alfven speed), dtmaxd (explicit-diffusion limit, 1e10 without
+    Laplacian) and the conservation totals E, L, m, mrsint (:156-183)."""
+    g, p, pr = _structs(model)
+    st = _f64(state)
+    out = np.zeros(8)
+    cs = model.c.cfl_safety if cfl_safety is None else cfl_safety
+    rc = lib().orc_step_size(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st), float(cs),
+                             _ptr(out))
+    if rc:
+        raise ValueError(f"orc_step_size rc={rc}")
+    return {"dt": out[0], "E": out[1], "L": out[2], "m": out[3], "mrsint": out[4], "dtmaxc": out[5],
+            "dtmaxb": out[6], "dtmaxd": out[7]}
+
+
+def lap_cross(model, state):
+    """The six sweeps of Laplacian_Cross_Terms_Radial/_Theta (rhs_mpi.f90:2070-2296) that the
+    RHS forms for a state (Laplacian on): d(r X_k)/dr for X = (dw/dth/r, du/dph/(r sin),
+    du/dth/r, dv/dph/(r sin)), then d(sin X_4)/dth and d(sin dw/dph/(r sin))/dth."""
+    g, p, pr = _structs(model)
+    st = _f64(state)
+    out = np.empty((6,) + st.shape[1:])
+    rc = lib().orc_lap_cross(ctypes.addressof(g), ctypes.addressof(p), ctypes.addressof(pr), _ptr(st), _ptr(out))
+    if rc:
+        raise ValueError(f"orc_lap_cross rc={rc}")
+    return out

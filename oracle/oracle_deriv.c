@@ -309,7 +309,7 @@ int orc_compact_fd6_py(double dri, int n1, const double *arr, const double *b1, 
     double *uppr, *lowr, *diag, *cp, *dp;
     double dli, c1, c2, c3, c4, c5, alpha, alpha2, gamma2, alpha1, beta1, cst, up1;
     int i;
-    if (n1 < 6) return 22;
+    if (n1 < 5) return 22;   /* the rows reach arr[4] and arr[n1-5] (derivatives.py:65-66, :118) */
     if ((dtype == 1 || dtype == 3) && !b1) return 22;
     if ((dtype == 2 || dtype == 3) && !b2) return 22;
     uppr = (double *)malloc(sizeof(double) * 5 * n1);

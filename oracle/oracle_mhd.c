@@ -268,6 +268,10 @@ static void vector_curl(const orc_mhd *m, const double *bt, const double *bp, co
         for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) jt[i] = jt[i] - work1[i] * (1.0 / m->r[ir]);
 }
 
+/* When set, orc_rhs copies the six Laplacian cross-term sweeps here (6 fields): the r sweeps
+ * d(r vars2(k))/dr, k = 1..4, then the theta sweeps d(sin vars2(4))/dth, d(sin vars2(5))/dth. */
+static double *g_cross_dump = NULL;
+
 /* RHS on state v (nv fields with A in 6-8) -> dv (nv fields); bout (3 fields) optional. */
 int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, const double *v,
             double *dv, double *bout)
@@ -711,6 +715,7 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                                       ? r2 * (v2[3][IDX(m, it, ip, nr - 1)] / r2 / m->dxdr[nr - 1]) : 0.0;
                     }
                 dr_(m, X, dX, code[k], wlo, whi, glo, ghi);
+                if (g_cross_dump) memcpy(g_cross_dump + k * N, dX, sizeof(double) * N);
                 for (ir = 0; ir < nr; ir++) {
                     const double c6 = (1.0 / 3.0) * m->mur[ir] / m->r[ir];
                     for (i = IDX(m, 0, 0, ir); i < IDX(m, 0, 0, ir + 1); i++) {
@@ -743,6 +748,7 @@ int orc_rhs(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profile
                                          : 0.0;
                     }
                 dth(m, X, dX, k == 0 ? p->ibct[3] : p->ibct[2], wlo, whi, glo, ghi);
+                if (g_cross_dump) memcpy(g_cross_dump + (4 + k) * N, dX, sizeof(double) * N);
                 for (ir = 0; ir < nr; ir++) {
                     const double rm = m->mur[ir] / m->r[ir];
                     for (ip = 0; ip < nphi; ip++)
@@ -1303,4 +1309,111 @@ int orc_rk3_tvd(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_pro
     }
     free(y1); free(k); free(D); free(tmp); free(B);
     return rc;
+}
+
+/* The six cross-term sweeps of Laplacian_Cross_Terms_Radial / _Theta (rhs_mpi.f90:2070-2296)
+ * that orc_rhs forms for state v (Laplacian on), into out (6 fields, order as g_cross_dump). */
+int orc_lap_cross(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, const double *v,
+                  double *out)
+{
+    const int nv = p->magnetic ? 8 : 5;
+    const long N = (long)g->nr * g->nth * g->nphi;
+    double *dv = (double *)malloc(sizeof(double) * nv * N);
+    int rc;
+    if (!dv) return 12;
+    if (!p->laplacian) { free(dv); return 22; }
+    g_cross_dump = out;
+    rc = orc_rhs(g, p, pr, v, dv, NULL);
+    g_cross_dump = NULL;
+    free(dv);
+    return rc;
+}
+
+/* ---------------------------------------------------------------------------------------------
+ * Compute_Step_Size (rhs_mpi.f90:92-275), stratified branch, on an FP64 state with A in slots
+ * 6-8.  B = curl A with the Compute_B wall conditions (Compute_B(0), :110).  Loops in the
+ * reference's order: ir, then it, with the phi sums of each (it, ir) row (:145-185).
+ *   dxmin = min(dr, r min(dth, sin(th) dphi)); dth = (th_N - th_1)/(nth-1), dphi = 1/dpi
+ *   T = invtconst rho^(gamma-1) exp(s/Cv) (:107)
+ *   ut = u^2 + v^2 + w^2 + (gamma-1) Cp xi_cs^2 T; dtmaxc = min dxmin/sqrt(ut) (:169-173)
+ *   dtmaxb = min dxmin/sqrt(ut + |B|^2/(4 pi rho))          (:174-181, magnetic)
+ *   dtmaxd = min(min dr, r1 min(min(sin th1, sin thN) dphi, dth))^2
+ *            / max(max ks, max mur/rho_bar, max etar)        (:194-198, Laplacian; else 1e10)
+ *   dt = min(dtmaxb, dtmaxc, dtmaxd) cfl_safety              (:229); dtmaxb/c start at 1e4
+ * out[0] = dt, out[1..4] = E_tot, L_tot, m_tot, mrsint (:156-183), out[5..7] = dtmaxc, dtmaxb,
+ * dtmaxd.
+ * ------------------------------------------------------------------------------------------- */
+int orc_step_size(const cssx_grid *g, const cssx_mhd_params *p, const cssx_mhd_profiles *pr, const double *v,
+                  double cfl_safety, double *out)
+{
+    orc_mhd M, *m = &M;
+    long N, np;
+    int nr, nth, nphi, it, ip, ir;
+    double *B = NULL, *work, *work2, *glo, *ghi;
+    double Etot = 0.0, Ltot = 0.0, mtot = 0.0, mrsint = 0.0, dtmaxc = 1e4, dtmaxb = 1e4, dtmaxd = 1e10;
+    double dth, dphi;
+    setup(m, g, p, pr);
+    N = m->N; nr = m->nr; nth = m->nth; nphi = m->nphi;
+    dth = (pr->theta[nth - 1] - pr->theta[0]) / (double)(nth - 1);
+    dphi = 1.0 / g->dpi;
+    np = (long)nphi * (nr > nth ? nr : nth) * 4 + 16;
+    work = (double *)malloc(sizeof(double) * N);
+    work2 = (double *)malloc(sizeof(double) * N);
+    glo = (double *)calloc(np * 4 > 4L * nth * nphi ? np * 4 : 4L * nth * nphi, sizeof(double));
+    ghi = (double *)calloc(np * 4 > 4L * nth * nphi ? np * 4 : 4L * nth * nphi, sizeof(double));
+    if (p->magnetic) B = (double *)calloc(3 * N, sizeof(double));
+    if (!work || !work2 || !glo || !ghi || (p->magnetic && !B)) return 12;
+    if (p->magnetic) compute_b(m, v + 5 * N, B, work, work2, glo, ghi, 1);
+    for (ir = 0; ir < nr; ir++) {
+        const double radius = pr->r[ir], drad = pr->dr[ir];
+        for (it = 0; it < nth; it++) {
+            const double sinth = m->sines[it];
+            const double dxmin = fmin(drad, radius * fmin(dth, sinth * dphi));
+            double srho = 0.0, sl = 0.0, se = 0.0, sb = 0.0, vmax = 1e300, vamax = 1e300;
+            for (ip = 0; ip < nphi; ip++) {
+                const long i = IDX(m, it, ip, ir);
+                const double rho = v[i], w = v[N + i], vv = v[2 * N + i], u = v[3 * N + i], s = v[4 * N + i];
+                const double tt = m->invtconst * pow(rho, m->gamm1) * exp(s * m->invCv);
+                const double wt = w * w, ut = u * u, vp = vv + radius * sinth * p->omega0;
+                srho += rho;
+                sl += rho * sqrt(wt + sinth * sinth * vp * vp);
+                se += rho * (m->Cv * tt + 0.5 * (ut + wt + vv * vv));
+                {
+                    const double ct = m->gamm1 * p->Cp * p->xi_cs * p->xi_cs * tt;
+                    const double us = ut + vv * vv + wt + ct;
+                    vmax = fmin(vmax, dxmin / sqrt(us));
+                    if (p->magnetic) {
+                        const double b2 = B[i] * B[i] + B[N + i] * B[N + i] + B[2 * N + i] * B[2 * N + i];
+                        sb += b2;
+                        vamax = fmin(vamax, dxmin / sqrt(us + b2 * ONE_OVER_4PI / rho));
+                    }
+                }
+            }
+            mtot += radius * radius * sinth * srho * drad * dth * dphi;
+            mrsint += radius * radius * radius * sinth * sinth * sinth * srho * drad * dth * dphi;
+            Ltot += radius * radius * radius * sinth * sl * drad * dth * dphi;
+            Etot += radius * radius * sinth * se * drad * dth * dphi;
+            dtmaxc = fmin(dtmaxc, vmax);
+            if (p->magnetic) {
+                Etot += 0.5 * radius * radius * sinth * sb * drad * dth * dphi * ONE_OVER_4PI;
+                dtmaxb = fmin(dtmaxb, vamax);
+            }
+        }
+    }
+    if (p->laplacian) {
+        double drmin = pr->dr[0], dmax = 0.0, sinmin = fmin(m->sines[0], m->sines[nth - 1]), x;
+        for (ir = 0; ir < nr; ir++) {
+            drmin = fmin(drmin, pr->dr[ir]);
+            dmax = fmax(dmax, pr->ks[ir]);
+            dmax = fmax(dmax, pr->mur[ir] / pr->rho_bar[ir]);
+            dmax = fmax(dmax, pr->etar[ir]);
+        }
+        x = fmin(drmin, pr->r[0] * fmin(sinmin * dphi, dth));
+        dtmaxd = x * x / dmax;
+    }
+    out[0] = fmin(fmin(dtmaxb, dtmaxc), dtmaxd) * cfl_safety;
+    out[1] = Etot; out[2] = Ltot; out[3] = mtot; out[4] = mrsint;
+    out[5] = dtmaxc; out[6] = dtmaxb; out[7] = dtmaxd;
+    free(work); free(work2); free(glo); free(ghi); free(B); free(m->sines);
+    return 0;
 }

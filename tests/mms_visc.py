@@ -28,10 +28,8 @@ from sympy.vector import CoordSys3D, curl, divergence, gradient
 C_LIGHT = 2.99792458e10
 
 
-def build(model):
-    c = model.c
-    S = CoordSys3D("S", transformation="spherical", variable_names=("r", "theta", "phi"))
-    r, th, ph = S.r, S.theta, S.phi
+def default_fields(model, r, th, ph):
+    """(rho, s, u_r, u_theta, u_phi, A_r, A_theta, A_phi) of the interior manufactured state."""
     R = sp.Rational
     rho = 1 + R(3, 10) * sp.cos(5 * r) * (2 + sp.sin(2 * th + R(3, 10)) * sp.cos(4 * ph))
     s = -R(1, 2) + R(1, 20) * sp.sin(4 * r) * sp.cos(3 * th) * sp.sin(4 * ph + R(1, 5))
@@ -41,6 +39,15 @@ def build(model):
     Ar = R(1, 10) * sp.sin(r) * sp.sin(th) * sp.cos(4 * ph + 1)
     At = R(1, 10) * sp.sin(2 * r) * sp.cos(th) * sp.cos(4 * ph)
     Ap = R(2, 10) * sp.cos(3 * r) * sp.sin(2 * th) * sp.sin(4 * ph)
+    return rho, s, ur, ut, up, Ar, At, Ap
+
+
+def build(model, fields=default_fields):
+    c = model.c
+    S = CoordSys3D("S", transformation="spherical", variable_names=("r", "theta", "phi"))
+    r, th, ph = S.r, S.theta, S.phi
+    R = sp.Rational
+    rho, s, ur, ut, up, Ar, At, Ap = fields(model, r, th, ph)
     # polytrope profiles
     gamm1 = c.gamma - 1.0
     zeta = 1 + model.alp * (c.r2 / r - 1)
@@ -96,10 +103,10 @@ def build(model):
     return out
 
 
-def evaluate(model):
+def evaluate(model, fields=default_fields):
     """(state, expected Laplacian contribution (8 fields) with the reference's departures,
     the departures themselves (8 fields each), the thermal-conduction part of ds)."""
-    fn = build(model)
+    fn = build(model, fields)
     Rg, Pg, Tg = np.meshgrid(model.r, model.phi, model.theta, indexing="ij")
     shp = Rg.shape
     bc = lambda v: np.broadcast_to(np.asarray(v, dtype=np.float64), shp)

@@ -149,21 +149,26 @@ def test_rk4_matches_oracle(mhd, shape):
     sol.close()
 
 
-def test_cfl_step(mhd):
-    c = Config(nr=24, nth=20, nphi=16)
+@pytest.mark.parametrize("case", ["plain", "flow_rotation", "laplacian"])
+def test_cfl_step(mhd, case):
+    """cssx_cfl against the oracle's Compute_Step_Size (rhs_mpi.f90:92-275), which the CPU tests
+    pin by closed forms; stretched grid, perturbed state, B from the seeded vector potential."""
+    kw = {"plain": {}, "flow_rotation": {"omega0": 0.3, "xi_cs": 0.6},
+          "laplacian": {"Laplacian": True, "SLDiffusion": False, "kappa_value": 3e-3}}[case]
+    c = Config(nr=24, nth=20, nphi=16, bamp=1.0, **kw)
     m = Model(c)
     st = m.initial_state()
+    if case == "flow_rotation":
+        rng = np.random.default_rng(9)
+        st[1:4] = 0.2 * rng.standard_normal(st[1:4].shape)
     sol = mhd(m)
     dt = sol.cfl(torch.as_tensor(st, device="cuda"))
-    # sound crossing time of the smallest cell bounds the step (rhs_mpi.f90:145-185)
-    _, B = oracle.rhs(m, st, want_b=True)
-    T = m.invtconst * st[0] ** m.gamm1 * np.exp(st[4] / m.Cv)
-    dxmin = np.minimum(m.dr[:, None, None], m.r[:, None, None] *
-                       np.minimum(m.dth, np.sin(m.theta)[None, None, :] * m.dphi))
-    ut = st[1] ** 2 + st[2] ** 2 + st[3] ** 2 + m.gamm1 * c.Cp * T
-    vb = dxmin / np.sqrt(ut + (B ** 2).sum(0) / (4 * np.pi) / st[0])
-    ref = min(min((dxmin / np.sqrt(ut)).min(), vb.min()), 1e4) * c.cfl_safety
-    assert abs(dt - ref) < 1e-12 * ref
+    ref = oracle.step_size(m, st)
+    assert abs(dt - ref["dt"]) < 1e-12 * ref["dt"], (dt, ref)
+    if case == "plain":
+        assert ref["dtmaxb"] < 0.999 * ref["dtmaxc"]      # the Alfven term decides this case
+    if case == "laplacian":
+        assert ref["dtmaxd"] < min(ref["dtmaxb"], ref["dtmaxc"])
     sol.close()
 
 
@@ -189,8 +194,9 @@ def test_c2_shape_step_finite_and_fp32_close(mhd):
 
 
 def test_conservation_totals():
-    """E, L, m, mrsint of Compute_Step_Size (rhs_mpi.f90:156-183) against numpy sums over the
-    same grid, and m of a uniform density against the exact shell-sector volume."""
+    """E, L, m, mrsint of Compute_Step_Size (rhs_mpi.f90:156-183) against the oracle, which the
+    CPU tests pin by closed forms, and m of a uniform density against the exact shell-sector
+    volume."""
     from cssx.mhd import Solver
     from cssx.model import Config, Model
     print("conservation totals", flush=True)
@@ -202,19 +208,9 @@ def test_conservation_totals():
     st[1:4] = 1e-2 * rng.standard_normal(st[1:4].shape)
     y = torch.as_tensor(st, device="cuda")
     got = sol.conservation(y)
-    _, B = sol.rhs(y, want_b=True)
-    B = B.cpu().numpy()
-    r = m.r[:, None, None]
-    sn = np.sin(m.theta)[None, None, :]
-    dV = r ** 2 * sn * m.dr[:, None, None] * m.dth * m.dphi
-    rho, w, v, u, s = st[0], st[1], st[2], st[3], st[4]
-    T = m.invtconst * rho ** m.gamm1 * np.exp(s / m.Cv)
-    E = np.sum(rho * (m.Cv * T + 0.5 * (u * u + w * w + v * v)) * dV) + np.sum(0.5 * (B ** 2).sum(0) * dV) / (4 * np.pi)
-    L = np.sum(r * rho * np.sqrt(w * w + sn ** 2 * (v + r * sn * c.omega0) ** 2) * dV)
-    M = np.sum(rho * dV)
-    MR = np.sum(r * sn ** 2 * rho * dV)
-    for k, ref in (("E", E), ("L", L), ("m", M), ("mrsint", MR)):
-        assert abs(got[k] - ref) < 1e-12 * abs(ref), (k, got[k], ref)
+    ref = oracle.step_size(m, st)
+    for k in ("E", "L", "m", "mrsint"):
+        assert abs(got[k] - ref[k]) < 1e-12 * abs(ref[k]), (k, got[k], ref[k])
     y1 = torch.ones_like(y)
     y1[1:4] = 0.0
     vol = (c.r2 ** 3 - c.r1 ** 3) / 3 * (np.cos(c.th1) - np.cos(c.th2)) * (c.phi2 - c.phi1)
