@@ -1,17 +1,19 @@
 #!/usr/bin/env python3
 """Benchmark: RK4 steps of the 8-field compressible MHD system on one MI355X.
 
-Workload: one 128 x 256 x 512 (r x theta x phi) subdomain, isentropic polytrope (n=1.5,
-density contrast 20), seeded entropy perturbation (1e-3) and random vector potential (1e-4),
-u = 0, dt from the CFL condition at 0.4, stress-free impenetrable radial walls, periodic phi.
-A "step" is one full Step_RK4 (four RHS evaluations + stage updates + Bound) on the whole
-subdomain with the state resident in HBM.
+Workload (default): one 256 x 512 x 1024 (r x theta x phi) subdomain, isentropic polytrope
+(n=1.5, density contrast 20), seeded entropy perturbation (1e-3) and random vector potential
+(1e-4), u = 0, dt from the CFL condition at 0.4, stress-free impenetrable radial walls, periodic
+phi, SLD diffusion on.  A "step" is one full Step_RK4 (four RHS evaluations + stage updates +
+Bound) on the whole subdomain with the state resident in HBM.
 
-`value` is the FP32-bulk path (FP32 stages/RHS, FP64 accumulation); the FP64 path, the
-FP32-vs-FP64 relative error, per-phase kernel times, the roofline of the dominant kernel and
-a CPU baseline (the FP64 oracle on a bounded sample) are reported beside it.
+`value` is the FP32-bulk path (FP32 stages/RHS/diffusion, FP64 state y0); the FP64 path, the
+100-step FP32-vs-FP64 precision study (max/RMS relative error per field every 10 steps),
+per-phase kernel times, the roofline of the dominant kernel, the same measurement at
+128 x 256 x 512 and a CPU baseline (the FP64 oracle, one step at 128 x 256 x 512) are
+reported beside it.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2|c3|c5]
 """
 from __future__ import annotations
 
@@ -29,10 +31,13 @@ for _p in (ROOT, os.path.join(ROOT, "css-code_amd")):
 
 METRIC = "RK4 Mpt-steps/s + HBM TB/s, FP32 vs FP64 at ≤1e-5 rel err; closure speedup"
 HBM_PEAK_GBS = 8000.0
-# compulsory stage traffic of one RK4 step, bytes per point (8 fields): FP64 17 field
-# accesses x 8 B; mixed (FP32 stage state, FP64 y0/accumulator) 8 x (20 + 32 + 32 + 28)
+# compulsory stage traffic of one RK4 step, bytes per point (8 fields).  FP64: 17 field
+# accesses x 8 B (stage 1 reads y0, writes acc and y1; stages 2-3 read y_s, y0, acc and write
+# acc, y_s+1; stage 4 reads y3, y0, acc and writes y0).  Mixed path as stored: FP32 stage states,
+# FP32 accumulator and FP32 copy of y0, FP64 y0 read and written once (stage 4, which also
+# refreshes the FP32 copy): per field 12 + 20 + 20 + (4 + 4 + 8 + 8 + 4) = 80 B.
 STEP_BYTES_F64 = 1088.0
-STEP_BYTES_MIXED = 896.0
+STEP_BYTES_MIXED = 640.0
 
 
 def _sync():
@@ -66,29 +71,23 @@ def profile_phases(sol, y0, dt, steps, precision):
     return tab
 
 
-def cpu_baseline(sld_coef, seconds_budget=20.0):
-    """The FP64 oracle RK4 step on a bounded sample of the same recipe (OpenMP threads)."""
-    import numpy as np
-
+def cpu_baseline(sld_coef, shape=(128, 256, 512), seeds=(5, 6)):
+    """The FP64 oracle (oracle/*.c, OpenMP) timed on one Step_RK4 of a bounded sample of the
+    workload: the same recipe and seeds on a 128 x 256 x 512 grid (1/8 of the 256 x 512 x 1024
+    points; about 15-30 s of CPU work on the host's cores)."""
     import oracle
     from cssx.model import Config, Model
-    c = Config(nr=32, nth=64, nphi=128, sld_coef=sld_coef)
+    c = Config(nr=shape[0], nth=shape[1], nphi=shape[2], sld_coef=sld_coef, seed_s=seeds[0], seed_a=seeds[1])
     m = Model(c)
-    st = m.initial_state()
-    g, p, pr = oracle._structs(m)
+    y = m.initial_state()
     t0 = time.perf_counter()
-    steps = 0
-    y = st.copy()
-    while True:
-        y = oracle.rk4(m, y, 1e-4, 1)
-        steps += 1
-        if time.perf_counter() - t0 > seconds_budget or steps >= 50:
-            break
+    y = oracle.rk4(m, y, 1e-4, 1)
     el = time.perf_counter() - t0
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": c.npts * steps / el / 1e6, "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle (oracle/*.c) Step_RK4 on a {c.nr}x{c.nth}x{c.nphi} subdomain of the same "
-                      f"recipe, {steps} step(s) in {el:.1f} s, OpenMP threads={threads}"}
+    return {"value": round(c.npts / el / 1e6, 4), "unit": "Mpt-steps/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle (oracle/*.c, OpenMP threads={threads}) one Step_RK4 with SLD of the same recipe "
+                      f"on {c.nr}x{c.nth}x{c.nphi} (1/8 of the workload's points) in {el:.1f} s; the rate is per point, "
+                      "so it stands for the full workload only as an extrapolation"}
 
 
 def precision_study(sol, y0, dt, steps, nv, every=10):
@@ -346,8 +345,9 @@ def main():
     ap.add_argument("--c3-chunk", type=int, default=64)
     ap.add_argument("--c3-members", type=int, default=64)
     ap.add_argument("--c3-ens-steps", type=int, default=5)
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
-                    help="c2: one 128x256x512 subdomain (the headline); c4: the 256x512x1024 precision study")
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c4",
+                    help="c4: one 256x512x1024 subdomain with the precision study (the headline); c2: 128x256x512")
+    ap.add_argument("--no-c2", action="store_true", help="c4: leave out the 128x256x512 measurement")
     ap.add_argument("--nr", type=int, default=0)
     ap.add_argument("--nth", type=int, default=0)
     ap.add_argument("--nphi", type=int, default=0)
@@ -393,22 +393,45 @@ def main():
         print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}))
         return
     seeds = {"c2": (2, 3), "c4": (5, 6)}[args.config]     # entropy / vector-potential seeds
+    out = run_mhd(args, (args.nr, args.nth, args.nphi), seeds, args.study_steps, full=True)
+    if args.config == "c4" and not args.no_c2:
+        # the same measurement at 128 x 256 x 512 (seeds 2/3), timed steps only
+        c2 = run_mhd(args, (128, 256, 512), (2, 3), 0, full=False)
+        out["c2"] = {k: c2[k] for k in ("value", "unit", "ms_per_step", "fp64", "fp32_speedup_vs_fp64",
+                                        "fp32_vs_fp64", "hbm_tbs_algorithmic", "roofline")}
+        out["c2"]["workload"] = c2["config"]["workload"]
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(out["config"]["sld_coef"])
+    print(json.dumps(_clean(out)))
+
+
+def run_mhd(args, shape, seeds, study_steps, full):
+    """RK4 steps of one subdomain in FP64 and FP32-bulk: the timed steps, their agreement, the
+    precision study and the per-kernel timings (all when `full`)."""
+    import torch
+
+    from cssx.mhd import Solver
+    from cssx.model import Config, Model
     lap = bool(args.laplacian)
-    c = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=0.0 if lap else args.sld_coef, seed_s=seeds[0],
+    c = Config(nr=shape[0], nth=shape[1], nphi=shape[2], sld_coef=0.0 if lap else args.sld_coef, seed_s=seeds[0],
                seed_a=seeds[1], Laplacian=lap, kappa_value=args.kappa, SLDiffusion=not lap)
     m = Model(c)
     y0 = torch.as_tensor(m.initial_state(), device="cuda").contiguous()
     sol = Solver(m)
     dt = sol.cfl(y0)
     npts = c.npts
+    workload = f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field" + ("_laplacian" if lap else "")
+    print(f"bench: {workload} dt={dt:.3e}", file=sys.stderr, flush=True)
 
     paths = set(args.paths.split(","))
     y64 = y32 = None
     ms64 = ms32 = float("nan")
     if "f64" in paths:
         y64, ms64 = run_path(sol, y0, dt, args.steps, args.warmup, "f64")
+        print(f"bench: {workload} f64 {ms64:.3f} ms/step", file=sys.stderr, flush=True)
     if "f32" in paths:
         y32, ms32 = run_path(sol, y0, dt, args.steps, args.warmup, "f32")
+        print(f"bench: {workload} f32 {ms32:.3f} ms/step", file=sys.stderr, flush=True)
 
     # FP32-bulk vs FP64 after the same K steps from the same state
     cmp = None
@@ -425,14 +448,16 @@ def main():
         cmp = {"steps": args.steps, "max_rel_err_state": max(rel_state), "max_rel_err_of_change": max(rel_change),
                "rms_rel_err_state": max(rms_state), "per_field_max_rel_err": [float(f"{x:.3e}") for x in rel_state],
                "finite": finite}
+    del y64, y32
 
     study = None
-    if args.study_steps > 0 and len(paths) == 2:
-        study = precision_study(sol, y0, dt, args.study_steps, c.nv)
+    if full and study_steps > 0 and len(paths) == 2:
+        study = precision_study(sol, y0, dt, study_steps, c.nv)
+        print(f"bench: {workload} precision study {study['max_rel_err_over_run']:.2e}", file=sys.stderr, flush=True)
 
     prof_steps = max(2, min(args.steps, 5))
     tab32 = profile_phases(sol, y0, dt, prof_steps, "f32") if "f32" in paths else None
-    tab64 = profile_phases(sol, y0, dt, prof_steps, "f64") if "f64" in paths else None
+    tab64 = profile_phases(sol, y0, dt, prof_steps, "f64") if ("f64" in paths and full) else None
 
     families = {"k_deriv": ("deriv_r", "deriv_theta", "deriv_phi"), "k_rhs_final": ("rhs_final",),
                 "k_sld": ("sld",), "pointwise_other": ("pointwise",), "rk_stage": ("rk_update",)}
@@ -470,7 +495,6 @@ def main():
 
     val32 = npts / (ms32 * 1e-3) / 1e6
     val64 = npts / (ms64 * 1e-3) / 1e6
-    workload = f"mhd_rk4_{c.nr}x{c.nth}x{c.nphi}_8field" + ("_laplacian" if lap else "")
     tr = measured_traffic(workload)
     roof32 = roofline(tab32)
     if roof32 is not None and tr is not None and roof32["kernel"] in tr.get("families", {}):
@@ -490,27 +514,32 @@ def main():
         "dtype": "f32" if "f32" in paths else "f64",
         "data": "synthetic",
         "config": {"workload": workload, "nr": c.nr, "nth": c.nth,
-                   "nphi": c.nphi, "fields": c.nv, "accumulate": "f64", "dt": dt, "sld_coef": c.sld_coef,
+                   "nphi": c.nphi, "fields": c.nv, "state_y0": "f64", "dt": dt, "sld_coef": c.sld_coef,
                    "parallelism": "single-gpu",
                    **({"laplacian": True, "kappa_value": c.kappa_value, "sld": False} if lap else {})},
         "fp64": {"value": round(val64, 2), "unit": "Mpt-steps/s", "ms_per_step": round(ms64, 4)},
         "fp32_speedup_vs_fp64": round(ms64 / ms32, 3) if len(paths) == 2 else None,
         "fp32_vs_fp64": cmp,
         "precision_study": study,
+        # compulsory stage bytes per step (STEP_BYTES_*) over the step time
         "hbm_tbs_algorithmic": {"f32": round(val32 * 1e6 * STEP_BYTES_MIXED / 1e12, 3),
-                                "f64": round(val64 * 1e6 * STEP_BYTES_F64 / 1e12, 3)},
-        # rocprof FETCH+WRITE bytes of one FP32-bulk step (committed summary) over this run's step time
-        "hbm_tbs_measured_f32": (round(tr["total_GB_per_step"] / ms32, 3) if tr and "f32" in paths else None),
-        "phases_ms_per_step": {p: ({k: round(v[0] / prof_steps, 4) for k, v in t.items()} if t else None)
-                               for p, t in (("f32", tab32), ("f64", tab64))},
+                                "f64": round(val64 * 1e6 * STEP_BYTES_F64 / 1e12, 3),
+                                "bytes_per_point": {"f32": STEP_BYTES_MIXED, "f64": STEP_BYTES_F64}},
+        # rocprofv3 FETCH+WRITE bytes of one FP32-bulk step of this same workload (committed
+        # counter summary) over this run's FP32 step time
+        "hbm_tbs_measured_f32": ({"value": round(tr["total_GB_per_step"] / ms32, 3), "GB_per_step": tr["total_GB_per_step"],
+                                  "source": tr["source"]} if tr and "f32" in paths else None),
         "roofline": roof32,
-        "roofline_f64": roofline(tab64),
-        "kernel_families": {p: (family_rows(t) if t else None) for p, t in (("f32", tab32), ("f64", tab64))},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(c.sld_coef)
+    if full:
+        out["roofline_f64"] = roofline(tab64)
+        out["phases_ms_per_step"] = {p: ({k: round(v[0] / prof_steps, 4) for k, v in t.items()} if t else None)
+                                     for p, t in (("f32", tab32), ("f64", tab64))}
+        out["kernel_families"] = {p: (family_rows(t) if t else None) for p, t in (("f32", tab32), ("f64", tab64))}
     sol.close()
-    print(json.dumps(_clean(out)))
+    del y0
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

@@ -290,6 +290,9 @@ __global__ __launch_bounds__(256) void k_compact_small(long npen, double dri, in
                                                        const double *__restrict__ arr, const double *__restrict__ b1,
                                                        const double *__restrict__ b2, double *darr,
                                                        double *__restrict__ res) {
+    // no fused multiply-adds: the rows and the recurrence round as the reference's numpy does,
+    // which decides the sign of a zero pivot in the singular case
+#pragma clang fp contract(off)
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npen) return;
     double a[N], d[N], lo[N], up[N];

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -49,6 +50,7 @@ constexpr double kJconst = 4.0 * 3.1415926535897932384626433832795028841972 / (2
 template <typename T>
 struct Prof {
     T *r = nullptr, *rinv = nullptr, *dxdr = nullptr, *grav = nullptr, *eps = nullptr;
+    T *dr = nullptr;    // cell widths dr_ind (Compute_Step_Size)
     T *sn = nullptr, *csc = nullptr, *cs = nullptr, *cot = nullptr;
     T *dsdr1 = nullptr, *dsdr2 = nullptr, *sr1 = nullptr;
     // hydrostatic background (nr): rho_bar, s_bar, T_bar and the physical radial derivatives
@@ -883,41 +885,13 @@ __global__ __launch_bounds__(256) void k_tvd_stage(Consts<TS> c, BoundFlags f, c
 //   m += rho dV,  mrsint += r sin^3(th) rho r^2 dr dth dphi,
 //   L += r rho sqrt(w^2 + sin^2(th) (v + r sin(th) Omega0)^2) dV,
 //   E += rho (Cv T + |u|^2/2) dV + |B|^2/(8 pi) dV.
-// red holds 6 values per block: min, min, E, L, m, mrsint.
-__global__ __launch_bounds__(256) void k_cfl(Consts<double> c, Prof<double> P, const double *__restrict__ dr,
-                                             double dth, double dphi, double Cp, const double *__restrict__ st,
-                                             const double *__restrict__ B, double *__restrict__ red) {
+// The grid strides over the points (at most kCflMaxBlocks blocks); red holds 6 values per
+// block: min, min, E, L, m, mrsint.
+constexpr int kCflMaxBlocks = 4096;
+
+__device__ __forceinline__ void cfl_block_reduce(double v[6], double *red) {
     __shared__ double sh[6][256];
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double vc = 1e4, vb = 1e4, e = 0.0, l = 0.0, ms = 0.0, mr = 0.0;
-    if (i < c.N) {
-        int it, ip, ir;
-        coords(i, c, it, ip, ir);
-        const long N = c.N;
-        const double radius = P.r[ir], sinth = P.sn[it];
-        const double dxmin = fmin(dr[ir], radius * fmin(dth, sinth * dphi));
-        const double rho = st[i], w = st[N + i], v = st[2 * N + i], u = st[3 * N + i], s = st[4 * N + i];
-        const double t = temperature(rho, s, c);
-        const double dvol = radius * radius * sinth * dr[ir] * dth * dphi;
-        ms = rho * dvol;
-        mr = radius * sinth * sinth * ms;
-        const double vv = v + radius * sinth * c.omega0;
-        l = radius * rho * sqrt(w * w + sinth * sinth * vv * vv) * dvol;
-        e = rho * (c.Cv * t + 0.5 * (u * u + w * w + v * v)) * dvol;
-        double ut = u * u + v * v + w * w + (c.gamm1 * Cp * c.xi2) * t;
-        vc = dxmin / sqrt(ut);
-        if (c.magnetic) {
-            const double b2 = B[i] * B[i] + B[N + i] * B[N + i] + B[2 * N + i] * B[2 * N + i];
-            vb = dxmin / sqrt(ut + b2 * kOneOver4Pi / rho);
-            e += 0.5 * b2 * dvol * kOneOver4Pi;
-        }
-    }
-    sh[0][threadIdx.x] = vc;
-    sh[1][threadIdx.x] = vb;
-    sh[2][threadIdx.x] = e;
-    sh[3][threadIdx.x] = l;
-    sh[4][threadIdx.x] = ms;
-    sh[5][threadIdx.x] = mr;
+    for (int q = 0; q < 6; q++) sh[q][threadIdx.x] = v[q];
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
@@ -928,7 +902,49 @@ __global__ __launch_bounds__(256) void k_cfl(Consts<double> c, Prof<double> P, c
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int q = 0; q < 6; q++) red[6 * blockIdx.x + q] = sh[q][0];
+        for (int q = 0; q < 6; q++) red[q] = sh[q][0];
+}
+
+__global__ __launch_bounds__(256) void k_cfl(Consts<double> c, Prof<double> P, double dth, double dphi, double Cp,
+                                             const double *__restrict__ st, const double *__restrict__ B,
+                                             double *__restrict__ red) {
+    double v[6] = {1e4, 1e4, 0.0, 0.0, 0.0, 0.0};
+    const long N = c.N;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        int it, ip, ir;
+        coords(i, c, it, ip, ir);
+        const double radius = P.r[ir], sinth = P.sn[it], drad = P.dr[ir];
+        const double dxmin = fmin(drad, radius * fmin(dth, sinth * dphi));
+        const double rho = st[i], w = st[N + i], vv = st[2 * N + i], u = st[3 * N + i], s = st[4 * N + i];
+        const double t = temperature(rho, s, c);
+        const double dvol = radius * radius * sinth * drad * dth * dphi;
+        const double ms = rho * dvol;
+        v[4] += ms;
+        v[5] += radius * sinth * sinth * ms;
+        const double vp = vv + radius * sinth * c.omega0;
+        v[3] += radius * rho * sqrt(w * w + sinth * sinth * vp * vp) * dvol;
+        double e = rho * (c.Cv * t + 0.5 * (u * u + w * w + vv * vv)) * dvol;
+        const double ut = u * u + vv * vv + w * w + (c.gamm1 * Cp * c.xi2) * t;
+        v[0] = fmin(v[0], dxmin / sqrt(ut));
+        if (c.magnetic) {
+            const double b2 = B[i] * B[i] + B[N + i] * B[N + i] + B[2 * N + i] * B[2 * N + i];
+            v[1] = fmin(v[1], dxmin / sqrt(ut + b2 * kOneOver4Pi / rho));
+            e += 0.5 * b2 * dvol * kOneOver4Pi;
+        }
+        v[2] += e;
+    }
+    cfl_block_reduce(v, red + 6 * blockIdx.x);
+}
+
+// one block: the nb block partials of k_cfl -> 6 values at out
+__global__ __launch_bounds__(256) void k_cfl_final(const double *__restrict__ red, int nb, double *__restrict__ out) {
+    double v[6] = {1e4, 1e4, 0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        v[0] = fmin(v[0], red[6 * b]);
+        v[1] = fmin(v[1], red[6 * b + 1]);
+        for (int q = 2; q < 6; q++) v[q] += red[6 * b + q];
+    }
+    cfl_block_reduce(v, out);
 }
 
 }  // namespace cssx
@@ -976,7 +992,7 @@ template <typename T>
 static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const int nr = m->nr, nth = m->nth;
     const std::vector<double> &drbar = m->h_drbar, &dsbar = m->h_dsbar, &hres = m->h_hres;
-    const size_t tot = 23 * (size_t)nr + 10 * (size_t)nth + 2;
+    const size_t tot = 24 * (size_t)nr + 10 * (size_t)nth + 2;
     std::vector<T> h(tot);
     size_t o = 0;
     auto put = [&](size_t n, auto gen) {
@@ -991,6 +1007,7 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const size_t o_x = put(nr, [&](size_t i) { return m->h_dxdr[i]; });
     const size_t o_g = put(nr, [&](size_t i) { return m->h_grav[i]; });
     const size_t o_e = put(nr, [&](size_t i) { return m->h_eps[i]; });
+    const size_t o_dr = put(nr, [&](size_t i) { return m->h_dr[i]; });
     const size_t o_rb = put(nr, [&](size_t i) { return m->h_rho_bar[i]; });
     const size_t o_sb = put(nr, [&](size_t i) { return m->h_s_bar[i]; });
     const size_t o_tb = put(nr, [&](size_t i) { return m->h_tbar[i]; });
@@ -1045,7 +1062,7 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     }
     if (hipMemcpy(d, h.data(), tot * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return CSSX_EHIP;
     P.buf = d;
-    P.r = d + o_r; P.rinv = d + o_ri; P.dxdr = d + o_x; P.grav = d + o_g; P.eps = d + o_e;
+    P.r = d + o_r; P.rinv = d + o_ri; P.dxdr = d + o_x; P.grav = d + o_g; P.eps = d + o_e; P.dr = d + o_dr;
     P.rbar = d + o_rb; P.sbar = d + o_sb; P.tbar = d + o_tb; P.tr = d + o_tr; P.hres = d + o_hr; P.drbar = d + o_drb; P.dsbar = d + o_dsb;
     P.sn = d + o_sn; P.csc = d + o_csc; P.cs = d + o_cs; P.cot = d + o_cot;
     P.dsdr1 = d + o_d1; P.dsdr2 = d + o_d2; P.sr1 = d + o_s1;
@@ -1091,11 +1108,11 @@ static int background_derivative(cssx_mhd *m, const double *prof_dev, int code, 
 }
 
 template <typename T>
-static Scratch<T> &scratch(cssx_mhd *m);
+static Scratch<T> &work(cssx_mhd *m);
 template <>
-Scratch<double> &scratch<double>(cssx_mhd *m) { return m->s64; }
+Scratch<double> &work<double>(cssx_mhd *m) { return m->s64; }
 template <>
-Scratch<float> &scratch<float>(cssx_mhd *m) { return m->s32; }
+Scratch<float> &work<float>(cssx_mhd *m) { return m->s32; }
 template <typename T>
 static Prof<T> &prof(cssx_mhd *m);
 template <>
@@ -1105,7 +1122,7 @@ Prof<float> &prof<float>(cssx_mhd *m) { return m->p32; }
 
 template <typename T>
 static int ensure_scratch(cssx_mhd *m) {
-    Scratch<T> &S = scratch<T>(m);
+    Scratch<T> &S = work<T>(m);
     if (S.ready) return CSSX_OK;
     const long N = m->N;
     WallPtrs W = wall_layout(m->nr, m->nth, m->nphi);
@@ -1180,7 +1197,7 @@ template <typename T>
 static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream, bool zero_walls = true) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
-    Scratch<T> &S = scratch<T>(m);
+    Scratch<T> &S = work<T>(m);
     Prof<T> &P = prof<T>(m);
     const Consts<T> c = make_consts<T>(m);
     const long N = m->N;
@@ -1211,21 +1228,14 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
 // applies that RK4 stage instead of writing the full tendency; with sld_dif the SLD diffusion
 // of st is computed into sld_dif once B is known (Compute_Diffusion(1) after RHS(1)).
 template <typename T>
+static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream, const StageFuse<T> *F,
+                    T *sld_dif, bool fork_closure);
+
+template <typename T>
 static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream, const StageFuse<T> *F = nullptr,
                     T *sld_dif = nullptr) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
-    Scratch<T> &S = scratch<T>(m);
-    Prof<T> &P = prof<T>(m);
-    const Consts<T> c = make_consts<T>(m);
-    const long N = m->N;
-    const int *ibcr = m->p.ibcr, *ibct = m->p.ibct;
-    T *D = S.D;
-    T *B = Bout ? Bout : S.B;
-    const WallPtrs W = wall_layout(m->nr, m->nth, m->nphi);
-    T *wall = S.wall;
-    const T *rho = st, *w = st + N, *v = st + 2 * N, *u = st + 3 * N, *s = st + 4 * N;
-    const unsigned nb = nblocks(N, 256);
     // 0. the learned closure's tendency of the stage state, on the side stream (it reads only
     //    the state, which nothing here writes before the final kernel).  While the caller's
     //    stream is being captured into a graph the closure stays on that stream: the fork has
@@ -1237,13 +1247,40 @@ static int rhs_impl(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         fork_closure = cap == hipStreamCaptureStatusNone;
     }
     if (fork_closure) {
+        const Prof<T> &P = prof<T>(m);
         if (hipEventRecord(m->ev_fork, stream) != hipSuccess || hipStreamWaitEvent(m->side, m->ev_fork, 0) != hipSuccess)
             return CSSX_EHIP;
-        if ((rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, c.fluct, P.rbar,
-                                   P.sbar, m->nr, m->nphi, m->nth, m->ext, m->side)))
-            return rc;
-        if (hipEventRecord(m->ev_join, m->side) != hipSuccess) return CSSX_EHIP;
+        rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, make_consts<T>(m).fluct,
+                              P.rbar, P.sbar, m->nr, m->nphi, m->nth, m->ext, m->side);
+        // the join event is recorded whatever the closure returned, so that every exit below
+        // leaves the caller's stream ordered after the side stream's work
+        const hipError_t e = hipEventRecord(m->ev_join, m->side);
+        if (rc || e != hipSuccess) {
+            if (e == hipSuccess) (void)hipStreamWaitEvent(stream, m->ev_join, 0);
+            else (void)hipStreamSynchronize(m->side);
+            return rc ? rc : CSSX_EHIP;
+        }
     }
+    rc = rhs_body<T>(m, st, dv, Bout, stream, F, sld_dif, fork_closure);
+    if (rc && fork_closure) (void)hipStreamWaitEvent(stream, m->ev_join, 0);   // failed before its own join
+    return rc;
+}
+
+template <typename T>
+static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream, const StageFuse<T> *F,
+                    T *sld_dif, bool fork_closure) {
+    int rc = CSSX_OK;
+    Scratch<T> &S = work<T>(m);
+    Prof<T> &P = prof<T>(m);
+    const Consts<T> c = make_consts<T>(m);
+    const long N = m->N;
+    const int *ibcr = m->p.ibcr, *ibct = m->p.ibct;
+    T *D = S.D;
+    T *B = Bout ? Bout : S.B;
+    const WallPtrs W = wall_layout(m->nr, m->nth, m->nphi);
+    T *wall = S.wall;
+    const T *rho = st, *w = st + N, *v = st + 2 * N, *u = st + 3 * N, *s = st + 4 * N;
+    const unsigned nb = nblocks(N, 256);
     // 1. B = curl A
     if (m->p.magnetic && (rc = curl_impl<T>(m, st + 5 * N, B, stream))) return rc;
     const T *bt = B, *bp = B + N, *br = B + 2 * N;
@@ -1383,6 +1420,7 @@ static SldParams<T> make_sld(const cssx_mhd *m) {
     const double tv[8] = {1e-8 * m->h_rho_bar[m->nr - 1], 1e-8, 1e-8, 1e-8, 1e-3, 1.0, 1.0, 1.0};
     for (int f = 0; f < 8; f++) {
         sp.tiny[f] = (T)tv[f];
+        sp.itiny2[f] = (T)(1.0 / (tv[f] * tv[f]));
         const int iv = f + 1;
         // Diffusion_R scales rho,w,v by sld_pr and s,A by sld_prm (:846-854); Diffusion_Th/Phi
         // scale rho,w,v,u by sld_pr and A by sld_prm (:1252-1258, :1476-1482)
@@ -1422,7 +1460,7 @@ template <typename T>
 static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
-    Scratch<T> &S = scratch<T>(m);
+    Scratch<T> &S = work<T>(m);
     Prof<T> &P = prof<T>(m);
     const Consts<T> c = make_consts<T>(m);
     const SldParams<T> sp = make_sld<T>(m);
@@ -1434,22 +1472,27 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
     const double b_speed = ts * N * (1.0 + (sp.dstype == 1 ? 3.0 + (sp.magnetic ? 4.0 : 0.0) : 0.0));
     CSSX_TIMED(CSSX_PHASE_SLD, stream, b_speed,
                hipLaunchKernelGGL(k_sld_speed<T>, dim3(nb), dim3(256), 0, stream, c, P, sp, st, B, vcc));
-    T *vt = S.D + 15 * N;   // face speeds, 3 fields (slots 15-17)
-    CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * 4.0,
-               hipLaunchKernelGGL(k_sld_faces<T>, dim3(nb), dim3(256), 0, stream, c, sp, (const T *)vcc, vt));
+    // tiles of 64 theta x PH phi columns; the r range is cut into segments so that the launch
+    // holds at least ~2048 workgroups
+    constexpr int PH = sizeof(T) == 8 ? 4 : 8;
+    const int ntt = (m->nth + 63) / 64, ntp = (m->nphi + PH - 1) / PH;
+    const long tiles = (long)ntt * ntp;
+    int nseg = (int)std::min<long>((2048 + tiles - 1) / tiles, std::max(1, m->nr / 16));
+    nseg = std::max(nseg, 1);
+    const int rseg = (m->nr + nseg - 1) / nseg;
+    nseg = (m->nr + rseg - 1) / rseg;
     const double nd = (double)(sp.last - sp.first + 1);
-    CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (2.0 * nd + 3.0 + (nv - nd)),
-               hipLaunchKernelGGL(k_sld_flux<T>, dim3(nb, nv), dim3(256), 0, stream, c, P, sp, st,
-                                  (const T *)vt, dif));
-    T *C = nullptr;
+    const double b_march = ts * N * (nd + 1.0 + (double)nv + ((sp.visc || sp.mass) ? 2.0 : 0.0));
+    CSSX_TIMED(CSSX_PHASE_SLD, stream, b_march,
+               hipLaunchKernelGGL((k_sld_march<T, PH>), dim3((unsigned)(tiles * nseg)), dim3(64 * PH), 0, stream, c,
+                                  P, sp, st, (const T *)vcc, dif, ntt, ntp, rseg));
     if (sp.ohmic) {
-        C = S.D + 6 * N;
+        T *C = S.D + 6 * N;
         if ((rc = curl_impl<T>(m, dif + 5 * N, C, stream, false))) return rc;
+        CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (3.0 + 3.0 + 2.0 + 2.0),
+                   hipLaunchKernelGGL(k_sld_finish<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B, (const T *)C,
+                                      dif));
     }
-    const double b_fin = ts * N * (2.0 + 4.0 + 4.0 + (sp.mass ? 1.0 + 3.0 : 0.0) + (C ? 6.0 : 0.0) + 1.0);
-    CSSX_TIMED(CSSX_PHASE_SLD, stream, b_fin,
-               hipLaunchKernelGGL(k_sld_finish<T>, dim3(nb), dim3(256), 0, stream, c, P, sp, st, B,
-                                  (const T *)C, dif));
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
 
@@ -1457,7 +1500,7 @@ template <typename T>
 static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t stream) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
-    Scratch<T> &S = scratch<T>(m);
+    Scratch<T> &S = work<T>(m);
     const Consts<T> c = make_consts<T>(m);
     const long NV = m->N * m->nv;
     const unsigned nb = nblocks(NV, 256);
@@ -1497,7 +1540,7 @@ template <typename T>
 static int tvd_impl(cssx_mhd *m, double *y0, double dt, int nsteps, int order, hipStream_t stream) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
-    Scratch<T> &S = scratch<T>(m);
+    Scratch<T> &S = work<T>(m);
     const Consts<T> c = make_consts<T>(m);
     const BoundFlags f = make_bound_flags(m);
     const long NV = m->N * m->nv;
@@ -1635,16 +1678,18 @@ extern "C" int cssx_mhd_create(const cssx_grid *grid, const cssx_mhd_params *par
 
 extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
     if (!m) return;
+    // the closure's side stream may still read ext and the state: drain it before any free
+    if (m->side) {
+        (void)hipStreamSynchronize(m->side);
+        (void)hipStreamDestroy(m->side);
+        m->side = nullptr;
+    }
     for (void *p : {(void *)m->p64.buf, (void *)m->p32.buf, (void *)m->s64.D, (void *)m->s64.B, (void *)m->s64.wall,
                     (void *)m->s64.y1, (void *)m->s64.k, (void *)m->s32.D, (void *)m->s32.B, (void *)m->s32.wall,
                     (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red, (void *)m->s64.dif,
                     (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f, (void *)m->ext, (void *)m->s64.L,
                     (void *)m->s64.lwall, (void *)m->s32.L, (void *)m->s32.lwall})
         if (p) (void)hipFree(p);
-    if (m->side) {
-        (void)hipStreamSynchronize(m->side);
-        (void)hipStreamDestroy(m->side);
-    }
     if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
     if (m->ev_join) (void)hipEventDestroy(m->ev_join);
     delete m;
@@ -1655,11 +1700,19 @@ extern "C" int cssx_mhd_set_closure(cssx_mhd *m, cssx_mlp *net) {
     if (net && m->nv != 8) return CSSX_EINVAL;
     if (net && !m->ext && hipMalloc((void **)&m->ext, (size_t)8 * m->N * sizeof(float)) != hipSuccess)
         return CSSX_ENOMEM;
-    if (net && !m->side &&
-        (hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking) != hipSuccess ||
-         hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess ||
-         hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming) != hipSuccess))
+    // each handle is created only if missing, so a call that failed part-way can be retried
+    if (net && !m->side && hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking) != hipSuccess) {
+        m->side = nullptr;
         return CSSX_EHIP;
+    }
+    if (net && !m->ev_fork && hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming) != hipSuccess) {
+        m->ev_fork = nullptr;
+        return CSSX_EHIP;
+    }
+    if (net && !m->ev_join && hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming) != hipSuccess) {
+        m->ev_join = nullptr;
+        return CSSX_EHIP;
+    }
     m->closure = net;
     return CSSX_OK;
 }
@@ -1699,7 +1752,7 @@ template <typename T>
 static int diffusion_entry(cssx_mhd *m, const T *st, const T *b, T *dif, hipStream_t stream) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
-    Scratch<T> &S = scratch<T>(m);
+    Scratch<T> &S = work<T>(m);
     const T *B = b;
     if (m->p.magnetic && !B) {
         if ((rc = curl_impl<T>(m, st + 5 * m->N, S.B, stream))) return rc;
@@ -1736,39 +1789,24 @@ static int cfl_partials(cssx_mhd *m, const double *y, hipStream_t st, std::vecto
     if (rc) return rc;
     const long N = m->N;
     if (m->p.magnetic && (rc = curl_impl<double>(m, y + 5 * N, m->s64.B, st))) return rc;
-    const unsigned nb = nblocks(N, 256);
+    const unsigned nb = std::min(nblocks(N, 256), (unsigned)kCflMaxBlocks);
     if (m->nred < (int)nb) {
         if (m->red) (void)hipFree(m->red);
         m->red = nullptr;
-        if (hipMalloc(&m->red, sizeof(double) * 6 * nb) != hipSuccess) return CSSX_ENOMEM;
-        m->nred = (int)nb;
+        if (hipMalloc(&m->red, sizeof(double) * 6 * kCflMaxBlocks) != hipSuccess) return CSSX_ENOMEM;
+        m->nred = kCflMaxBlocks;
     }
-    double *dr = nullptr;
-    if (hipMalloc(&dr, sizeof(double) * m->nr) != hipSuccess) return CSSX_ENOMEM;
-    if (hipMemcpyAsync(dr, m->h_dr.data(), sizeof(double) * m->nr, hipMemcpyHostToDevice, st) != hipSuccess) {
-        (void)hipFree(dr);
-        return CSSX_EHIP;
-    }
-    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, make_consts<double>(m), m->p64, (const double *)dr, m->dth,
-                       m->dphi, m->p.Cp, y, (const double *)m->s64.B, m->red);
+    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, st, make_consts<double>(m), m->p64, m->dth, m->dphi, m->p.Cp,
+                       y, (const double *)m->s64.B, m->red);
     h.assign(6 * (size_t)nb, 0.0);
     hipError_t e = hipMemcpyAsync(h.data(), m->red, sizeof(double) * 6 * nb, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    (void)hipFree(dr);
     return e == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
 
-extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream) {
-    if (!m || !y0_f64 || !dt_out || !(cfl_safety > 0.0)) return CSSX_EINVAL;
-    std::vector<double> h;
-    int rc = cfl_partials(m, (const double *)y0_f64, (hipStream_t)stream, h);
-    if (rc) return rc;
-    double dtc = 1e4, dtb = 1e4;
-    for (size_t b = 0; b < h.size() / 6; b++) {
-        dtc = std::min(dtc, h[6 * b]);
-        dtb = std::min(dtb, h[6 * b + 1]);
-    }
-    // explicit diffusion limit (rhs_mpi.f90:194-198)
+// dt from the crossing-time minima, with the explicit diffusion limit (rhs_mpi.f90:194-198,
+// :222-229)
+static int cfl_dt(const cssx_mhd *m, double dtc, double dtb, double cfl_safety, double *dt_out) {
     double dtd = 1e10;
     if (m->p.laplacian) {
         const int nr = m->nr, nth = m->nth;
@@ -1782,6 +1820,19 @@ extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, doub
     }
     *dt_out = std::min(std::min(dtb, dtc), dtd) * cfl_safety;
     return std::isfinite(*dt_out) ? CSSX_OK : CSSX_ENAN;
+}
+
+extern "C" int cssx_cfl(cssx_mhd *m, const void *y0_f64, double cfl_safety, double *dt_out, void *stream) {
+    if (!m || !y0_f64 || !dt_out || !(cfl_safety > 0.0)) return CSSX_EINVAL;
+    std::vector<double> h;
+    int rc = cfl_partials(m, (const double *)y0_f64, (hipStream_t)stream, h);
+    if (rc) return rc;
+    double dtc = 1e4, dtb = 1e4;
+    for (size_t b = 0; b < h.size() / 6; b++) {
+        dtc = std::min(dtc, h[6 * b]);
+        dtb = std::min(dtb, h[6 * b + 1]);
+    }
+    return cfl_dt(m, dtc, dtb, cfl_safety, dt_out);
 }
 
 extern "C" int cssx_conservation(cssx_mhd *m, const void *y0_f64, double *out4, void *stream) {

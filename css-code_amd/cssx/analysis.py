@@ -14,6 +14,8 @@ in one library call on the GPU.
   shell_avg         shell_avg.py:13-75 with integrate.py's Simpson rule, as one weighted
                     reduction on the GPU
   rossby_from_chk   rossby_from_checkpoint.py:15-220 ('adv', 'scale', 'ens', 'turn'; no plots)
+  slice_vorticity   the radial vorticity of plot_shell_slice.py:449-512 (Vorticity) on shell-slice
+                    data, every pencil of a slice record in two library calls
 """
 from __future__ import annotations
 
@@ -181,3 +183,26 @@ def rossby_from_chk(iter, case, omega0, method="adv", return_rad=False, base_dir
     else:
         raise ValueError(f"unknown method in rossby_from_chk: {method}")
     return (ro, radius) if return_rad else ro
+
+
+def slice_vorticity(data, theta, phi, r, iut, iup, irec=0):
+    """omega_r of shell-slice data (nth, nphi, nradii, nq, nrec) as plot_shell_slice.py:484-510
+    forms it: (1/(r sin)) d(sin u_phi)/dtheta - (1/(r sin)) du_theta/dphi, both with the numpy
+    compact_fd6 arguments the script passes -- theta: ibc 1, dtype 0; phi: ibc 3, dtype 0, so
+    the phi derivative closes with walls (zero slope at the first point), not periodically,
+    whatever its comment says; dti, dpi = 1/(max - min) of the grids.  r: the slice radii.
+    Returns wr (nth, nphi, nradii)."""
+    dev = _dev()
+    theta = np.asarray(theta, dtype=np.float64)
+    phi = np.asarray(phi, dtype=np.float64)
+    r = np.asarray(r, dtype=np.float64)
+    D = torch.as_tensor(np.asarray(data, dtype=np.float64)[:, :, :, :, irec], device=dev)
+    up = D[:, :, :, iup].contiguous()
+    ut = D[:, :, :, iut].contiguous()
+    dpi = 1.0 / (phi.max() - phi.min())
+    dti = 1.0 / (theta.max() - theta.min())
+    sn = torch.as_tensor(np.sin(theta), device=dev)[:, None, None]
+    rr = torch.as_tensor(r, device=dev)[None, None, :]
+    wr = compact_fd6(dti, (sn * up).contiguous(), 0.0, 0.0, 1, dtype=0, axis=0) / rr / sn
+    wr = wr - compact_fd6(dpi, ut, 0.0, 0.0, 3, dtype=0, axis=1) / (rr * sn)
+    return wr.cpu().numpy()

@@ -123,6 +123,28 @@ class Solver:
         _lib.check(getattr(_lib.lib(), name)(self._h, prec, y0.data_ptr(), float(dt), int(nsteps), _stream()), name)
         return y0
 
+    def capture(self, y0, dt, nsteps=1, precision="f64", scheme="rk4"):
+        """Capture `nsteps` steps of `scheme` on the FP64 state y0 (in place) in a HIP graph
+        (torch.cuda.CUDAGraph holds it); ``StepGraph.replay(n)`` then advances y0 by n * nsteps
+        steps with one graph launch each.  One eager step runs first, on a copy of y0, so that
+        the scratch buffers and plans exist before capture (the capture itself allocates
+        nothing); y0 is left unchanged until the first replay.  An attached closure runs on the
+        captured stream (no side-stream fork under capture)."""
+        self._check_state(y0, torch.float64)
+        warm = y0.clone()
+        self.rk4(warm, dt, 1, precision, scheme)
+        torch.cuda.synchronize()
+        del warm
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.rk4(y0, dt, nsteps, precision, scheme)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        return StepGraph(g, y0, nsteps)
+
     def cfl(self, y0, cfl_safety=None):
         """Compute_Step_Size: the CFL time step (synchronizes)."""
         self._check_state(y0, torch.float64)
@@ -138,3 +160,18 @@ class Solver:
         out = (ctypes.c_double * 4)()
         _lib.check(_lib.lib().cssx_conservation(self._h, y0.data_ptr(), out, _stream()), "cssx_conservation")
         return {"E": out[0], "L": out[1], "m": out[2], "mrsint": out[3]}
+
+
+class StepGraph:
+    """A captured sequence of RK4 steps of one state (Solver.capture)."""
+
+    def __init__(self, graph, state, nsteps):
+        self.graph = graph
+        self.state = state          # the captured kernels read and write this tensor
+        self.nsteps = nsteps
+
+    def replay(self, n: int = 1):
+        """n launches of the graph on the current stream: n * nsteps steps of the state."""
+        for _ in range(n):
+            self.graph.replay()
+        return self.state

@@ -23,10 +23,13 @@
  *   Step_RK3_TVD           step_rk4.f90:166-335
  *   Step_RK2_TVD           step_rk4.f90:31-162
  *   Compute_Diffusion      diffusion.f90:70-474, Diffusion_R/Th/Phi :476-1495 (SLD)
+ *   Compute_Step_Size      rhs_mpi.f90:92-275 (stratified branch: CFL and conservation totals)
  * Parity status: the RHS is pinned by a manufactured solution, hydrostatic balance and wall
- * invariants; the diffusion by conservation, the minmod identity, continuum geometry and the
- * corrections' energy/momentum identities, except its wall ghost rules (parity unpinned: only
- * the Fortran could pin them); the steppers by their order of accuracy.
+ * invariants, its Laplacian cross-term wall values by stress-free manufactured states; the
+ * diffusion by conservation, the minmod identity, continuum geometry, the corrections'
+ * energy/momentum identities and, for its wall ghost rules, the closed-form limiter response of
+ * fields linear up to a wall; Compute_Step_Size by closed forms on uniform grids; the steppers by
+ * their order of accuracy.
  * The subdomain has physical walls at both r ends and both theta ends; each wall is handled as
  * the rank touching it would (the Fortran's If (myr1==1) ... ElseIf (myr2==nr) chains need
  * two ranks per direction to visit both walls; here both are applied).

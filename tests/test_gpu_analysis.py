@@ -89,3 +89,71 @@ def test_read_checkpoint_layout(tmp_path):
     assert np.isclose(rad[0], 0.7) and np.isclose(rad[-1], 0.9) and np.allclose(tm, [1e-3, 2.0], rtol=1e-6)
     *_, ierr = analysis.read_checkpoint(6, "run1", base_dir=str(tmp_path) + "/")
     assert ierr
+
+
+def test_slice_vorticity_matches_script_loops():
+    """plot_shell_slice.py:484-510 with the oracle's compact_fd6 restatement in the script's
+    loops (theta ibc 1 dtype 0; phi ibc 3 dtype 0 with the unused periodic ghosts)."""
+    from cssx import analysis
+    print("slice vorticity", flush=True)
+    rng = np.random.default_rng(5)
+    nth, nphi, nrad, nq = 14, 20, 3, 4
+    theta = np.linspace(np.pi / 4, 3 * np.pi / 4, nth)
+    phi = np.linspace(0, np.pi / 2, nphi)
+    r = np.array([0.75, 0.8, 0.9])
+    data = rng.standard_normal((nth, nphi, nrad, nq, 2))
+    iut, iup, irec = 1, 3, 1
+    got = analysis.slice_vorticity(data, theta, phi, r, iut, iup, irec)
+    f6 = lambda *a, **k: oracle.compact_fd6_py(*a, **k)[0]
+    wr = np.empty((nth, nphi, nrad))
+    sin_t = np.sin(theta)
+    up, ut = data[:, :, :, iup, irec], data[:, :, :, iut, irec]
+    dpi, dti = 1. / (phi.max() - phi.min()), 1. / (theta.max() - theta.min())
+    for ir in range(nrad):
+        for ip in range(nphi):
+            wr[:, ip, ir] = f6(dti, sin_t * up[:, ip, ir], 0., 0., 1, 0) / r[ir] / sin_t
+        for it in range(nth):
+            b1, b2 = ut[it, nphi - 4:, ir], ut[it, 0:4, ir]
+            wr[it, :, ir] = wr[it, :, ir] - f6(dpi, ut[it, :, ir], b1, b2, 3, 0) / (r[ir] * sin_t[it])
+    assert np.abs(got - wr).max() < 1e-11 * np.abs(wr).max()
+
+
+@pytest.mark.parametrize("method", ["adv", "scale", "turn", "ens"])
+def test_rossby_from_checkpoint(tmp_path, method):
+    """rossby_from_checkpoint.py:95-190 on a written checkpoint, against the script's arithmetic
+    on the same data with the oracle's compact_fd6 restatement and the Simpson shell average."""
+    from cssx import analysis
+    from cssx import checkpoint as ck
+    print(f"rossby {method}", flush=True)
+    rng = np.random.default_rng(7)
+    nr, nphi, nth = 9, 16, 13
+    st = rng.standard_normal((8, nr, nphi, nth)) * 0.1
+    st[0] = 1.0 + 0.2 * np.linspace(1, 0, nr)[:, None, None] + 0.01 * rng.standard_normal((nr, nphi, nth))
+    st[4] = 0.3 + 0.01 * rng.standard_normal((nr, nphi, nth))
+    ck.write(str(tmp_path) + "/run/Checkpoints/", 3, st, deltat=1e-3, sim_time=1.0, r1=0.72, r2=0.97,
+             th1=np.pi / 4, th2=3 * np.pi / 4, phi1=0.0, phi2=np.pi / 2)
+    base = str(tmp_path) + "/"
+    om = 2.6e-6
+    got = analysis.rossby_from_chk(3, "run", om, method=method, base_dir=base)
+    f6 = lambda *a, **k: oracle.compact_fd6_py(*a, **k)[0]
+    if method == "ens":
+        curl, rad, theta, phi = analysis.vectorcurl(3, "run", base_dir=base)
+        ref_curl = _curl_loops(analysis.read_checkpoint(3, "run", vel_curl=True, base_dir=base)[0], rad, theta, phi)
+        assert np.abs(curl - ref_curl).max() < 1e-11 * np.abs(ref_curl).max()
+        ens = np.sqrt((ref_curl ** 2).sum(axis=3))
+        ref = analysis.shell_avg(ens, theta, phi, numq=1).reshape(nr) / (2 * om)
+    else:
+        data, rad, theta, phi, *_ = analysis.read_checkpoint(3, "run", vel_curl=(method == "adv"), base_dir=base)
+        avg = analysis.shell_avg(data if method == "adv" else data[..., :5], theta, phi)
+        if method == "adv":
+            ref = np.sqrt(avg[:, 2] ** 2) / (2 * (rad[-1] - rad[0]) * om)
+        elif method == "scale":
+            H = -avg[:, 0] / f6(1. / (rad[-1] - rad[0]), avg[:, 0], 0., 0., 1, 0)
+            ref = np.sqrt(avg[:, 3] ** 2) / (2 * H * om)
+        else:
+            gamma, Cp = 1.525, 3.5e8
+            logp = gamma * np.log(avg[:, 0]) + avg[:, 4] / (Cp / gamma)
+            H = -1. / f6(1. / (rad[1] - rad[0]), logp, 0., 0., 1, 0)
+            ref = 2 * np.pi / (1.4 * H / np.sqrt(avg[:, 3] ** 2) * om)
+    assert got.shape == (nr,)
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()

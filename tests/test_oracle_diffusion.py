@@ -16,8 +16,8 @@ No reference test touches the diffusion, so the oracle is pinned by what the sch
     -(1/2) B . curl(dA/dt) / (4 pi rho T) (:430-461, the curl checked against the RHS's B);
   * the SLDiffusion branch of Step_RK4 / Step_RK3_TVD / Step_RK2_TVD adds exactly dt * D to
     the step (step_rk4.f90:366-383, :197-214, :60-77).
-The wall ghost rules of Diffusion_R/Th (diffusion.f90:586-711, 1008-1120) are restated, not
-pinned by a closed form: only the Fortran could pin them (parity unpinned).
+The wall ghost rules of Diffusion_R/Th (diffusion.f90:586-711, 1008-1120) are pinned in
+test_oracle_sld_walls.py.
 """
 import numpy as np
 import pytest
