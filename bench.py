@@ -131,7 +131,7 @@ def run_c5(args):
     fc = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=args.sld_coef, seed_s=5, seed_a=6)
     fine = Solver(Model(fc))
     coarse = Solver(Model(CL.coarse_config(fc, factor)))
-    y = torch.as_tensor(fine.model.initial_state(), device="cuda").contiguous()
+    y = fine.model.initial_state(device="cuda")
     dt = fine.cfl(y)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -168,7 +168,7 @@ def run_c5(args):
     # FP32 fluctuation state the coarse run hands it, and on an FP64 absolute state
     from cssx.mhd import Solver
     cm = Model(CL.coarse_config(fc, factor))
-    yc = torch.as_tensor(cm.initial_state(), device="cuda").contiguous()
+    yc = cm.initial_state(device="cuda")
     solc = Solver(cm)
     y32 = solc.to_f32(yc)
     solc.close()
@@ -416,7 +416,7 @@ def run_mhd(args, shape, seeds, study_steps, full):
     c = Config(nr=shape[0], nth=shape[1], nphi=shape[2], sld_coef=0.0 if lap else args.sld_coef, seed_s=seeds[0],
                seed_a=seeds[1], Laplacian=lap, kappa_value=args.kappa, SLDiffusion=not lap)
     m = Model(c)
-    y0 = torch.as_tensor(m.initial_state(), device="cuda").contiguous()
+    y0 = m.initial_state(device="cuda")
     sol = Solver(m)
     dt = sol.cfl(y0)
     npts = c.npts

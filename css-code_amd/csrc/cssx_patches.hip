@@ -352,6 +352,17 @@ __global__ __launch_bounds__(256) void k_patch_smooth(const T *__restrict__ in, 
 
 using namespace cssx;
 
+extern "C" int cssx_normal_fill(void *out, int prec, long n, unsigned long long seed, void *stream) {
+    if (!out || n < 1 || (prec != CSSX_PREC_F64 && prec != CSSX_PREC_F32)) return CSSX_EINVAL;
+    const unsigned g = (unsigned)((n + 255) / 256);
+    hipStream_t s = (hipStream_t)stream;
+    if (prec == CSSX_PREC_F64)
+        hipLaunchKernelGGL(k_patch_noise<double>, dim3(g), dim3(256), 0, s, (double *)out, n, seed);
+    else
+        hipLaunchKernelGGL(k_patch_noise<float>, dim3(g), dim3(256), 0, s, (float *)out, n, seed);
+    return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
+}
+
 extern "C" int cssx_patch_fill(void *out, void *tmp, int prec, int nr, int nth, int nphi, unsigned long long seed,
                                void *stream) {
     if (!out || !tmp || nr < 1 || nth < 1 || nphi < 1 || (prec != CSSX_PREC_F64 && prec != CSSX_PREC_F32))

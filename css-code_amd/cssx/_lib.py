@@ -82,6 +82,7 @@ SIGNATURES = {
     "cssx_patchset_autotune": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(ctypes.c_int), _fp], ctypes.c_int),
     "cssx_patch_fill": ([_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _vp],
                         ctypes.c_int),
+    "cssx_normal_fill": ([_vp, ctypes.c_int, ctypes.c_long, ctypes.c_ulonglong, _vp], ctypes.c_int),
     # learned closure
     "cssx_mlp_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong,
                          ctypes.POINTER(_vp)], ctypes.c_int),

@@ -408,7 +408,7 @@ def compare(fine_cfg: Config, factor: int, net: Network | None, coarse_steps: in
     closed = Solver(cmodel) if net is not None else None
     if closed is not None:
         _lib.check(_lib.lib().cssx_mhd_set_closure(closed._h, net.handle), "cssx_mhd_set_closure")
-    y = y_fine if y_fine is not None else torch.as_tensor(fine.model.initial_state(), device="cuda").contiguous()
+    y = y_fine if y_fine is not None else fine.model.initial_state(device="cuda")
     dtf = fine.cfl(y)
     yc0 = filter_state(y, fine, plain, factor)
     dtc = min(factor * dtf, plain.cfl(yc0))

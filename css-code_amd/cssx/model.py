@@ -248,8 +248,14 @@ class Model:
         sp[nr + 1] = c.sld_speedh + c.sld_speedr
         self.diffusion_speed = sp ** 2
 
-    def initial_state(self) -> np.ndarray:
-        """vars0 (nv, nr, nphi, nth): isentropic rho, zero velocity, perturbed entropy, random A."""
+    def initial_state(self, device=None):
+        """vars0 (nv, nr, nphi, nth): isentropic rho, zero velocity, perturbed entropy, random A.
+
+        device=None builds it with numpy on the host; a torch CUDA device builds the same recipe
+        on that GPU (the normals from the same SplitMix64 stream by cssx_normal_fill, equal to
+        the host's to rounding) and returns a contiguous FP64 tensor there."""
+        if device is not None:
+            return self._initial_state_device(device)
         c = self.c
         nr, nth, nphi = c.nr, c.nth, c.nphi
         st = np.zeros((c.nv, nr, nphi, nth))
@@ -264,6 +270,36 @@ class Model:
             for comp in range(3):
                 xa = normal(c.seed_a * 1000003 + comp, c.npts).reshape(nr, nphi, nth)
                 st[5 + comp] = c.bamp * c.r2 * xa * wa
+        return st
+
+    def _initial_state_device(self, device):
+        import torch
+        c = self.c
+        nr, nth, nphi = c.nr, c.nth, c.nphi
+        dev = torch.device(device)
+        st = torch.zeros((c.nv, nr, nphi, nth), dtype=torch.float64, device=dev)
+        stream = ctypes.c_void_p(int(torch.cuda.current_stream(dev).cuda_stream))
+
+        def normals(seed, out):
+            _lib.check(_lib.lib().cssx_normal_fill(out.data_ptr(), _lib.PREC_F64, out.numel(), seed, stream),
+                       "cssx_normal_fill")
+            return out
+
+        def prof(a):
+            return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)
+
+        st[0] = prof(self.rho_bar)[:, None, None]
+        xth = np.pi * (self.theta - c.th1) / (c.th2 - c.th1)
+        yr = np.pi * (self.r - c.r1) / (c.r2 - c.r1)
+        win = prof(1 - np.cos(yr) ** 16)[:, None, None] * prof(1 - np.cos(xth) ** 16)[None, None, :]
+        xi = normals(c.seed_s, torch.empty((nr, nphi, nth), dtype=torch.float64, device=dev))
+        st[4] = prof(self.s_bar)[:, None, None] * (1.0 + c.samp * xi * win)
+        del xi
+        if c.magnetic:
+            wa = prof(1 - np.cos(yr) ** 2)[:, None, None] * prof(1 - np.cos(xth) ** 2)[None, None, :]
+            for comp in range(3):
+                a = normals(c.seed_a * 1000003 + comp, st[5 + comp])
+                a.mul_(c.bamp * c.r2).mul_(wa)
         return st
 
     # ---- C-ABI structs -------------------------------------------------------------------------

@@ -131,7 +131,7 @@ class Ensemble:
             c = Config(nr=nr, nth=nth, nphi=nphi, sld_coef=float(self.sld[k]), seed_s=seed * 1000 + k,
                        seed_a=seed * 1000 + members + k)
             sol = Solver(Model(c))
-            y = torch.as_tensor(sol.model.initial_state(), device="cuda").contiguous()
+            y = sol.model.initial_state(device="cuda")
             self.solvers.append(sol)
             self.states.append(y)
             self.dts.append(sol.cfl(y))

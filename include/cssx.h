@@ -257,6 +257,9 @@ void cssx_patchset_destroy(cssx_patchset *ps);
 int cssx_patchset_run(cssx_patchset *ps, int variant, void *stream);
 /* times each variant (reps runs), keeps the fastest; synchronizes.  ms_out (4 floats) may be NULL */
 int cssx_patchset_autotune(cssx_patchset *ps, int reps, void *stream, int *best, float *ms_out);
+/* n seeded N(0,1) values (SplitMix64 + Box-Muller over the flat index; no reference
+ * counterpart): the synthetic initial perturbations generated in place on the device */
+int cssx_normal_fill(void *out, int prec, long n, unsigned long long seed, void *stream);
 /* seeded N(0,1) field (SplitMix64 + Box-Muller over the flat index) smoothed by [1,2,1]/4 along
  * each axis (edge values repeated); tmp: scratch of the same size */
 int cssx_patch_fill(void *out, void *tmp, int prec, int nr, int nth, int nphi, unsigned long long seed,

@@ -134,30 +134,63 @@ def test_deriv_field_matches_oracle(cssx_mod, shape, case):
     ops.close()
 
 
-def test_deriv_analytic_sphere_c1(cssx_mod):
-    """Config-1 style field on a uniform grid: derivatives against the analytic ones."""
-    _, O = cssx_mod
-    nr, nth, nphi = 64, 64, 128
-    r = np.linspace(0.72, 0.97, nr)
-    th = np.linspace(np.pi / 4, 3 * np.pi / 4, nth)
-    ph = np.arange(nphi) * (np.pi / 2) / nphi
-    R, P, T = np.meshgrid(r, ph, th, indexing="ij")
-    f = np.sin(9 * R + 0.3) * np.sin(3 * T) * np.cos(4 * P)
-    fr = 9 * np.cos(9 * R + 0.3) * np.sin(3 * T) * np.cos(4 * P)
-    ft_ = 3 * np.sin(9 * R + 0.3) * np.cos(3 * T) * np.cos(4 * P)
-    fp = -4 * np.sin(9 * R + 0.3) * np.sin(3 * T) * np.sin(4 * P)
-    dri = (nr - 1) / (r[-1] - r[0])
-    dti = (nth - 1) / (th[-1] - th[0])
-    dpi = nphi / (np.pi / 2)
-    ops = O.Operators(nr, nth, nphi, dri, dti, dpi)
-    x = torch.as_tensor(f, device="cuda")
-    gr = ops.dbydr(x, 1).cpu().numpy()
-    gt = ops.dbydt(x, 1).cpu().numpy()
-    gp = ops.dbydp(x).cpu().numpy()
-    assert np.abs(gr - fr).max() < 2e-6 * np.abs(fr).max()
-    assert np.abs(gt - ft_).max() < 2e-6 * np.abs(ft_).max()
-    assert np.abs(gp - fp).max() < 1e-8 * np.abs(fp).max()
+def test_deriv_c1_recipe(cssx_mod):
+    """The plain-derivative workload (64 x 64 x 128, tanh-stretched r on [0.72, 0.97], the
+    seeded 8-term field of tests/c1_field.py, phi halos taken from the field): cssx_deriv on
+    every axis against the FP64 oracle (1e-12; FP32 3e-5) and the analytic derivatives (the
+    bounds the oracle itself meets), with one-sided walls (code 1) and with Neumann walls
+    carrying the exact wall derivative (code 4); and the compact_fd6 drop-in on every r and
+    theta pencil and on sampled phi pencils (dtype 3, halo ghosts) against the CSS-Python
+    restatement."""
+    import c1_field
+    D, O = cssx_mod
+    c = c1_field.build()
+    f = c["f"]
+    nr, nphi, nth = f.shape
+    dxdr = c["dxdr"][:, None, None]
+    ref = {
+        0: oracle.deriv_field(0, 1, f, bdry=1, scale=c["dri"]),
+        1: oracle.deriv_field(1, 1, f, bdry=1, scale=c["dti"]),
+        2: oracle.deriv_field(2, 1, f, lo_wall=False, hi_wall=False, ghost_lo=c["ghost_lo"],
+                              ghost_hi=c["ghost_hi"], scale=c["dpi"]),
+    }
+    exact = {0: c["fr"], 1: c["ft"], 2: c["fp"]}
+    bound = {0: 5e-6, 1: 1e-5, 2: 1e-9}
+    ops = O.Operators(nr, nth, nphi, c["dri"], c["dti"], c["dpi"])
+    for prec, tol in ((torch.float64, 1e-12), (torch.float32, 3e-5)):
+        x = torch.as_tensor(f, dtype=prec, device="cuda")
+        gl = torch.as_tensor(c["ghost_lo"], dtype=prec, device="cuda")
+        gh = torch.as_tensor(c["ghost_hi"], dtype=prec, device="cuda")
+        got = {0: ops.dbydr(x, 1), 1: ops.dbydt(x, 1), 2: ops.dbydp(x, gl, gh)}
+        for ax in range(3):
+            g = got[ax].double().cpu().numpy()
+            assert _rel(g, ref[ax]) < tol, (ax, prec, _rel(g, ref[ax]))
+            phys = g * dxdr if ax == 0 else g
+            assert _rel(phys, exact[ax]) < bound[ax] + 2 * tol, (ax, prec, _rel(phys, exact[ax]))
+    # Neumann walls with the exact wall derivatives in slot 0 of the r ghost buffers
+    gl = np.zeros((4, nphi, nth))
+    gh = np.zeros((4, nphi, nth))
+    gl[0] = c["fr"][0] / c["dxdr"][0]
+    gh[0] = c["fr"][-1] / c["dxdr"][-1]
+    r4 = oracle.deriv_field(0, 1, f, bdry=4, ghost_lo=gl, ghost_hi=gh, scale=c["dri"], hneu=c["dri"])
+    g4 = ops.dbydr(torch.as_tensor(f, device="cuda"), 4, torch.as_tensor(gl, device="cuda"),
+                   torch.as_tensor(gh, device="cuda")).cpu().numpy()
+    assert _rel(g4, r4) < 1e-12
+    assert _rel(g4 * dxdr, c["fr"]) < 2e-8
     ops.close()
+    # the CSS-Python drop-in: r pencils (axis 0) and theta pencils (axis 2), walls, ibc 1
+    pr_ = D.compact_fd6(1.0, f, 0.0, 0.0, 1, 0, axis=0)
+    assert _rel(pr_, ref[0]) < 1e-12
+    pt_ = D.compact_fd6(1.0 / (np.pi / 2), f, 0.0, 0.0, 1, 0, axis=2)
+    assert _rel(pt_, ref[1]) < 1e-12
+    # phi pencils with their halos (dtype 3: the Python's halo-row scaling included)
+    pp_ = D.compact_fd6(1.0 / (np.pi / 2), f, c["ghost_lo"], c["ghost_hi"], 0, 3, axis=1)
+    rng = np.random.default_rng(12)
+    for _ in range(24):
+        ir, it = rng.integers(0, nr), rng.integers(0, nth)
+        want, _ = oracle.compact_fd6_py(1.0 / (np.pi / 2), f[ir, :, it], c["ghost_lo"][ir, :, it],
+                                       c["ghost_hi"][ir, :, it], 0, 3)
+        assert np.abs(pp_[ir, :, it] - want).max() < 1e-12 * np.abs(want).max()
 
 
 LONG_CASES = []

@@ -193,6 +193,46 @@ def test_c2_shape_step_finite_and_fp32_close(mhd):
     sol.close()
 
 
+def test_device_initial_state_matches_host():
+    """The GPU-built initial state (cssx_normal_fill + the same recipe) against numpy's, on an
+    uneven shape: equal to the rounding of the device sin/cos/log (1e-14 of each field)."""
+    m = Model(Config(nr=24, nth=40, nphi=33, seed_s=9, seed_a=10))
+    host = m.initial_state()
+    dev = m.initial_state(device="cuda").cpu().numpy()
+    for iv in range(host.shape[0]):
+        sc = max(np.abs(host[iv]).max(), 1e-300)
+        assert np.abs(dev[iv] - host[iv]).max() <= 1e-14 * sc, iv
+    # the perturbations themselves (not just the background) agree
+    pert = host[4] - m.s_bar[:, None, None]
+    assert np.abs((dev[4] - m.s_bar[:, None, None]) - pert).max() <= 1e-9 * np.abs(pert).max()
+
+
+def test_c4_shape_two_steps_fp32_within_1e5(mhd):
+    """The 256 x 512 x 1024 workload at full size: two RK4 steps (SLD on) in FP64 and in the
+    FP32-bulk path from the same state; finite, and every field of the FP32 state within 1e-5
+    of the FP64 state's max norm."""
+    print("c4 two steps", flush=True)
+    c = Config(nr=256, nth=512, nphi=1024, sld_coef=0.1, seed_s=5, seed_a=6)
+    m = Model(c)
+    st = m.initial_state(device="cuda")
+    sol = mhd(m)
+    dt = sol.cfl(st)
+    y64 = st.clone()
+    sol.rk4(y64, dt, 2, "f64")
+    y32 = st.clone()
+    sol.rk4(y32, dt, 2, "f32")
+    torch.cuda.synchronize()
+    assert torch.isfinite(y64).all() and torch.isfinite(y32).all()
+    for iv in range(8):
+        err = (y32[iv] - y64[iv]).abs().max().item()
+        sc = y64[iv].abs().max().item()
+        print(f"c4 field {iv}: max|y32-y64|/max|y64| = {err / sc:.2e}", flush=True)
+        assert err <= 1e-5 * sc, (iv, err, sc)
+    sol.close()
+    del st, y64, y32
+    torch.cuda.empty_cache()
+
+
 def test_conservation_totals():
     """E, L, m, mrsint of Compute_Step_Size (rhs_mpi.f90:156-183) against the oracle, which the
     CPU tests pin by closed forms, and m of a uniform density against the exact shell-sector

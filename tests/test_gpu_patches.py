@@ -72,6 +72,74 @@ def test_patch_derivatives_match_oracle(prec, tol, bdry):
     ch.close()
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 3e-5)])
+@pytest.mark.parametrize("bdry", [1, 4])
+def test_patch_long_pencils_match_oracle(prec, tol, bdry):
+    """Axis lengths up to the workload's maximum of 512 on every axis: theta items of 34-64
+    lanes (MR = 8), and strided r / phi items of 16-32 segments."""
+    from cssx.patches import PatchChunk
+    print(f"long patches {prec} bdry={bdry}", flush=True)
+    shapes = [(512, 12, 10), (10, 512, 12), (12, 10, 512), (37, 300, 33), (450, 8, 17), (9, 449, 270)]
+    ch = PatchChunk(shapes, prec=prec, bdry=bdry)
+    rng = np.random.default_rng(100 + bdry)
+    fs = []
+    for i, (nr, nth, nphi) in enumerate(shapes):
+        f = rng.standard_normal((nr, nphi, nth))
+        ch.field(i).copy_(torch.as_tensor(f))
+        fs.append(ch.field(i).double().cpu().numpy())
+    refs = [_oracle_derivs(f, bdry) for f in fs]
+    for variant in range(4):
+        for t in ch.out:
+            t.fill_(float("nan"))
+        ch.run(variant)
+        torch.cuda.synchronize()
+        for i in range(len(shapes)):
+            for ax in range(3):
+                got = ch.field(i, ax).double().cpu().numpy()
+                ref = refs[i][ax]
+                err = np.abs(got - ref).max() / np.abs(ref).max()
+                assert err < tol, (variant, shapes[i], ax, err)
+    ch.close()
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_ensemble_member_in_graph_matches_member_alone(precision):
+    """Members of a 48 x 96 x 96 ensemble stepped by the captured graph (four members on
+    concurrent side streams) against the same members built as stand-alone solvers and stepped
+    alone on the default stream (bit for bit), and, in FP64, against the oracle's Step_RK4
+    (1e-11 of each field's max norm)."""
+    from cssx.mhd import Solver
+    from cssx.model import Model
+    from cssx.patches import Ensemble
+    print(f"ensemble member vs alone {precision}", flush=True)
+    ens = Ensemble(members=4, shape=(48, 96, 96), seed=4, precision=precision)
+    y0 = [y.clone() for y in ens.states]
+    ens.capture()
+    # capture steps once eagerly (warm-up) and leaves the states one step on; restart them
+    for y, y_init in zip(ens.states, y0):
+        y.copy_(y_init)
+    ens.replay(1)
+    torch.cuda.synchronize()
+    for k in (0, 3):
+        cfg = ens.solvers[k].model.c
+        alone = Solver(Model(cfg))
+        y = alone.model.initial_state(device="cuda")
+        assert torch.equal(y, y0[k])
+        dt = alone.cfl(y)
+        assert dt == ens.dts[k]
+        alone.rk4(y, dt, 1, precision)
+        torch.cuda.synchronize()
+        assert torch.equal(y, ens.states[k]), (k, (y - ens.states[k]).abs().max().item())
+        if precision == "f64":
+            ref = oracle.rk4(alone.model, y0[k].cpu().numpy(), dt, 1)
+            got = ens.states[k].cpu().numpy()
+            for iv in range(ref.shape[0]):
+                err = np.abs(got[iv] - ref[iv]).max() / np.abs(ref[iv]).max()
+                assert err < 1e-11, (k, iv, err)
+        alone.close()
+    ens.close()
+
+
 def test_ensemble_graph_replays_match_eager():
     """Graph replays of one ensemble step (members on concurrent side streams) and eager
     multi-stream steps, each bit-identical to stepping every member on one stream."""

@@ -225,3 +225,45 @@ def test_reference_known_answer_sin(ibc):
     res, _ = oracle.compact_fd6_py(dri, f, 0.0, 0.0, ibc, 0, df)
     l2 = np.sqrt(dr * np.sum((res - exact) ** 2))
     assert l2 < 1e-6, l2
+
+
+def test_c1_recipe_oracle_against_analytic_and_python():
+    """The plain-derivative workload (64 x 64 x 128, tanh-stretched r, seeded 8-term field):
+    the oracle's d/dr (times dx/dr), d/dtheta and d/dphi (phi halos from the field itself)
+    against the analytic derivatives, with the one-sided wall closures (code 1) and with the
+    exact wall derivatives as Neumann values (code 4); and the CSS-Python compact_fd6
+    restatement on sampled r and theta pencils (dtype 0, where it equals the Fortran rows)."""
+    import c1_field
+    c = c1_field.build()
+    f = c["f"]
+    gr = oracle.deriv_field(0, 1, f, bdry=1, scale=c["dri"]) * c["dxdr"][:, None, None]
+    gt = oracle.deriv_field(1, 1, f, bdry=1, scale=c["dti"])
+    gp = oracle.deriv_field(2, 1, f, lo_wall=False, hi_wall=False, ghost_lo=c["ghost_lo"], ghost_hi=c["ghost_hi"],
+                            scale=c["dpi"])
+
+    def rel(a, b):
+        return np.abs(a - b).max() / np.abs(b).max()
+
+    # one-sided closures (3rd/4th order) dominate the wall rows; the interior is 6th order
+    assert rel(gr, c["fr"]) < 5e-6 and rel(gr[4:-4], c["fr"][4:-4]) < 1e-7
+    assert rel(gt, c["ft"]) < 1e-5 and rel(gt[..., 4:-4], c["ft"][..., 4:-4]) < 1e-7
+    assert rel(gp, c["fp"]) < 1e-9
+    # an error of one mode's sign or a dropped metric factor is O(1)
+    assert rel(oracle.deriv_field(0, 1, f, bdry=1, scale=c["dri"]), c["fr"]) > 0.1
+    # Neumann walls with the exact wall derivatives (slot 0 of the ghost buffers)
+    nr, nphi, nth = f.shape
+    gl = np.zeros((4, nphi, nth))
+    gh = np.zeros((4, nphi, nth))
+    gl[0] = c["fr"][0] / c["dxdr"][0]
+    gh[0] = c["fr"][-1] / c["dxdr"][-1]
+    g4 = oracle.deriv_field(0, 1, f, bdry=4, ghost_lo=gl, ghost_hi=gh, scale=c["dri"], hneu=c["dri"])
+    assert rel(g4 * c["dxdr"][:, None, None], c["fr"]) < 2e-8
+    # CSS-Python compact_fd6 (dri = 1 / domain length in index space, dli = n - 1 inside)
+    rng = np.random.default_rng(11)
+    for _ in range(16):
+        ip, it = rng.integers(0, nphi), rng.integers(0, nth)
+        pr_, _ = oracle.compact_fd6_py(1.0, f[:, ip, it], 0.0, 0.0, 1, 0)
+        assert np.abs(pr_ * c["dxdr"] - gr[:, ip, it]).max() < 1e-10 * np.abs(gr).max()
+        ir = rng.integers(0, nr)
+        pt, _ = oracle.compact_fd6_py(1.0 / (np.pi / 2), f[ir, ip, :], 0.0, 0.0, 1, 0)
+        assert np.abs(pt - gt[ir, ip, :]).max() < 1e-10 * np.abs(gt).max()
