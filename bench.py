@@ -188,17 +188,32 @@ def run_c5(args):
     infer_tf = 90112.0 * cm.c.npts / (infer_ms * 1e-3) / 1e12
     del snaps, ts, train_snaps, yc, y32
     torch.cuda.empty_cache()
-    recs, times = CL.compare(fc, factor, net, args.c5_steps, args.c5_stat_every, "f32")
+    def _progress(r):
+        print(f"c5: coarse step {r['step']} ke fine/plain/closure "
+              f"{r['filtered_fine']['ke']:.4e} {r['coarse']['ke']:.4e} {r.get('coarse_closure', {}).get('ke', 0.0):.4e}",
+              file=sys.stderr, flush=True)
+
+    recs, times = CL.compare(fc, factor, net, args.c5_steps, args.c5_stat_every, "f32", progress=_progress)
     ep, kp = CL.stat_errors(recs, "coarse")
     ec, kc = CL.stat_errors(recs, "coarse_closure")
+    sp_err = CL.scalar_errors(recs, "coarse")
+    sc_err = CL.scalar_errors(recs, "coarse_closure")
     flops_train = 3 * 90112.0 * n * len(hist)
     ach = flops_train / t_train / 1e12
-    speedup = times["fine_s"] / max(times["coarse_closure_s"], 1e-12)
-    worst_c = float(max(ec.max(), kc))
+    ratio = times["fine_s"] / max(times["coarse_closure_s"], 1e-12)
+    worst_c = float(max(ec.max(), kc, max(sc_err.values())))
+    worst_p = float(max(ep.max(), kp, max(sp_err.values())))
+    matched = worst_c <= 0.05
+    # the closure speedup is defined at matched statistics; an unmatched run reports no value,
+    # only its wall-time ratio and by how much the statistics miss
     out = {
         "metric": METRIC + " [closure speedup]",
-        "value": round(speedup, 2),
-        "unit": "x (wall filtered-fine / wall coarse+closure)",
+        "value": round(ratio, 2) if matched else None,
+        "unit": "x (wall filtered-fine / wall coarse+closure, statistics within 5%)",
+        "matched": matched,
+        "target": "coarse+closure statistics within 5% of the filtered fine run, >= 10x less wall time",
+        "worst_stat_rel_err": {"coarse_closure": round(worst_c, 4), "coarse_plain": round(worst_p, 4)},
+        "wall_time_ratio_unmatched": None if matched else round(ratio, 2),
         "n_gpus": 1, "steps": args.c5_steps, "warmup": 0, "higher_is_better": True, "scaling": "strong",
         "vs_baseline": None, "dtype": "bf16 network (f32 accumulate), f32-bulk solver", "data": "synthetic",
         "config": {"workload": f"closure_{fc.nr}x{fc.nth}x{fc.nphi}_to_{fc.nr // 4}x{fc.nth // 4}x{fc.nphi // 4}",
@@ -209,9 +224,11 @@ def run_c5(args):
         "wall_s": {"fine_run_and_filter": round(times["fine_s"], 3), "coarse_plain": round(times["coarse_plain_s"], 3),
                    "coarse_closure": round(times["coarse_closure_s"], 3), "training_data": round(t_data, 3),
                    "training": round(t_train, 3)},
-        "stats_max_rel_err": {"coarse_plain": {"fields": [float(f"{x:.3e}") for x in ep], "ke": float(f"{kp:.3e}")},
-                              "coarse_closure": {"fields": [float(f"{x:.3e}") for x in ec], "ke": float(f"{kc:.3e}")},
-                              "closure_within_5pct": worst_c <= 0.05},
+        "stats_max_rel_err": {"coarse_plain": {"fields": [float(f"{x:.3e}") for x in ep], "ke": float(f"{kp:.3e}"),
+                                               "scalar_output": {k: float(f"{v:.3e}") for k, v in sp_err.items()}},
+                              "coarse_closure": {"fields": [float(f"{x:.3e}") for x in ec], "ke": float(f"{kc:.3e}"),
+                                                 "scalar_output": {k: float(f"{v:.3e}") for k, v in sc_err.items()}},
+                              "closure_within_5pct": matched},
         "apriori_residual_rms": {"fields": list(CL.FIELDS), "gate": [bool(x) for x in mask],
                                  "held_out": {"validation": val_id, "test": test_id},
                                  "plain": [float(f"{x:.3e}") for x in before],
@@ -337,7 +354,7 @@ def main():
     ap.add_argument("--c5-every", type=int, default=20, help="closure: fine steps between snapshots")
     ap.add_argument("--c5-samples", type=int, default=2_000_000, help="closure: training rows")
     ap.add_argument("--c5-epochs", type=int, default=10)
-    ap.add_argument("--c5-steps", type=int, default=100, help="closure: coarse steps of the comparison runs")
+    ap.add_argument("--c5-steps", type=int, default=1000, help="closure: coarse steps of the comparison runs")
     ap.add_argument("--c5-stat-every", type=int, default=10)
     ap.add_argument("--c5-sld", type=int, default=1,
                     help="closure targets include each grid's SLD diffusion (1) or the RHS alone (0)")

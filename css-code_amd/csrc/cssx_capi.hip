@@ -22,17 +22,7 @@ std::mutex g_plan_mu;
 std::map<std::tuple<int, PlanKey, int>, void *> g_plans;
 }  // namespace
 
-int get_plan(const PlanKey &key, int prec, const void **out) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return CSSX_EHIP;
-    std::lock_guard<std::mutex> lk(g_plan_mu);
-    auto tk = std::make_tuple(dev, key, prec);
-    auto it = g_plans.find(tk);
-    if (it != g_plans.end()) {
-        *out = it->second;
-        return CSSX_OK;
-    }
-    std::vector<double> plan;
+int host_plan(const PlanKey &key, std::vector<double> &plan) {
     if (key.variant == 1) {
         const int MR = key.seg;
         if (MR != 4 && MR != 8 && MR != 16) return CSSX_EINVAL;
@@ -46,6 +36,22 @@ int get_plan(const PlanKey &key, int prec, const void **out) {
         if (M <= 0 || seg_count(key.n, M) > kMaxSeg) return CSSX_EINVAL;
         build_plan(key, seg_count(key.n, M), plan);
     }
+    return CSSX_OK;
+}
+
+int get_plan(const PlanKey &key, int prec, const void **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CSSX_EHIP;
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto tk = std::make_tuple(dev, key, prec);
+    auto it = g_plans.find(tk);
+    if (it != g_plans.end()) {
+        *out = it->second;
+        return CSSX_OK;
+    }
+    std::vector<double> plan;
+    int rc = host_plan(key, plan);
+    if (rc) return rc;
     void *d = nullptr;
     size_t bytes = plan.size() * (prec == CSSX_PREC_F64 ? sizeof(double) : sizeof(float));
     if (hipMalloc(&d, bytes) != hipSuccess) return CSSX_ENOMEM;
@@ -110,37 +116,58 @@ static void fill_launch_geom(DerivLaunch<T> &L, const AxisGeom &g) {
     L.njobs = 0;
 }
 
+// Kernel family of a batched launch: 2 = the wide kernel (long strided pencils, several
+// segments per wavefront), 1 = the contiguous kernel, 0 = the strided kernel; seg = its rows
+// per segment (or per lane).  The wide and contiguous D2 upper-wall closures need the last
+// segment to own >= 2 rows.
+static void launch_family(const DerivSpec *sp, int nspec, const AxisGeom &g, bool f64, int &variant, int &seg) {
+    int WM = 0, WG = 0;
+    if (g.srow != 1) wide_shape(g.n, f64, WM, WG);
+    bool wide = WM > 0;
+    for (int i = 0; i < nspec && wide; i++)
+        if (sp[i].order == 2 && sp[i].hi_kind == END_WALL && g.n % WM == 1) wide = false;
+    bool contig = !wide;
+    for (int i = 0; i < nspec; i++) contig = contig && use_contig(g, sp[i].order, sp[i].hi_kind, f64);
+    variant = wide ? 2 : contig ? 1 : 0;
+    seg = wide ? WM : contig ? choose_MR(g.n, f64) : strided_M(g.n);
+}
+
+PlanKey launch_plan_key(const DerivSpec *specs, int nspec, int i, const AxisGeom &g, bool f64) {
+    int variant = 0, seg = 0;
+    launch_family(specs, nspec, g, f64, variant, seg);
+    const DerivSpec &s = specs[i];
+    PlanKey key{g.n, s.order, s.lo_kind == END_WALL ? END_WALL : END_HALO,
+                s.hi_kind == END_WALL ? END_WALL : END_HALO, (s.lo_kind == END_WALL || s.hi_kind == END_WALL) ? s.bdry : 0};
+    key.variant = variant;
+    key.seg = seg;
+    return key;
+}
+
 template <typename T>
 static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStream_t stream) {
     DerivLaunch<T> L;
     fill_launch_geom(L, g);
-    // long strided pencils: the wide kernel (several segments per wavefront); its D2 upper-wall
-    // closure, like the contiguous kernel's, needs the last segment to own >= 2 rows
-    int WM = 0, WG = 0;
-    if (g.srow != 1) wide_shape(g.n, sizeof(T) == 8, WM, WG);
-    bool wide = WM > 0;
-    for (int i = 0; i < nspec && wide; i++)
-        if ((&sp)[i].order == 2 && (&sp)[i].hi_kind == END_WALL && g.n % WM == 1) wide = false;
-    bool contig = !wide;
     const bool f64 = sizeof(T) == 8;
-    for (int i = 0; i < nspec; i++) contig = contig && use_contig(g, (&sp)[i].order, (&sp)[i].hi_kind, f64);
-    if (contig) { L.M = choose_MR(g.n, f64); L.S = lanes_per_pencil(g.n, L.M); }
-    if (wide) { L.M = WM; L.S = (g.n + WM - 1) / WM; }
+    int variant = 0, seg = 0;
+    launch_family(&sp, nspec, g, f64, variant, seg);
+    const bool wide = variant == 2, contig = variant == 1;
+    if (contig) { L.M = seg; L.S = lanes_per_pencil(g.n, L.M); }
+    if (wide) { L.M = seg; L.S = (g.n + seg - 1) / seg; }
     for (int i = 0; i < nspec; i++) {
         const DerivSpec &s = (&sp)[i];
-        PlanKey key{g.n, s.order, s.lo_kind == END_WALL ? END_WALL : END_HALO,
-                    s.hi_kind == END_WALL ? END_WALL : END_HALO, (s.lo_kind == END_WALL || s.hi_kind == END_WALL) ? s.bdry : 0};
-        key.variant = wide ? 2 : contig ? 1 : 0;
-        key.seg = L.M;
-        const void *plan = nullptr;
-        int rc = get_plan(key, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
-        if (rc) return rc;
+        const PlanKey key = launch_plan_key(&sp, nspec, i, g, f64);
+        const void *plan = s.plan;
+        if (!plan) {
+            int rc = get_plan(key, f64 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
+            if (rc) return rc;
+        }
         DerivJob<T> &J = L.jobs[L.njobs++];
         J.in = (const T *)s.in; J.out = (T *)s.out; J.rhs_out = (T *)s.rhs_out;
         J.glo = (const T *)s.glo; J.ghi = (const T *)s.ghi;
         J.g_k = s.g_k >= 0 ? s.g_k : g.g_k; J.g_a = s.g_k >= 0 ? s.g_a : g.g_a; J.g_b = s.g_k >= 0 ? s.g_b : g.g_b;
         J.plan = (const T *)plan;
         J.pre = (const T *)s.pre; J.post = (const T *)s.post;
+        J.mul_a = (const T *)s.mul_a; J.mul_b = (const T *)s.mul_b;
         J.scale = (T)s.scale; J.neu_mul = (T)s.neu_mul; J.halo_s = (T)s.halo_s;
         J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
         J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind; J.lo_r0_val = s.lo_r0_val;
@@ -380,9 +407,13 @@ __global__ __launch_bounds__(256) void k_compact_small(long npen, double dri, in
     }
 }
 
-extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
-                                const double *b1, const double *b2, double *darr, double *result,
-                                void *stream) {
+// Argument checks, the register-resident kernel for n < 8, and the derivative spec for n >= 8
+// of cssx_compact_fd6.  Returns CSSX_OK with *done set
+// when the short-pencil kernel already ran.
+static int fd6_prepare(int n, long npencils, double dri, int ibc, int dtype, const double *arr, const double *b1,
+                       const double *b2, double *darr, double *result, hipStream_t st, bool *done, DerivSpec &s,
+                       AxisGeom &g) {
+    *done = false;
     if (n < 5 || npencils < 1 || !arr || !result || ibc < 0 || ibc > 5 || dtype < 0 || dtype > 3 ||
         npencils > (long)INT32_MAX)
         return CSSX_EINVAL;
@@ -390,13 +421,13 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     if ((lo_halo && !b1) || (hi_halo && !b2)) return CSSX_EINVAL;
     if (n < 8) {
         const dim3 grid((unsigned)((npencils + 255) / 256));
-        hipStream_t st = (hipStream_t)stream;
         if (n == 5)
             hipLaunchKernelGGL(k_compact_small<5>, grid, dim3(256), 0, st, npencils, dri, ibc, dtype, arr, b1, b2, darr, result);
         else if (n == 6)
             hipLaunchKernelGGL(k_compact_small<6>, grid, dim3(256), 0, st, npencils, dri, ibc, dtype, arr, b1, b2, darr, result);
         else
             hipLaunchKernelGGL(k_compact_small<7>, grid, dim3(256), 0, st, npencils, dri, ibc, dtype, arr, b1, b2, darr, result);
+        *done = true;
         return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
     }
     if (choose_M(n) < 0) return CSSX_EINVAL;
@@ -405,11 +436,9 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     // derivatives.py:82-115); at a lower wall it keeps the Dirichlet-type tridiagonal row
     // (5 is odd, :154-160) but its R0 is the preloaded darr[0] (no formula fills it, :134-135).
     const int bd = (ibc == 5) ? 3 : ibc;
-    AxisGeom g;
     g.phase = CSSX_PHASE_OTHER;
     g.n = n; g.na = (int)npencils; g.nb = 1; g.sa = 1; g.sb = 0; g.srow = npencils;
     g.g_k = npencils; g.g_a = 1; g.g_b = 0;
-    DerivSpec s;
     s.order = 1;
     s.in = arr; s.out = result; s.rhs_out = darr; s.rhs_scale = dli;
     s.lo_kind = lo_halo ? END_HALO : END_WALL;
@@ -422,6 +451,17 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     s.scale = dli * dri;
     s.neu_mul = 1.0 / dli;
     s.halo_s = dli / 60.0;   // derivatives.py:91,141,177,181 multiply the whole halo row by dli/60
+    return CSSX_OK;
+}
+
+extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
+                                const double *b1, const double *b2, double *darr, double *result,
+                                void *stream) {
+    DerivSpec s;
+    AxisGeom g;
+    bool done = false;
+    const int rc = fd6_prepare(n, npencils, dri, ibc, dtype, arr, b1, b2, darr, result, (hipStream_t)stream, &done, s, g);
+    if (rc || done) return rc;
     return run_derivs(&s, 1, g, CSSX_PREC_F64, (hipStream_t)stream);
 }
 

@@ -281,29 +281,54 @@ __global__ __launch_bounds__(1024, (sizeof(T) == 4 && M <= 16) ? 8 : 1) void k_d
     if (!active) return;
     const T *__restrict__ post = J.post;
     const T scale = J.scale, beta = J.beta;
+    const bool has_pm = J.mul_a || J.mul_b;
+    const T pm = (J.mul_a ? J.mul_a[a] : T(1)) * (J.mul_b ? J.mul_b[b] : T(1));
     const __amdgpu_buffer_rsrc_t rout = field_rsrc(J.out + bofs);
+    // with beta != 0 the old values of four rows are loaded ahead of their stores (a store may
+    // alias the next row's load, so a load per store would wait a memory round trip each)
+    constexpr int C = 4;
     if (full) {
 #pragma unroll
-        for (int jj = 0; jj < M; jj++) {
-            const int j = st + jj;
-            // the interface row (last row of a non-final segment) is the interface value itself
-            const T x = (jj < M - 1 || last) ? d[jj] + lane_val(p_vL, jj) * xl + lane_val(p_vR, jj) * xr : xr;
-            T o = scale * x;
-            if (post) o *= lane_val(p_post, jj);
-            if (beta != T(0)) o += beta * bload<T>(rout, lofs, (unsigned)j * rowb);
-            bstore(o, rout, lofs, (unsigned)j * rowb);
+        for (int jc = 0; jc < M; jc += C) {
+            T old[C];
+#pragma unroll
+            for (int t = 0; t < C; t++)
+                old[t] = (beta != T(0) && jc + t < M) ? bload<T>(rout, lofs, (unsigned)(st + jc + t) * rowb) : T(0);
+#pragma unroll
+            for (int t = 0; t < C; t++) {
+                const int jj = jc + t;
+                if (jj < M) {
+                    const int j = st + jj;
+                    // the interface row (last row of a non-final segment) is the interface value
+                    const T x = (jj < M - 1 || last) ? d[jj] + lane_val(p_vL, jj) * xl + lane_val(p_vR, jj) * xr : xr;
+                    T o = scale * x;
+                    if (post) o *= lane_val(p_post, jj);
+                    if (has_pm) o *= pm;
+                    if (beta != T(0)) o += beta * old[t];
+                    bstore(o, rout, lofs, (unsigned)j * rowb);
+                }
+            }
         }
     } else {
         const int mi = last ? m : m - 1;
 #pragma unroll
-        for (int jj = 0; jj < M; jj++) {
-            if (jj < m) {
-                const int j = st + jj;
-                const T x = (jj < mi) ? d[jj] + vL[j] * xl + vR[j] * xr : xr;
-                T o = scale * x;
-                if (post) o *= post[j];
-                if (beta != T(0)) o += beta * bload<T>(rout, lofs, (unsigned)j * rowb);
-                bstore(o, rout, lofs, (unsigned)j * rowb);
+        for (int jc = 0; jc < M; jc += C) {
+            T old[C];
+#pragma unroll
+            for (int t = 0; t < C; t++)
+                old[t] = (beta != T(0) && jc + t < m) ? bload<T>(rout, lofs, (unsigned)(st + jc + t) * rowb) : T(0);
+#pragma unroll
+            for (int t = 0; t < C; t++) {
+                const int jj = jc + t;
+                if (jj < m) {
+                    const int j = st + jj;
+                    const T x = (jj < mi) ? d[jj] + vL[j] * xl + vR[j] * xr : xr;
+                    T o = scale * x;
+                    if (post) o *= post[j];
+                    if (has_pm) o *= pm;
+                    if (beta != T(0)) o += beta * old[t];
+                    bstore(o, rout, lofs, (unsigned)j * rowb);
+                }
             }
         }
     }

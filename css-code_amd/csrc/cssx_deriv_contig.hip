@@ -299,6 +299,8 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     T *__restrict__ out = J.out;
     const T *__restrict__ post = J.post;
     const T scale = J.scale, beta = J.beta;
+    const bool has_pm = J.mul_a || J.mul_b;
+    const T pm = (J.mul_a ? J.mul_a[a] : T(1)) * (J.mul_b ? J.mul_b[b] : T(1));
     T o[MR], tvl[MR], tvr[MR];
     lds_rows<T, MR>(vL, nlT, j, tvl);
     lds_rows<T, MR>(vR, nlT, j, tvr);
@@ -308,6 +310,7 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
         T x = local(jj) ? d[jj] + tvl[jj] * xl + tvr[jj] * xr : xr;
         o[jj] = scale * x;
         if (post) o[jj] *= post[row];
+        if (has_pm) o[jj] *= pm;
     }
     if (beta != T(0)) {
 #pragma unroll

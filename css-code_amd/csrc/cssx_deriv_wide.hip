@@ -220,17 +220,30 @@ __global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
     if (!active) return;
     const T *__restrict__ post = J.post;
     const T scale = J.scale, beta = J.beta;
+    const bool has_pm = J.mul_a || J.mul_b;
+    const T pm = (J.mul_a ? J.mul_a[a] : T(1)) * (J.mul_b ? J.mul_b[b] : T(1));
     const __amdgpu_buffer_rsrc_t rout = field_rsrc(J.out + bofs);
+    // with beta != 0 the old values of four rows are loaded ahead of their stores (a store may
+    // alias the next row's load, so a load per store would wait a memory round trip each)
+    constexpr int C = 4;
 #pragma unroll
-    for (int jj = 0; jj < M; jj++) {
-        if (jj < m) {
-            const int row = st + jj;
-            const T x = (jj < mi) ? d[jj] + vL[pst + jj] * xl + vR[pst + jj] * xr : xr;
-            T o = scale * x;
-            if (post) o *= post[row];
-            const unsigned ofs = lofs + (unsigned)row * rowb;
-            if (beta != T(0)) o += beta * bload<T>(rout, ofs, 0u);
-            bstore(o, rout, ofs, 0u);
+    for (int jc = 0; jc < M; jc += C) {
+        T old[C];
+#pragma unroll
+        for (int t = 0; t < C; t++)
+            old[t] = (beta != T(0) && jc + t < m) ? bload<T>(rout, lofs + (unsigned)(st + jc + t) * rowb, 0u) : T(0);
+#pragma unroll
+        for (int t = 0; t < C; t++) {
+            const int jj = jc + t;
+            if (jj < m) {
+                const int row = st + jj;
+                const T x = (jj < mi) ? d[jj] + vL[pst + jj] * xl + vR[pst + jj] * xr : xr;
+                T o = scale * x;
+                if (post) o *= post[row];
+                if (has_pm) o *= pm;
+                if (beta != T(0)) o += beta * old[t];
+                bstore(o, rout, lofs + (unsigned)row * rowb, 0u);
+            }
         }
     }
 }

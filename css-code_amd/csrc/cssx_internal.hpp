@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "cssx_plan.hpp"
 
 struct cssx_mlp;
@@ -25,10 +27,12 @@ struct DerivSpec {
     const void *glo = nullptr, *ghi = nullptr;
     long g_k = -1, g_a = 0, g_b = 0;      // -1: use the axis default ghost layout
     const void *pre = nullptr, *post = nullptr;
+    const void *mul_a = nullptr, *mul_b = nullptr;   // per-pencil multipliers (fast / slow index)
     double scale = 1.0, neu_mul = 1.0, halo_s = 1.0, rhs_scale = 1.0, beta = 0.0;
     int bdry = 0;
     int lo_kind = END_WALL, hi_kind = END_WALL;
     int lo_r0_val = 0;
+    const void *plan = nullptr;           // device plan to use instead of the plan cache's
 };
 
 // per-phase event timing (cssx_timing.hip); token < 0 = disabled
@@ -41,6 +45,11 @@ int closure_tendency(cssx_mlp *n, int prec, const void *st, int fluct, const voi
 
 void axis_geometry(int axis, int nth, int nphi, int nr, AxisGeom &g);
 int get_plan(const PlanKey &key, int prec, const void **out);
+// the plan key (variant and segment size included) that run_derivs uses for spec s of a
+// launch of `nspec` specs along geometry g
+PlanKey launch_plan_key(const DerivSpec *specs, int nspec, int i, const AxisGeom &g, bool f64);
+// the FP64 plan of a key, as get_plan would upload it
+int host_plan(const PlanKey &key, std::vector<double> &plan);
 int run_derivs(const DerivSpec *specs, int nspec, const AxisGeom &g, int prec, hipStream_t stream);
 
 }  // namespace cssx

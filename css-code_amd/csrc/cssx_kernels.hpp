@@ -19,6 +19,8 @@ struct DerivJob {
     const T *plan;      // device plan buffer (cssx_plan.hpp layout)
     const T *pre;       // optional per-row multiplier applied to the loaded field
     const T *post;      // optional per-row multiplier applied to f' (e.g. dx/dr)
+    const T *mul_a;     // optional multiplier of f' by the pencil's fast index a
+    const T *mul_b;     // optional multiplier of f' by the pencil's slow index b
     T scale;            // 1/dx (order 1) or 1/dx^2 (order 2)
     T neu_mul;          // converts a prescribed wall derivative to index units
     T halo_s;           // multiplier of the stencil part of the halo rows (1 = Fortran)

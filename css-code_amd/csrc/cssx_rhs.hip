@@ -2,7 +2,9 @@
 //
 // RHS(call_num) of rhs_mpi.f90:279-783 (stratified, magnetic, do_radheat=F; with or without the
 // Laplacian / SLaplacian explicit diffusion) is evaluated as:
-//   1. curl A: six batched derivative sweeps + k_curl_combine (Compute_B, derivatives.f90:1830-1892)
+//   1. curl A: six batched derivative sweeps whose epilogues apply the metric factors and add
+//      the second term of each component onto the first, + k_curl_walls on the wall points
+//      (Compute_B, derivatives.f90:1830-1892)
 //   2. k_walls_state: wall values of the closures that depend on the state (rhs_mpi.f90:410-455,
 //      :578-637)
 //   3. the 19 first-derivative sweeps of Source_Theta_Terms / Phi_Terms / Radial_Terms, batched
@@ -23,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -50,6 +53,8 @@ constexpr double kJconst = 4.0 * 3.1415926535897932384626433832795028841972 / (2
 template <typename T>
 struct Prof {
     T *r = nullptr, *rinv = nullptr, *dxdr = nullptr, *grav = nullptr, *eps = nullptr;
+    T *drr = nullptr;   // dx/dr / r: the r sweeps of the curl
+
     T *dr = nullptr;    // cell widths dr_ind (Compute_Step_Size)
     T *sn = nullptr, *csc = nullptr, *cs = nullptr, *cot = nullptr;
     T *dsdr1 = nullptr, *dsdr2 = nullptr, *sr1 = nullptr;
@@ -239,34 +244,39 @@ __device__ __forceinline__ T bound_value(int iv, int it, int ip, int ir, T val, 
 }
 
 // ---------------------------------------------------------------------------------------------
-// 1. curl combine + Compute_B wall conditions (derivatives.f90:2163-2293, :1840-1883)
+// 1. Compute_B wall conditions (derivatives.f90:1840-1883) on the wall points of B = curl A:
+//    at the theta walls B_theta = 0 (perfectly conducting) or B_phi = B_r = 0 (normal field);
+//    at a radial-field r wall B_theta = B_phi = 0.  Thread q < nr*nphi takes the theta-wall
+//    pair of pencil (ip, ir); thread q < nth*nphi the r-wall pair of column (it, ip).
 // ---------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_curl_combine(Consts<T> c, Prof<T> P, const T *__restrict__ C0,
-                                                      const T *__restrict__ C1, const T *__restrict__ C2,
-                                                      const T *__restrict__ C3, const T *__restrict__ C4,
-                                                      const T *__restrict__ C5, T *__restrict__ B, int perf_cond,
-                                                      int bot_radial, int top_radial, int zero_walls) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N) return;
-    int it, ip, ir;
-    coords(i, c, it, ip, ir);
-    const T ri = P.rinv[ir], csc = P.csc[it];
-    T br = C0[i] * csc * ri;
-    T bp = -C1[i] * ri;
-    T bt = ri * C2[i] * csc;
-    br = br - ri * C3[i] * csc;
-    bp = bp + C4[i] * ri;
-    bt = bt - C5[i] * ri;
-    const bool thw = zero_walls && (it == 0 || it == c.nth - 1);
-    if (thw) {
-        if (perf_cond) bt = T(0);
-        else { bp = T(0); br = T(0); }
+__global__ __launch_bounds__(256) void k_curl_walls(Consts<T> c, T *__restrict__ B, int perf_cond, int bot_radial,
+                                                    int top_radial) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long N = c.N, plane = (long)c.nth * c.nphi;
+    if (q < (long)c.nr * c.nphi) {
+        const long row = q * c.nth;   // q = ir * nphi + ip
+        for (int side = 0; side < 2; side++) {
+            const long i = row + (side ? c.nth - 1 : 0);
+            if (perf_cond) {
+                B[i] = T(0);
+            } else {
+                B[N + i] = T(0);
+                B[2 * N + i] = T(0);
+            }
+        }
     }
-    if (zero_walls && ((ir == 0 && bot_radial) || (ir == c.nr - 1 && top_radial))) { bt = T(0); bp = T(0); }
-    B[i] = bt;
-    B[c.N + i] = bp;
-    B[2 * c.N + i] = br;
+    if (q < plane) {
+        if (bot_radial) {
+            B[q] = T(0);
+            B[N + q] = T(0);
+        }
+        if (top_radial) {
+            const long i = (long)(c.nr - 1) * plane + q;
+            B[i] = T(0);
+            B[N + i] = T(0);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -992,11 +1002,13 @@ template <typename T>
 static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const int nr = m->nr, nth = m->nth;
     const std::vector<double> &drbar = m->h_drbar, &dsbar = m->h_dsbar, &hres = m->h_hres;
-    const size_t tot = 24 * (size_t)nr + 10 * (size_t)nth + 2;
-    std::vector<T> h(tot);
+    // the profiles back to back in one host vector that grows with every table put into it
+    std::vector<T> h;
+    h.reserve(32 * (size_t)nr + 12 * (size_t)nth + 2);
     size_t o = 0;
     auto put = [&](size_t n, auto gen) {
         const size_t at = o;
+        h.resize(o + n);
         for (size_t i = 0; i < n; i++) h[o + i] = (T)gen(i);
         o += n;
         return at;
@@ -1005,6 +1017,7 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const size_t o_r = put(nr, [&](size_t i) { return m->h_r[i]; });
     const size_t o_ri = put(nr, [&](size_t i) { return 1.0 / m->h_r[i]; });
     const size_t o_x = put(nr, [&](size_t i) { return m->h_dxdr[i]; });
+    const size_t o_drr = put(nr, [&](size_t i) { return m->h_dxdr[i] / m->h_r[i]; });
     const size_t o_g = put(nr, [&](size_t i) { return m->h_grav[i]; });
     const size_t o_e = put(nr, [&](size_t i) { return m->h_eps[i]; });
     const size_t o_dr = put(nr, [&](size_t i) { return m->h_dr[i]; });
@@ -1054,15 +1067,17 @@ static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const size_t o_eta = put(nr, [&](size_t i) { return opt(m->h_etar, i); });
     const size_t o_w9 = put(nr, [&](size_t i) { return opt(m->h_d2xdr2, i) / m->h_dxdr[i]; });
     const size_t o_d2s = put(nr, [&](size_t i) { return opt(m->h_d2sbar, i); });
+    const size_t tot = h.size();
     T *d = nullptr;
     if (P.buf) {
-        d = P.buf;
+        d = P.buf;   // the same tables as the first upload: the same size
     } else if (hipMalloc(&d, tot * sizeof(T)) != hipSuccess) {
         return CSSX_ENOMEM;
     }
     if (hipMemcpy(d, h.data(), tot * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return CSSX_EHIP;
     P.buf = d;
     P.r = d + o_r; P.rinv = d + o_ri; P.dxdr = d + o_x; P.grav = d + o_g; P.eps = d + o_e; P.dr = d + o_dr;
+    P.drr = d + o_drr;
     P.rbar = d + o_rb; P.sbar = d + o_sb; P.tbar = d + o_tb; P.tr = d + o_tr; P.hres = d + o_hr; P.drbar = d + o_drb; P.dsbar = d + o_dsb;
     P.sn = d + o_sn; P.csc = d + o_csc; P.cs = d + o_cs; P.cot = d + o_cot;
     P.dsdr1 = d + o_d1; P.dsdr2 = d + o_d2; P.sr1 = d + o_s1;
@@ -1197,27 +1212,46 @@ template <typename T>
 static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream, bool zero_walls = true) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
-    Scratch<T> &S = work<T>(m);
     Prof<T> &P = prof<T>(m);
     const Consts<T> c = make_consts<T>(m);
     const long N = m->N;
     const int *ibcr = m->p.ibcr, *ibct = m->p.ibct;
-    T *D = S.D;
-    const unsigned nb = nblocks(N, 256);
     const T *at = A, *ap = A + N, *ar = A + 2 * N;
-    DerivSpec th[2] = {spec<T>(m, CSSX_AXIS_THETA, ap, D + 0 * N, ibct[11], nullptr, nullptr, P.sn),
-                       spec<T>(m, CSSX_AXIS_THETA, ar, D + 1 * N, ibct[12], nullptr, nullptr)};
+    T *bt = B, *bp = B + N, *br = B + 2 * N;
+    // B = curl A with every factor applied in the sweeps' epilogues, the second sweep of each
+    // component adding to the first (no derivative fields are written):
+    //   B_r     =  csc/r d/dth (sin A_phi)  -  csc/r d/dph A_theta
+    //   B_phi   = -1/r   d/dth A_r          +  (dx/dr)/r d/dx (r A_theta)
+    //   B_theta =  csc/r d/dph A_r          -  (dx/dr)/r d/dx (r A_phi)
+    // theta pencils: post = csc (row theta), mul_b = 1/r (slow index r)
+    DerivSpec th[2] = {spec<T>(m, CSSX_AXIS_THETA, ap, br, ibct[11], nullptr, nullptr, P.sn, P.csc),
+                       spec<T>(m, CSSX_AXIS_THETA, ar, bp, ibct[12], nullptr, nullptr)};
+    th[0].mul_b = P.rinv;
+    th[1].mul_b = P.rinv;
+    th[1].scale = -th[1].scale;
     if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, 2, stream))) return rc;
-    DerivSpec ph[2] = {spec<T>(m, CSSX_AXIS_PHI, ar, D + 2 * N, 0, nullptr, nullptr),
-                       spec<T>(m, CSSX_AXIS_PHI, at, D + 3 * N, 0, nullptr, nullptr)};
+    // phi pencils: mul_a = csc (fast index theta), mul_b = 1/r (slow index r)
+    DerivSpec ph[2] = {spec<T>(m, CSSX_AXIS_PHI, ar, bt, 0, nullptr, nullptr),
+                       spec<T>(m, CSSX_AXIS_PHI, at, br, 0, nullptr, nullptr)};
+    for (DerivSpec &s : ph) {
+        s.mul_a = P.csc;
+        s.mul_b = P.rinv;
+    }
+    ph[1].scale = -ph[1].scale;
+    ph[1].beta = 1.0;
     if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, 2, stream))) return rc;
-    DerivSpec rr[2] = {spec<T>(m, CSSX_AXIS_R, at, D + 4 * N, ibcr[10], nullptr, nullptr, P.r, P.dxdr),
-                       spec<T>(m, CSSX_AXIS_R, ap, D + 5 * N, ibcr[11], nullptr, nullptr, P.r, P.dxdr)};
+    // r pencils: pre = r, post = (dx/dr)/r (row r)
+    DerivSpec rr[2] = {spec<T>(m, CSSX_AXIS_R, at, bp, ibcr[10], nullptr, nullptr, P.r, P.drr),
+                       spec<T>(m, CSSX_AXIS_R, ap, bt, ibcr[11], nullptr, nullptr, P.r, P.drr)};
+    rr[0].beta = rr[1].beta = 1.0;
+    rr[1].scale = -rr[1].scale;
     if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, 2, stream))) return rc;
-    CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, 9.0 * N * sizeof(T),
-               hipLaunchKernelGGL(k_curl_combine<T>, dim3(nb), dim3(256), 0, stream, c, P, D, D + N, D + 2 * N,
-                                  D + 3 * N, D + 4 * N, D + 5 * N, B, m->p.theta_perf_cond,
-                                  m->p.bottom_radial_field, m->p.top_radial_field, (int)zero_walls));
+    if (zero_walls) {
+        const long nw = std::max((long)m->nr * m->nphi, (long)m->nth * m->nphi);
+        CSSX_TIMED(CSSX_PHASE_POINTWISE, stream, 0.0,
+                   hipLaunchKernelGGL(k_curl_walls<T>, dim3(nblocks(nw, 256)), dim3(256), 0, stream, c, B,
+                                      m->p.theta_perf_cond, m->p.bottom_radial_field, m->p.top_radial_field));
+    }
     return hipGetLastError() == hipSuccess ? CSSX_OK : CSSX_EHIP;
 }
 
@@ -1456,6 +1490,18 @@ static SldParams<T> make_sld(const cssx_mhd *m) {
 // Compute_Diffusion(1): dif (nv fields) = the SLD tendency of state st, B = curl A with the
 // Compute_B wall conditions (magnetic runs).  dif must not alias the derivative scratch (the
 // Ohmic curl uses slots 0-8, the face speeds 15-17, the SLD speed slot 18).
+// The FP32 SLD runs in two passes (rho..s, then A), as FP64 does: each pass keeps more
+// wavefronts per SIMD than the single eight-field pass (126 and 93 VGPRs against 160), which
+// measured 1 ms per 256x512x1024 step faster.  CSSX_SLD_SPLIT=0 selects the single pass (a
+// tuning hook, read once).
+static bool sld_split() {
+    static const int v = [] {
+        const char *e = std::getenv("CSSX_SLD_SPLIT");
+        return (e && std::atoi(e) == 0) ? 0 : 1;
+    }();
+    return v != 0;
+}
+
 template <typename T>
 static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream) {
     int rc = ensure_scratch<T>(m);
@@ -1472,20 +1518,40 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
     const double b_speed = ts * N * (1.0 + (sp.dstype == 1 ? 3.0 + (sp.magnetic ? 4.0 : 0.0) : 0.0));
     CSSX_TIMED(CSSX_PHASE_SLD, stream, b_speed,
                hipLaunchKernelGGL(k_sld_speed<T>, dim3(nb), dim3(256), 0, stream, c, P, sp, st, B, vcc));
-    // tiles of 64 theta x PH phi columns; the r range is cut into segments so that the launch
-    // holds at least ~2048 workgroups
-    constexpr int PH = sizeof(T) == 8 ? 4 : 8;
-    const int ntt = (m->nth + 63) / 64, ntp = (m->nphi + PH - 1) / PH;
+    // tiles of 64 theta x 4 phi columns (one thread each); the r range is cut into segments of
+    // at least 16 levels so that the launch holds ~16K wavefronts where the grid allows
+    const int ntt = (m->nth + 63) / 64, ntp = (m->nphi + 3) / 4;
     const long tiles = (long)ntt * ntp;
-    int nseg = (int)std::min<long>((2048 + tiles - 1) / tiles, std::max(1, m->nr / 16));
+    int nseg = (int)std::min<long>((4096 + tiles - 1) / tiles, std::max(1, m->nr / 16));
     nseg = std::max(nseg, 1);
     const int rseg = (m->nr + nseg - 1) / nseg;
     nseg = (m->nr + rseg - 1) / rseg;
     const double nd = (double)(sp.last - sp.first + 1);
     const double b_march = ts * N * (nd + 1.0 + (double)nv + ((sp.visc || sp.mass) ? 2.0 : 0.0));
-    CSSX_TIMED(CSSX_PHASE_SLD, stream, b_march,
-               hipLaunchKernelGGL((k_sld_march<T, PH>), dim3((unsigned)(tiles * nseg)), dim3(64 * PH), 0, stream, c,
-                                  P, sp, st, (const T *)vcc, dif, ntt, ntp, rseg));
+    // rho..s and the A components in two passes (FP32 may take all eight in one, see
+    // sld_split).  The kernel addresses a field with 32-bit buffer offsets.
+    if ((double)N * sizeof(T) >= 4294967295.0) return CSSX_EINVAL;
+    const dim3 grid((unsigned)(tiles * nseg));
+    const double na = sp.last >= 5 ? (double)(sp.last - 4) : 0.0;   // diffused A components
+    auto two_pass = [&]() {
+        CSSX_TIMED(CSSX_PHASE_SLD, stream, b_march - ts * N * 2.0 * na,
+                   hipLaunchKernelGGL((k_sld_col<T, 0, 5>), grid, dim3(256), 0, stream, c, P, sp, st, (const T *)vcc,
+                                      dif, ntt, ntp, rseg));
+        if (na > 0.0)
+            CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (2.0 * na + 1.0),
+                       hipLaunchKernelGGL((k_sld_col<T, 5, 3>), grid, dim3(256), 0, stream, c, P, sp, st,
+                                          (const T *)vcc, dif, ntt, ntp, rseg));
+    };
+    if constexpr (sizeof(T) == 4) {
+        if (sp.last >= 5 && !sld_split())
+            CSSX_TIMED(CSSX_PHASE_SLD, stream, b_march,
+                       hipLaunchKernelGGL((k_sld_col<T, 0, 8>), grid, dim3(256), 0, stream, c, P, sp, st,
+                                          (const T *)vcc, dif, ntt, ntp, rseg));
+        else
+            two_pass();
+    } else {
+        two_pass();
+    }
     if (sp.ohmic) {
         T *C = S.D + 6 * N;
         if ((rc = curl_impl<T>(m, dif + 5 * N, C, stream, false))) return rc;

@@ -3,14 +3,12 @@
 //
 // Two launches plus, with Ohmic heating, the curl of the A tendency and a third:
 //   k_sld_speed   vcc = speed^2(r) [+ |u|^2 + B^2/(4 pi rho) for dstype 1]   (:177-302)
-//   k_sld_march   every field's tendency: a workgroup owns a tile of 64 theta x PH phi columns
-//                 and marches up r.  Each thread keeps a five-level window of its column of
-//                 every diffused field in registers and carries the upper r face's flux to the
-//                 next level as that level's lower face; the current level of the tile (with
-//                 two cells of halo) sits in LDS, where each cell's lower theta and phi face
-//                 fluxes are left for the neighbouring cells to take as their upper faces, so
-//                 every face of the grid is limited once.  Then the viscous heating and mass
-//                 diffusion corrections (:375-428), which need only the point itself.
+//   k_sld_col     every field's tendency: one thread per (theta, phi) column marching up r,
+//                 with a four-level window of every diffused field in registers and the upper
+//                 r face's flux carried to the next level (every r face limited once); the
+//                 theta and phi faces of each cell from the neighbouring columns at the level
+//                 (cache hits).  Then the viscous heating and mass diffusion corrections
+//                 (:375-428), which need only the point itself.
 //   (curl of the A tendency, the same batched derivative sweeps as Compute_B without its wall
 //    zeroing, VectorCurlA of :437)
 //   k_sld_finish  Ohmic heating (:430-461)
@@ -21,6 +19,8 @@
 // (:290-302) and phi is periodic.  Face values enter only through the face-flux weights rfc
 // and sph, which are zero on the walls.
 #pragma once
+
+#include "cssx_bufio.hpp"
 
 namespace cssx {
 
@@ -148,178 +148,171 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
     return (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
 }
 
-// value at (it, row) of field f at one r level for the theta stencils: inside the grid the
-// stored value (background removed), one or two cells past a theta wall that wall's ghost, and
-// zero further out (beyond a tile that overhangs the grid; finite, never weighted)
-template <typename T>
-__device__ T sld_tile_value(const Consts<T> &c, const Prof<T> &P, const SldParams<T> &sp, const T *__restrict__ fp,
-                            int f, long row, int it, int ir) {
-    if (it >= 0 && it < c.nth) return sld_val(c, P, fp, f, row + it, ir);
-    if (it >= -2 && it < 0) return sld_ghost_t(c, P, sp, fp, f, row, ir, false);
-    if (it >= c.nth && it <= c.nth + 1) return sld_ghost_t(c, P, sp, fp, f, row, ir, true);
-    return T(0);
-}
-
-// speed^2 of (it, row) with the ghost cells copying the nearest cell (diffusion.f90:290-302)
-template <typename T>
-__device__ __forceinline__ T sld_tile_vcc(const Consts<T> &c, const T *__restrict__ vcc, long row, int it) {
-    return vcc[row + min(max(it, 0), c.nth - 1)];
-}
-
 // Compute_Diffusion's fluxes (Diffusion_R :476-891, Diffusion_Th :893-1263, Diffusion_Phi
 // :1265-1495) and the viscous-heating / mass-diffusion corrections (:375-428) of every field.
-// Workgroup = 64 theta lanes x PH phi rows (one wavefront per row), one tile, r levels
-// [r0, r0 + rseg).  The tile index comes from the flat block index, XCD-contiguous.
-template <typename T, int PH>
-__global__ __launch_bounds__(64 * PH) void k_sld_march(Consts<T> c, Prof<T> P, SldParams<T> sp,
-                                                        const T *__restrict__ st, const T *__restrict__ vcc,
-                                                        T *__restrict__ D, int ntt, int ntp, int rseg) {
-    constexpr int NF = 8, TW = 68, TR = PH + 4;
-    __shared__ T sv[NF + 1][TR][TW];          // level values (fields, then vcc) with 2-cell halo
-    __shared__ T fth[NF][PH][65];             // lower theta face fluxes (65th: the tile's upper edge)
-    __shared__ T fph[NF][PH + 1][64];         // lower phi face fluxes (row PH: the tile's upper edge)
+// One thread per (theta, phi) column: a workgroup is 4 wavefronts, 64 consecutive theta columns
+// each, on 4 consecutive phi rows, and it marches up the r levels [r0, r0 + rseg).  Each thread
+// keeps a four-level window of its column of every diffused field in registers and carries the
+// upper r face's flux to the next level as that level's lower face, so every r face is
+// limited once.  The theta and phi faces of a cell are evaluated from the neighbouring values
+// at the level, which neighbouring lanes and wavefronts load as their own columns (cache
+// hits).  No LDS and no barriers: a thread whose column lies outside the grid returns at once.
+// The tile index comes from the flat block index, XCD-contiguous, so that phi-neighbouring
+// rows share one L2.
+template <typename T, int F0, int NF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
+                                                  const T *__restrict__ st, const T *__restrict__ vcc,
+                                                  T *__restrict__ D, int ntt, int ntp, int rseg) {
+    // fields [F0, F0 + NF): rho..s (with the point corrections, which need all five) and A
+    static_assert((F0 == 0 && NF >= 5 && NF <= 8) || (F0 >= 5 && F0 + NF <= 8), "field group");
+    constexpr bool CORR = F0 == 0;
+    // FP32 states hold rho and s as fluctuations already; FP64 states are absolute
+    constexpr bool FLUCT = sizeof(T) == 4;
     const long lb = xcd_block(blockIdx.x, gridDim.x);
     const int tt = (int)(lb % ntt), tp = (int)((lb / ntt) % ntp), seg = (int)(lb / ((long)ntt * ntp));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nth = c.nth, nphi = c.nphi, nr = c.nr;
+    const int it = tt * 64 + lane, ip = tp * 4 + wv;
+    if (it >= nth || ip >= nphi) return;
     const long N = c.N, plane = (long)nth * nphi;
-    const int th0 = tt * 64, ph0 = tp * PH;
     const int r0 = seg * rseg, r1 = min(nr, r0 + rseg);
-    const int it = th0 + lane, ip = (ph0 + wv) % nphi;
-    const bool own = it < nth;                 // this lane's column is a grid column
-    const long col = (long)ip * nth + (own ? it : 0);
-    auto active = [&](int f) { return f >= sp.first && f <= sp.last; };
+    const bool wall_lo = tt == 0, wall_hi = tt == ntt - 1;   // wavefront holds a theta wall cell
+    // Loads are raw buffer loads: a scalar resource per field, the level's byte offset in a
+    // scalar register and one of nine 32-bit lane byte offsets within the level plane, shared
+    // by every field, so that no per-lane address is live per field across the march (a field
+    // stays below 4 GiB).  The few stores per level use plain global stores (no resources).
+    constexpr unsigned E = sizeof(T);
+    const unsigned oc = (unsigned)(ip * nth + it) * E;
+    const unsigned orow = (unsigned)(ip * nth);
+    const unsigned otm2 = (orow + (unsigned)max(it - 2, 0)) * E, otm1 = (orow + (unsigned)max(it - 1, 0)) * E;
+    const unsigned otp1 = (orow + (unsigned)min(it + 1, nth - 1)) * E, otp2 = (orow + (unsigned)min(it + 2, nth - 1)) * E;
+    const unsigned opm2 = (unsigned)(((ip + nphi - 2) % nphi) * nth + it) * E;
+    const unsigned opm1 = (unsigned)(((ip + nphi - 1) % nphi) * nth + it) * E;
+    const unsigned opp1 = (unsigned)(((ip + 1) % nphi) * nth + it) * E;
+    const unsigned opp2 = (unsigned)(((ip + 2) % nphi) * nth + it) * E;
+    const unsigned pb = (unsigned)plane * E;   // bytes per level
+    const __amdgpu_buffer_rsrc_t rv = field_rsrc(vcc);
+    // field f (level j, lane offset ofs) and the speeds; the tendency of field f at level j
+    auto ldf = [&](int f, int j, unsigned ofs) -> T { return bload<T>(field_rsrc(st + (long)f * N), ofs, (unsigned)j * pb); };
+    auto ldv = [&](int j, unsigned ofs) -> T { return bload<T>(rv, ofs, (unsigned)j * pb); };
+    auto dl = [&](int f, int j) -> T * { return D + (long)f * N + (long)j * plane; };
+    // the background removed from rho (f = 0) and s (f = 4) of an absolute state at level j
+    auto bg = [&](int f, int j) -> T {
+        if constexpr (FLUCT) return T(0);
+        return f == 0 ? P.rbar[j] : f == 4 ? P.sbar[j] : T(0);
+    };
     // the column of field f at level j, with the r-wall ghosts below and above the grid
     auto colval = [&](int f, int j) -> T {
         const T *fp = st + (long)f * N;
-        if (j < 0) return sld_ghost_r(c, P, sp, fp, f, col, plane, false);
-        if (j >= nr) return sld_ghost_r(c, P, sp, fp, f, col, plane, true);
-        return sld_val(c, P, fp, f, col + (long)j * plane, j);
+        if (j < 0) return sld_ghost_r(c, P, sp, fp, f, (long)(ip * nth + it), plane, false);
+        if (j >= nr) return sld_ghost_r(c, P, sp, fp, f, (long)(ip * nth + it), plane, true);
+        return ldf(f, j, oc) - bg(f, j);
     };
-    auto colvcc = [&](int j) -> T { return vcc[col + (long)min(max(j, 0), nr - 1) * plane]; };
-    // four-level windows w[f][k] = level ir - 1 + k; frl = the flux of level ir's lower r face
+    auto colvcc = [&](int j) -> T { return ldv(min(max(j, 0), nr - 1), oc); };
+    // four-level windows w[q][k] = level ir - 1 + k of field F0 + q; frl = the flux of level ir's
+    // lower r face.  Every field of the group is evaluated; rho, the only field a configuration
+    // can leave undiffused (no mass diffusion), has its tendency zeroed at the store.
     T w[NF][4], frl[NF];
-    T vc_cur = T(0), vc_next = T(0);
-    if (own) {
-        vc_cur = colvcc(r0);
-        vc_next = colvcc(r0 + 1);
+    T vc_cur = colvcc(r0), vc_next = colvcc(r0 + 1);
+    {
         const T vlow = colvcc(r0 - 1);
         const T rl = P.rfc[r0];
         const T spd = sp.coef * sqrt(T(0.5) * (vc_cur + vlow));
+        // levels r0-2 .. r0+2 clamped into the grid, then the wall ghosts where a level lies
+        // outside it (uniform branches: only the first and the last segment take them)
 #pragma unroll
-        for (int f = 0; f < NF; f++) {
-            frl[f] = T(0);
-            if (!active(f)) continue;
-            const T wm2 = colval(f, r0 - 2);
+        for (int q = 0; q < NF; q++) {
+            const int f = F0 + q;
+            const int jm = max(r0 - 2, 0);
+            T wm2 = ldf(f, jm, oc) - bg(f, jm);
 #pragma unroll
-            for (int k = 0; k < 4; k++) w[f][k] = colval(f, r0 - 1 + k);
-            frl[f] = spd * (sld_delue(wm2, w[f][0], w[f][1], w[f][2], sp.tiny[f], sp.itiny2[f]) * rl * rl);
+            for (int k = 0; k < 4; k++) {
+                const int j = min(max(r0 - 1 + k, 0), nr - 1);
+                w[q][k] = ldf(f, j, oc) - bg(f, j);
+            }
+            if (r0 < 2) {
+                const T g = colval(f, -1);
+                wm2 = g;
+                if (r0 == 0) w[q][0] = g;
+            }
+            if (r0 + 2 >= nr) {
+                const T g = colval(f, nr);
+                if (r0 + 1 >= nr) w[q][2] = g;
+                w[q][3] = g;
+            }
+            frl[q] = spd * (sld_delue(wm2, w[q][0], w[q][1], w[q][2], sp.tiny[f], sp.itiny2[f]) * rl * rl);
         }
     }
-    const int hc_rows = 4 * TW, hc_cols = 4 * PH, nhalo = hc_rows + hc_cols;
-    for (int ir = r0; ir < r1; ir++) {
-        // next level of every window (consumed at the end of the step)
+    const T wtl = P.sph[it], wth = P.sph[it + 1];
+    T gtop[NF];   // the top wall's ghosts, set before the tail levels
+    // one r level; TAIL: within three levels of the top wall, where the next window level may
+    // be the wall's ghost (the levels below keep that branch out of the main loop)
+    auto level = [&](int ir, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
+        // next level of every window (consumed at the end of the step); past the top wall the
+        // ghost (levels beyond nr meet only the wall face, whose weight is zero)
+        const bool past = TAIL && ir + 3 >= nr;
+        const int jn = min(ir + 3, nr - 1);
+        const T bgn0 = bg(0, jn), bgn4 = bg(4, jn), bg0 = bg(0, ir), bg4 = bg(4, ir);
         T nx[NF];
-        T vc_nx = T(0);
-        if (own) {
 #pragma unroll
-            for (int f = 0; f < NF; f++) nx[f] = active(f) ? colval(f, ir + 3) : T(0);
-            vc_nx = colvcc(ir + 2);
+        for (int q = 0; q < NF; q++) {
+            const int f = F0 + q;
+            const T x = ldf(f, jn, oc) - (f == 0 ? bgn0 : f == 4 ? bgn4 : T(0));
+            nx[q] = past ? gtop[q] : x;
         }
-        // ---- the tile's level ir in LDS: own cells, then the halo ----
+        const T vc_nx = ldv(min(ir + 2, nr - 1), oc);
+        // face speeds sld_coef sqrt(mean of the two cells' speed^2); ghost cells copy the nearest
+        // cell (diffusion.f90:290-302)
+        const T vtm = ldv(ir, otm1), vtp = ldv(ir, otp1), vpm = ldv(ir, opm1), vpp = ldv(ir, opp1);
+        const T s_tl = sp.coef * sqrt(T(0.5) * (vc_cur + vtm)), s_th = sp.coef * sqrt(T(0.5) * (vtp + vc_cur));
+        const T s_pl = sp.coef * sqrt(T(0.5) * (vpm + vc_cur)), s_ph = sp.coef * sqrt(T(0.5) * (vc_cur + vpp));
+        const T spd = sp.coef * sqrt(T(0.5) * (vc_next + vc_cur));
+        const T rh = P.rfc[ir + 1];
+        const T gr = P.rv[ir], gtt = P.rfac[ir] * P.ct[it], gp = P.gt[it] * (P.rfac[ir] * sp.invdphi);
+        T out[5];
 #pragma unroll
-        for (int f = 0; f < NF; f++)
-            if (active(f)) sv[f][wv + 2][lane + 2] = own ? w[f][1] : T(0);
-        sv[NF][wv + 2][lane + 2] = own ? vc_cur : T(0);
-        if (!own) {
-            // a tile overhanging the grid: the cells just past the upper theta wall are ghosts
-            const long row = (long)ir * plane + (long)ip * nth;
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-                if (active(f)) sv[f][wv + 2][lane + 2] = sld_tile_value(c, P, sp, st + (long)f * N, f, row, it, ir);
-            sv[NF][wv + 2][lane + 2] = sld_tile_vcc(c, vcc, row, it);
+        for (int q = 0; q < NF; q++) {
+            const int f = F0 + q;
+            if (f < sp.first) {   // rho without mass diffusion
+                out[0] = T(0);
+                continue;
+            }
+            const T b = f == 0 ? bg0 : f == 4 ? bg4 : T(0);
+            const T tiny = sp.tiny[f], it2 = sp.itiny2[f];
+            // theta neighbours at this level; past a wall, that wall's ghost
+            T a2 = ldf(f, ir, otm2) - b, a1 = ldf(f, ir, otm1) - b, b1 = ldf(f, ir, otp1) - b, b2 = ldf(f, ir, otp2) - b;
+            if (wall_lo) {
+                const T g = sld_ghost_t(c, P, sp, st + (long)f * N, f, (long)ir * plane + orow, ir, false);
+                a2 = it < 2 ? g : a2;
+                a1 = it < 1 ? g : a1;
+            }
+            if (wall_hi) {
+                const T g = sld_ghost_t(c, P, sp, st + (long)f * N, f, (long)ir * plane + orow, ir, true);
+                b1 = it + 1 >= nth ? g : b1;
+                b2 = it + 2 >= nth ? g : b2;
+            }
+            const T pm2 = ldf(f, ir, opm2) - b, pm1 = ldf(f, ir, opm1) - b;
+            const T pp1 = ldf(f, ir, opp1) - b, pp2 = ldf(f, ir, opp2) - b;
+            const T f0 = w[q][1];
+            const T ft_lo = (s_tl * wtl) * sld_delue(a2, a1, f0, b1, tiny, it2);
+            const T ft_hi = (s_th * wth) * sld_delue(a1, f0, b1, b2, tiny, it2);
+            const T fp_lo = s_pl * sld_delue(pm2, pm1, f0, pp1, tiny, it2);
+            const T fp_hi = s_ph * sld_delue(pm1, f0, pp1, pp2, tiny, it2);
+            const T fh = spd * (sld_delue(w[q][0], w[q][1], w[q][2], w[q][3], tiny, it2) * rh * rh);
+            const T Dr = sp.mr[f] * (gr * (fh - frl[q]));
+            frl[q] = fh;
+            const T Dt = sp.mt[f] * (gtt * (ft_hi - ft_lo));
+            const T Dp = gp * (sp.mt[f] * (fp_hi - fp_lo));
+            const T o = (Dr + Dt) + Dp;
+            if (f < 5) out[f] = o;
+            else if (c.magnetic) at_row(dl(f, ir), oc) = o;       // A: no correction
         }
-        for (int h = threadIdx.x; h < nhalo; h += 64 * PH) {
-            int hr, hcol;
-            if (h < hc_rows) {                      // rows -2, -1, PH, PH+1, all 68 columns
-                const int q = h / TW;
-                hr = q < 2 ? q : PH + q;
-                hcol = h % TW;
-            } else {                                // columns -2, -1, 64, 65 of rows 0..PH-1
-                const int q = h - hc_rows;
-                hr = 2 + q / 4;
-                const int k = q % 4;
-                hcol = k < 2 ? k : 64 + k;
-            }
-            const int hit = th0 + hcol - 2;
-            const int hip = ((ph0 + hr - 2) % nphi + nphi) % nphi;
-            const long row = (long)ir * plane + (long)hip * nth;
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-                if (active(f)) sv[f][hr][hcol] = sld_tile_value(c, P, sp, st + (long)f * N, f, row, hit, ir);
-            sv[NF][hr][hcol] = sld_tile_vcc(c, vcc, row, hit);
-        }
-        __syncthreads();
-        // ---- lower theta and phi face fluxes of every cell; the tile's upper edges ----
-        {
-            const int tc = lane + 2, tr = wv + 2;
-            const T vt = sp.coef * sqrt(T(0.5) * (sv[NF][tr][tc] + sv[NF][tr][tc - 1]));
-            const T vp = sp.coef * sqrt(T(0.5) * (sv[NF][tr - 1][tc] + sv[NF][tr][tc]));
-            const T wt = it <= nth ? P.sph[it] : T(0);
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                if (!active(f)) continue;
-                const T tiny = sp.tiny[f], it2 = sp.itiny2[f];
-                fth[f][wv][lane] = (vt * wt) * sld_delue(sv[f][tr][tc - 2], sv[f][tr][tc - 1], sv[f][tr][tc],
-                                                         sv[f][tr][tc + 1], tiny, it2);
-                fph[f][wv][lane] = vp * sld_delue(sv[f][tr - 2][tc], sv[f][tr - 1][tc], sv[f][tr][tc],
-                                                  sv[f][tr + 1][tc], tiny, it2);
-            }
-            if (threadIdx.x < 64) {                 // phi faces of row PH (the next tile's row 0)
-                const int r = PH + 2;
-                const T v2 = sp.coef * sqrt(T(0.5) * (sv[NF][r - 1][tc] + sv[NF][r][tc]));
-#pragma unroll
-                for (int f = 0; f < NF; f++)
-                    if (active(f))
-                        fph[f][PH][lane] = v2 * sld_delue(sv[f][r - 2][tc], sv[f][r - 1][tc], sv[f][r][tc],
-                                                          sv[f][r + 1][tc], sp.tiny[f], sp.itiny2[f]);
-            } else if (threadIdx.x < 64 + PH) {     // theta faces of column 64 (the next tile's 0)
-                const int rr = threadIdx.x - 64 + 2, cc = 66, ite = th0 + 64;
-                const T v2 = sp.coef * sqrt(T(0.5) * (sv[NF][rr][cc] + sv[NF][rr][cc - 1]));
-                const T w2 = ite <= nth ? P.sph[ite] : T(0);
-#pragma unroll
-                for (int f = 0; f < NF; f++)
-                    if (active(f))
-                        fth[f][rr - 2][64] = (v2 * w2) * sld_delue(sv[f][rr][cc - 2], sv[f][rr][cc - 1], sv[f][rr][cc],
-                                                                   sv[f][rr][cc + 1], sp.tiny[f], sp.itiny2[f]);
-            }
-        }
-        __syncthreads();
-        // ---- the tendency of this level ----
-        if (own) {
-            const long i = (long)ir * plane + col;
-            const T rh = P.rfc[ir + 1];
-            const T spd = sp.coef * sqrt(T(0.5) * (vc_next + vc_cur));
-            const T gr = P.rv[ir], gtt = P.rfac[ir] * P.ct[it], gp = P.gt[it] * (P.rfac[ir] * sp.invdphi);
-            T out[5];
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                T o = T(0);
-                if (active(f)) {
-                    const T fh = spd * (sld_delue(w[f][0], w[f][1], w[f][2], w[f][3], sp.tiny[f], sp.itiny2[f]) * rh * rh);
-                    const T Dr = sp.mr[f] * (gr * (fh - frl[f]));
-                    frl[f] = fh;
-                    const T Dt = sp.mt[f] * (gtt * (fth[f][wv][lane + 1] - fth[f][wv][lane]));
-                    const T Dp = gp * (sp.mt[f] * (fph[f][wv + 1][lane] - fph[f][wv][lane]));
-                    o = (Dr + Dt) + Dp;
-                }
-                if (f < 5) out[f] = o;
-                else if (c.magnetic) D[(long)f * N + i] = o;       // A: no correction
-            }
+        if constexpr (CORR) {
             // viscous heating and mass diffusion (:375-428), at the point
             if (sp.visc || sp.mass) {
                 // the stored slots (the windows hold rho and s with the FP64 background removed)
-                const Thermo<T> th = thermo(st[i], st[4 * N + i], ir, c, P);
+                const Thermo<T> th = thermo(ldf(0, ir, oc), ldf(4, ir, oc), ir, c, P);
                 const T invrho = T(1) / th.rho, itemp = T(1) / th.t;
                 const T u1 = w[1][1], u2 = w[2][1], u3 = w[3][1];
                 if (sp.visc) {
@@ -336,19 +329,24 @@ __global__ __launch_bounds__(64 * PH) void k_sld_march(Consts<T> c, Prof<T> P, S
                 }
             }
 #pragma unroll
-            for (int f = 0; f < 5; f++) D[(long)f * N + i] = out[f];
-            // shift the windows
-#pragma unroll
-            for (int f = 0; f < NF; f++) {
-                if (!active(f)) continue;
-#pragma unroll
-                for (int k = 0; k < 3; k++) w[f][k] = w[f][k + 1];
-                w[f][3] = nx[f];
-            }
-            vc_cur = vc_next;
-            vc_next = vc_nx;
+            for (int f = 0; f < 5; f++) at_row(dl(f, ir), oc) = out[f];
         }
-        __syncthreads();
+        // shift the windows
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) w[q][k] = w[q][k + 1];
+            w[q][3] = nx[q];
+        }
+        vc_cur = vc_next;
+        vc_next = vc_nx;
+    };
+    const int rm = max(r0, min(r1, nr - 3));
+    for (int ir = r0; ir < rm; ir++) level(ir, std::false_type{});
+    if (rm < r1) {
+#pragma unroll
+        for (int q = 0; q < NF; q++) gtop[q] = colval(F0 + q, nr);
+        for (int ir = rm; ir < r1; ir++) level(ir, std::true_type{});
     }
 }
 

@@ -390,17 +390,28 @@ def kinetic_energy(state: torch.Tensor) -> float:
     return float((0.5 * state[0] * (state[1] ** 2 + state[2] ** 2 + state[3] ** 2)).mean().item())
 
 
-def stats_of(state: torch.Tensor, model: Model) -> dict:
-    m, r = field_stats(state, model)
-    return {"rms": r.tolist(), "ke": kinetic_energy(state)}
+SCALARS = ("kinener", "cke", "mcke", "drke", "magener")
+
+
+def stats_of(state: torch.Tensor, sol: Solver) -> dict:
+    """Per-field RMS (rho and s about the background) and the Scalar_Output energies of
+    report_mpi.f90:173-262 (kinetic energy, its radial / radial+meridional / azimuthal parts,
+    magnetic energy) of an FP64 state on the solver's grid."""
+    from .diagnostics import scalar_quantities
+    m, r = field_stats(state, sol.model)
+    q = scalar_quantities(sol, state)
+    out = {"rms": r.tolist(), "ke": kinetic_energy(state)}
+    out.update({k: q[k] for k in SCALARS if k in q})
+    return out
 
 
 def compare(fine_cfg: Config, factor: int, net: Network | None, coarse_steps: int, every: int,
-            precision: str = "f32", y_fine: torch.Tensor | None = None):
+            precision: str = "f32", y_fine: torch.Tensor | None = None, progress=None):
     """Runs the fine grid and, from its filtered initial state, the coarse grid with and without
     the closure for `coarse_steps` steps of 4x the fine step (factor fine steps each); every
     `every` coarse steps it records the statistics of the filtered fine state and of both coarse
-    states.  Returns the records and the wall times (fine run + filtering; coarse runs)."""
+    states.  Returns the records and the wall times (fine run + filtering; coarse runs).
+    progress, if given, is called with each record as it is taken."""
     fine = Solver(Model(fine_cfg))
     cc = coarse_config(fine_cfg, factor)
     cmodel = Model(cc)
@@ -420,10 +431,12 @@ def compare(fine_cfg: Config, factor: int, net: Network | None, coarse_steps: in
 
     def rec(k):
         yf = filter_state(y, fine, plain, factor)
-        r = {"step": k, "time": k * dtc, "filtered_fine": stats_of(yf, cmodel), "coarse": stats_of(yp, cmodel)}
+        r = {"step": k, "time": k * dtc, "filtered_fine": stats_of(yf, plain), "coarse": stats_of(yp, plain)}
         if yq is not None:
-            r["coarse_closure"] = stats_of(yq, cmodel)
+            r["coarse_closure"] = stats_of(yq, plain)
         recs.append(r)
+        if progress is not None:
+            progress(r)
 
     rec(0)
     for k0 in range(0, coarse_steps, every):
@@ -466,3 +479,18 @@ def stat_errors(recs, key: str):
         kb = r["filtered_fine"]["ke"]
         ke = max(ke, abs(r[key]["ke"] - kb) / max(abs(kb), 1e-300))
     return fld, ke
+
+
+def scalar_errors(recs, key: str) -> dict:
+    """Max over records (after the initial one) of the relative error of each Scalar_Output
+    energy against the filtered fine run."""
+    out = {}
+    for name in SCALARS:
+        e = 0.0
+        for r in recs[1:]:
+            if name not in r[key]:
+                continue
+            b = r["filtered_fine"][name]
+            e = max(e, abs(r[key][name] - b) / max(abs(b), 1e-300))
+        out[name] = e
+    return out

@@ -328,7 +328,10 @@ int cssx_gather_rows(const void *src, long row_bytes, const long *idx, long n, v
 int cssx_closure_tendency(cssx_mlp *net, const void *state, int prec, int fluct, const void *rho_bar,
                           const void *s_bar, int nr, int nphi, int nth, float *ext, void *stream);
 /* attach (net != NULL) or detach a closure: every later RHS evaluation of m (cssx_rhs and the
- * steppers' stages) adds the network's tendency of its input state.  net is not owned. */
+ * steppers' stages) adds the network's tendency of its input state.  net is not owned.  Each
+ * tendency writes the network's packed weights and inference scratch, so one network must not
+ * serve solvers whose RHS evaluations can overlap on different streams (each solver runs its
+ * closure on its own side stream). */
 int cssx_mhd_set_closure(cssx_mhd *m, cssx_mlp *net);
 
 #ifdef __cplusplus

@@ -137,7 +137,7 @@ def test_deriv_field_matches_oracle(cssx_mod, shape, case):
 def test_deriv_c1_recipe(cssx_mod):
     """The plain-derivative workload (64 x 64 x 128, tanh-stretched r on [0.72, 0.97], the
     seeded 8-term field of tests/c1_field.py, phi halos taken from the field): cssx_deriv on
-    every axis against the FP64 oracle (1e-12; FP32 3e-5) and the analytic derivatives (the
+    every axis against the FP64 oracle (128 eps max|f| scale) and the analytic derivatives (the
     bounds the oracle itself meets), with one-sided walls (code 1) and with Neumann walls
     carrying the exact wall derivative (code 4); and the compact_fd6 drop-in on every r and
     theta pencil and on sampled phi pencils (dtype 3, halo ghosts) against the CSS-Python
@@ -157,14 +157,26 @@ def test_deriv_c1_recipe(cssx_mod):
     exact = {0: c["fr"], 1: c["ft"], 2: c["fp"]}
     bound = {0: 5e-6, 1: 1e-5, 2: 1e-9}
     ops = O.Operators(nr, nth, nphi, c["dri"], c["dti"], c["dpi"])
-    for prec, tol in ((torch.float64, 1e-12), (torch.float32, 3e-5)):
+    # Rounding bound: 128 eps(T) max|f| scale, absolute.  The Dirichlet-type wall rows combine
+    # values with coefficient sums near 46 (2 (10/3 + 3 + 6 + 1/3) + 8 (43/96 + 5/6 + 9/8 + 1/6
+    # + 1/96)) and the solve carries that rounding on at most ~3x; for this smooth field the
+    # index-space r derivative is only ~0.02 |f|, so the rounding is large relative to the
+    # result (measured on the r axis: 72 eps max|f| scale in FP64, 46 in FP32, of which the FP64
+    # oracle's own response to FP32-rounded input is 2).
+    scales = {0: c["dri"], 1: c["dti"], 2: c["dpi"]}
+    fmax = np.abs(f).max()
+    for prec in (torch.float64, torch.float32):
+        eps = float(torch.finfo(prec).eps)
         x = torch.as_tensor(f, dtype=prec, device="cuda")
         gl = torch.as_tensor(c["ghost_lo"], dtype=prec, device="cuda")
         gh = torch.as_tensor(c["ghost_hi"], dtype=prec, device="cuda")
         got = {0: ops.dbydr(x, 1), 1: ops.dbydt(x, 1), 2: ops.dbydp(x, gl, gh)}
         for ax in range(3):
             g = got[ax].double().cpu().numpy()
-            assert _rel(g, ref[ax]) < tol, (ax, prec, _rel(g, ref[ax]))
+            lim = 128 * eps * fmax * scales[ax]
+            err = np.abs(g - ref[ax]).max()
+            assert err < lim, (ax, prec, err, lim)
+            tol = lim / np.abs(ref[ax]).max()
             phys = g * dxdr if ax == 0 else g
             assert _rel(phys, exact[ax]) < bound[ax] + 2 * tol, (ax, prec, _rel(phys, exact[ax]))
     # Neumann walls with the exact wall derivatives in slot 0 of the r ghost buffers
@@ -175,14 +187,15 @@ def test_deriv_c1_recipe(cssx_mod):
     r4 = oracle.deriv_field(0, 1, f, bdry=4, ghost_lo=gl, ghost_hi=gh, scale=c["dri"], hneu=c["dri"])
     g4 = ops.dbydr(torch.as_tensor(f, device="cuda"), 4, torch.as_tensor(gl, device="cuda"),
                    torch.as_tensor(gh, device="cuda")).cpu().numpy()
-    assert _rel(g4, r4) < 1e-12
+    eps64 = float(np.finfo(np.float64).eps)
+    assert np.abs(g4 - r4).max() < 128 * eps64 * fmax * c["dri"]
     assert _rel(g4 * dxdr, c["fr"]) < 2e-8
     ops.close()
     # the CSS-Python drop-in: r pencils (axis 0) and theta pencils (axis 2), walls, ibc 1
     pr_ = D.compact_fd6(1.0, f, 0.0, 0.0, 1, 0, axis=0)
-    assert _rel(pr_, ref[0]) < 1e-12
+    assert np.abs(pr_ - ref[0]).max() < 128 * eps64 * fmax * c["dri"]
     pt_ = D.compact_fd6(1.0 / (np.pi / 2), f, 0.0, 0.0, 1, 0, axis=2)
-    assert _rel(pt_, ref[1]) < 1e-12
+    assert np.abs(pt_ - ref[1]).max() < 128 * eps64 * fmax * c["dti"]
     # phi pencils with their halos (dtype 3: the Python's halo-row scaling included)
     pp_ = D.compact_fd6(1.0 / (np.pi / 2), f, c["ghost_lo"], c["ghost_hi"], 0, 3, axis=1)
     rng = np.random.default_rng(12)
@@ -190,7 +203,7 @@ def test_deriv_c1_recipe(cssx_mod):
         ir, it = rng.integers(0, nr), rng.integers(0, nth)
         want, _ = oracle.compact_fd6_py(1.0 / (np.pi / 2), f[ir, :, it], c["ghost_lo"][ir, :, it],
                                        c["ghost_hi"][ir, :, it], 0, 3)
-        assert np.abs(pp_[ir, :, it] - want).max() < 1e-12 * np.abs(want).max()
+        assert np.abs(pp_[ir, :, it] - want).max() < 128 * eps64 * fmax * c["dpi"]
 
 
 LONG_CASES = []

@@ -93,7 +93,7 @@ def run():
     from cssx.model import Config, Model
     if not torch.cuda.is_available():
         pytest.skip("needs the GPU")
-    c = Config(nr=20, nth=40, nphi=32, omega0=0.2, bamp=0.5)
+    c = Config(nr=24, nth=40, nphi=32, omega0=0.2, bamp=0.5)   # nr >= 18 quantities + 4 header rows
     m = Model(c)
     st = m.initial_state()
     rng = np.random.default_rng(11)
@@ -144,7 +144,11 @@ def test_shell_average_record(run, tmp_path):
     lo[0] = (m.dsdr1 / m.dxdr[0])[None, :]
     ds = oracle.deriv_field(0, 1, s, bdry=m.ibcr[5], ghost_lo=lo, scale=m.dri) * m.dxdr[:, None, None]
     got = d["values"][:, q.index(9), 0]
-    assert np.abs(got - mean(ds)).max() <= 1e-11 * np.abs(mean(ds)).max() + 1e-14
+    # rounding bound of the compact derivative: 128 eps max|s| dri max(dx/dr); the entropy (the
+    # absolute state) is far larger than its radial derivative, and the wall rows combine it
+    # with coefficient sums near 46
+    lim = 128 * np.finfo(np.float64).eps * np.abs(s).max() * m.dri * np.abs(m.dxdr).max()
+    assert np.abs(got - mean(ds)).max() <= lim
 
 
 @pytest.mark.gpu
