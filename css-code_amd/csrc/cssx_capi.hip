@@ -212,6 +212,7 @@ extern "C" const char *cssx_strerror(int code) {
     case CSSX_EINVAL: return "invalid argument (shape, boundary code, axis or pointer)";
     case CSSX_ENAN: return "non-finite value detected";
     case CSSX_EHIP: return "HIP runtime error";
+    case CSSX_ESIZE: return "size beyond the fixed scratch of a no-allocation entry point";
     default: return "unknown error";
     }
 }
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(256) void k_compact_small(long npen, double dri, in
 }
 
 // Argument checks, the register-resident kernel for n < 8, and the derivative spec for n >= 8
-// of cssx_compact_fd6.  Returns CSSX_OK with *done set
+// shared by cssx_compact_fd6 and cssx_compact_fd6_noalloc.  Returns CSSX_OK with *done set
 // when the short-pencil kernel already ran.
 static int fd6_prepare(int n, long npencils, double dri, int ibc, int dtype, const double *arr, const double *b1,
                        const double *b2, double *darr, double *result, hipStream_t st, bool *done, DerivSpec &s,
@@ -463,6 +464,68 @@ extern "C" int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int d
     const int rc = fd6_prepare(n, npencils, dri, ibc, dtype, arr, b1, b2, darr, result, (hipStream_t)stream, &done, s, g);
     if (rc || done) return rc;
     return run_derivs(&s, 1, g, CSSX_PREC_F64, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------------------------
+// No-allocation variant of cssx_compact_fd6.  Its only scratch, the FP64 plan table of the
+// launch (5n + 80 doubles for the strided kernel, 5n + 960 for the wide one: at most 6080 for
+// n <= 1024), lives in `scratch`, one fixed array per device.  The plan is built on the host
+// into a host-side cache (kept for the process, so the asynchronous copy's source stays valid)
+// and copied into the array on the caller's stream ahead of the launch.  All calls on one
+// device share the array: calls that overlap on different streams of one device race.
+// ---------------------------------------------------------------------------------------------
+constexpr int kScratchDoubles = 8192;   // 64 KiB per device
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;                 // device -> address of its `scratch`
+std::map<PlanKey, std::vector<double>> g_host_plans;   // FP64 plans, host memory only
+}  // namespace
+
+// the current device's address of `scratch`, looked up once per device
+static int scratch_address(double **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return CSSX_EHIP;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        double *p = nullptr;
+        if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CSSX_EHIP;
+        it = g_scratch_ptr.emplace(dev, p).first;
+    }
+    *out = it->second;
+    return CSSX_OK;
+}
+
+extern "C" int cssx_compact_fd6_noalloc(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
+                                        const double *b1, const double *b2, double *darr, double *result,
+                                        void *stream) {
+    DerivSpec s;
+    AxisGeom g;
+    bool done = false;
+    const hipStream_t st = (hipStream_t)stream;
+    int rc = fd6_prepare(n, npencils, dri, ibc, dtype, arr, b1, b2, darr, result, st, &done, s, g);
+    if (rc || done) return rc;
+    double *ptr = nullptr;
+    if ((rc = scratch_address(&ptr))) return rc;
+    const PlanKey key = launch_plan_key(&s, 1, 0, g, true);
+    const std::vector<double> *plan = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = g_host_plans.find(key);
+        if (it == g_host_plans.end()) {
+            std::vector<double> h;
+            if ((rc = host_plan(key, h))) return rc;
+            it = g_host_plans.emplace(key, std::move(h)).first;
+        }
+        plan = &it->second;
+    }
+    if (plan->size() > (size_t)kScratchDoubles) return CSSX_ESIZE;
+    if (hipMemcpyAsync(ptr, plan->data(), plan->size() * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return CSSX_EHIP;
+    s.plan = ptr;
+    return run_derivs(&s, 1, g, CSSX_PREC_F64, st);
 }
 
 extern "C" int cssx_plan_tables(int n, int order, int lo_kind, int hi_kind, int bdry, int variant,

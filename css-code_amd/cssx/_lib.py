@@ -16,6 +16,7 @@ CSSX_ENOMEM = 12
 CSSX_EINVAL = 22
 CSSX_ENAN = 33
 CSSX_EHIP = 100
+CSSX_ESIZE = 27
 PREC_F64 = 0
 PREC_F32 = 1
 AXIS_R, AXIS_THETA, AXIS_PHI = 0, 1, 2
@@ -60,6 +61,8 @@ SIGNATURES = {
                    ctypes.c_int),
     "cssx_compact_fd6": ([ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                           _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+    "cssx_compact_fd6_noalloc": ([ctypes.c_int, ctypes.c_long, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                  _vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
     "cssx_mhd_create": ([_vp, _vp, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
     "cssx_mhd_destroy": ([_vp], None),
     "cssx_rhs": ([_vp, ctypes.c_int, _vp, _vp, _vp, _vp], ctypes.c_int),

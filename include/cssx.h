@@ -25,6 +25,7 @@ extern "C" {
 #define CSSX_EINVAL 22      /* bad shape, boundary code, axis or pointer */
 #define CSSX_ENAN 33        /* non-finite value detected by a checked call */
 #define CSSX_EHIP 100       /* a HIP runtime call failed */
+#define CSSX_ESIZE 27       /* problem larger than a no-allocation entry point's fixed scratch */
 
 #define CSSX_PREC_F64 0
 #define CSSX_PREC_F32 1
@@ -84,6 +85,17 @@ int cssx_deriv(cssx_ctx *ctx, int axis, int order, int bdry, int prec, const voi
  * the reference's Thomas solve does. */
 int cssx_compact_fd6(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
                      const double *b1, const double *b2, double *darr, double *result, void *stream);
+
+/* cssx_compact_fd6 that allocates nothing.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as cssx_compact_fd6,
+ * bit for bit; its one scratch buffer, the FP64 plan table of the launch, lives in a fixed
+ * 64 KiB array per device (8192 doubles; the largest plan, n = 1024, takes 6080) that the
+ * library holds until the process ends.  The plan is copied into it on `stream` before the
+ * launch, so calls that overlap on different streams of one device race; a plan that would not
+ * fit returns CSSX_ESIZE. */
+int cssx_compact_fd6_noalloc(int n, long npencils, double dri, int ibc, int dtype, const double *arr,
+                             const double *b1, const double *b2, double *darr, double *result, void *stream);
 
 /* ---------------------------------------------------------------------------------------------
  * Compressible MHD right-hand side and RK4 stepping (rhs_mpi.f90, step_rk4.f90, diffusion.f90)
