@@ -164,6 +164,16 @@ def run_c5(args):
     # feedback of a run)
     mask = CL.field_gate(net, snaps[val_id], ts, threshold=0.95)
     before, after = CL.apriori_gain(net, snaps[test_id], ts, mask)
+    # the correction's gain, set a posteriori on a calibration case (seeds 8/9) apart from the
+    # evaluated one
+    gains = [0.0, 0.1, 0.2, 0.3, 0.5, 0.75, 1.0]
+    cc = Config(nr=args.nr, nth=args.nth, nphi=args.nphi, sld_coef=args.sld_coef, seed_s=8, seed_a=9)
+
+    def _gain_progress(g, sc, w):
+        print(f"c5: calibration gain {g} mean stat err {sc:.5g} worst {w:.5g}", file=sys.stderr, flush=True)
+
+    gain, calib_score, calib_worst = CL.calibrate_gain(cc, factor, net, ts, mask, gains, args.c5_calib_steps,
+                                                       args.c5_stat_every, "f32", progress=_gain_progress)
     # the closure tendency of a whole coarse state (the fused inference kernel), timed on the
     # FP32 fluctuation state the coarse run hands it, and on an FP64 absolute state
     from cssx.mhd import Solver
@@ -229,6 +239,9 @@ def run_c5(args):
                               "coarse_closure": {"fields": [float(f"{x:.3e}") for x in ec], "ke": float(f"{kc:.3e}"),
                                                  "scalar_output": {k: float(f"{v:.3e}") for k, v in sc_err.items()}},
                               "closure_within_5pct": matched},
+        "closure_gain": {"gains": gains, "calibration_mean_stat_rel_err": [float(f"{x:.5g}") for x in calib_score],
+                         "calibration_worst_stat_rel_err": [float(f"{x:.5g}") for x in calib_worst],
+                         "chosen": gain, "calibration_case": "seeds 8/9", "calibration_coarse_steps": args.c5_calib_steps},
         "apriori_residual_rms": {"fields": list(CL.FIELDS), "gate": [bool(x) for x in mask],
                                  "held_out": {"validation": val_id, "test": test_id},
                                  "plain": [float(f"{x:.3e}") for x in before],
@@ -356,6 +369,7 @@ def main():
     ap.add_argument("--c5-epochs", type=int, default=10)
     ap.add_argument("--c5-steps", type=int, default=1000, help="closure: coarse steps of the comparison runs")
     ap.add_argument("--c5-stat-every", type=int, default=10)
+    ap.add_argument("--c5-calib-steps", type=int, default=400, help="closure: coarse steps of the gain calibration run")
     ap.add_argument("--c5-sld", type=int, default=1,
                     help="closure targets include each grid's SLD diffusion (1) or the RHS alone (0)")
     ap.add_argument("--c3-patches", type=int, default=4096)

@@ -110,7 +110,7 @@ static int lanes_per_pencil(int n, int MR) {
 template <typename T>
 static void fill_launch_geom(DerivLaunch<T> &L, const AxisGeom &g) {
     L.n = g.n;
-    L.M = strided_M(g.n);
+    L.M = strided_M(g.n, sizeof(T) == 8);
     L.S = seg_count(g.n, L.M);
     L.na = g.na; L.nb = g.nb; L.sa = g.sa; L.sb = g.sb; L.srow = g.srow;
     L.njobs = 0;
@@ -129,7 +129,7 @@ static void launch_family(const DerivSpec *sp, int nspec, const AxisGeom &g, boo
     bool contig = !wide;
     for (int i = 0; i < nspec; i++) contig = contig && use_contig(g, sp[i].order, sp[i].hi_kind, f64);
     variant = wide ? 2 : contig ? 1 : 0;
-    seg = wide ? WM : contig ? choose_MR(g.n, f64) : strided_M(g.n);
+    seg = wide ? WM : contig ? choose_MR(g.n, f64) : strided_M(g.n, f64);
 }
 
 PlanKey launch_plan_key(const DerivSpec *specs, int nspec, int i, const AxisGeom &g, bool f64) {
@@ -153,6 +153,7 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
     const bool wide = variant == 2, contig = variant == 1;
     if (contig) { L.M = seg; L.S = lanes_per_pencil(g.n, L.M); }
     if (wide) { L.M = seg; L.S = (g.n + seg - 1) / seg; }
+    double passes = 0.0;
     for (int i = 0; i < nspec; i++) {
         const DerivSpec &s = (&sp)[i];
         const PlanKey key = launch_plan_key(&sp, nspec, i, g, f64);
@@ -171,14 +172,18 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
         J.scale = (T)s.scale; J.neu_mul = (T)s.neu_mul; J.halo_s = (T)s.halo_s;
         J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
         J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind; J.lo_r0_val = s.lo_r0_val;
+        // algorithmic passes over the field: read f, write f' (and read the old f' when
+        // accumulating into it)
+        passes += s.beta != 0.0 ? 3.0 : 2.0;
         if (L.njobs == kMaxJobs || i == nspec - 1) {
             const int phase = g.phase;
             const int tok = timing_begin(phase, stream);
             hipError_t e = wide ? launch_deriv_wide<T>(L, s.order, stream)
                                 : contig ? launch_deriv_contig<T>(L, s.order, stream) : launch_deriv<T>(L, s.order, stream);
-            timing_end(tok, stream, (double)L.njobs * g.n * (double)g.na * g.nb * 2.0 * sizeof(T));
+            timing_end(tok, stream, passes * g.n * (double)g.na * g.nb * sizeof(T));
             if (e != hipSuccess) return CSSX_EHIP;
             L.njobs = 0;
+            passes = 0.0;
         }
     }
     return CSSX_OK;
@@ -245,7 +250,8 @@ extern "C" int cssx_ctx_create(const cssx_grid *grid, cssx_ctx **out) {
                                 continue;
                             key.variant = var;
                             for (int prec = 0; prec < 2; prec++) {
-                                key.seg = var == 0 ? strided_M(ns[ax]) : choose_MR(ns[ax], prec == CSSX_PREC_F64);
+                                key.seg = var == 0 ? strided_M(ns[ax], prec == CSSX_PREC_F64)
+                                                   : choose_MR(ns[ax], prec == CSSX_PREC_F64);
                                 int rc = get_plan(key, prec, &p);
                                 if (rc) {
                                     delete c;

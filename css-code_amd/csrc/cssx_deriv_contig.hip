@@ -113,8 +113,17 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
         }
     }
     if (J.pre) {
+        // a row profile: the lane's run as 16-byte vectors where the pencil splits into whole
+        // runs and the table is 16-byte aligned (lanes past the pencil read the last run)
+        T pv[MR];
+        if (FULL && (reinterpret_cast<uintptr_t>(J.pre) & 15) == 0) {
+            load_run<T, MR>(J.pre + min(st, n - MR), pv);
+        } else {
 #pragma unroll
-        for (int k = 0; k < MR; k++) v[k] *= J.pre[min(st + k, n - 1)];
+            for (int k = 0; k < MR; k++) pv[k] = J.pre[min(st + k, n - 1)];
+        }
+#pragma unroll
+        for (int k = 0; k < MR; k++) v[k] *= pv[k];
     }
     if constexpr (FULL) {
         if (!active) {
@@ -301,15 +310,22 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     const T scale = J.scale, beta = J.beta;
     const bool has_pm = J.mul_a || J.mul_b;
     const T pm = (J.mul_a ? J.mul_a[a] : T(1)) * (J.mul_b ? J.mul_b[b] : T(1));
-    T o[MR], tvl[MR], tvr[MR];
+    T o[MR], tvl[MR], tvr[MR], pw[MR];
     lds_rows<T, MR>(vL, nlT, j, tvl);
     lds_rows<T, MR>(vR, nlT, j, tvr);
+    if (post) {   // as pre above
+        if (FULL && (reinterpret_cast<uintptr_t>(post) & 15) == 0) {
+            load_run<T, MR>(post + min(st, n - MR), pw);
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < MR; jj++) pw[jj] = post[min(st + jj, n - 1)];
+        }
+    }
 #pragma unroll
     for (int jj = 0; jj < MR; jj++) {
-        const int row = min(st + jj, n - 1);
         T x = local(jj) ? d[jj] + tvl[jj] * xl + tvr[jj] * xr : xr;
         o[jj] = scale * x;
-        if (post) o[jj] *= post[row];
+        if (post) o[jj] *= pw[jj];
         if (has_pm) o[jj] *= pm;
     }
     if (beta != T(0)) {

@@ -6,13 +6,14 @@
 
 namespace cssx {
 
-int strided_M(int n) {
+int strided_M(int n, bool f64) {
     static const int forced = [] {
         const char *e = std::getenv("CSSX_DERIV_M");
         const int v = e ? std::atoi(e) : 0;
         return (v == 8 || v == 16 || v == 32 || v == 64) ? v : 0;
     }();
     if (forced && seg_count(n, forced) <= kMaxSeg && n >= forced) return forced;
+    if (!f64 && n > 128 && n <= 256) return 32;
     return choose_M(n);
 }
 

@@ -72,7 +72,10 @@ inline int choose_M(int n) {
 // environment variable CSSX_DERIV_M (8, 16, 32 or 64) names a size that fits (S <= kMaxSeg);
 // read once per process.  A tuning hook for the segment length, which trades the serial
 // sweep length against the halo rows re-read per segment.
-int strided_M(int n);
+// FP32 pencils of 129-256 rows take 32-row segments (8 per pencil: fewer halo rows re-read per
+// row; measured 9% faster on 256-row r pencils than 16-row segments); FP64 keeps choose_M,
+// whose 32-row instantiation would spill.
+int strided_M(int n, bool f64 = true);
 
 // Tridiagonal coefficients l_j, u_j (diagonal 1) of the compact system.
 //   order 1: alpha = 1/3, wall rows derivatives.f90:108-125 (row 0 upper = 4 for codes 1,3

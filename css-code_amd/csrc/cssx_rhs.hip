@@ -1002,11 +1002,14 @@ template <typename T>
 static int upload_profiles(cssx_mhd *m, Prof<T> &P) {
     const int nr = m->nr, nth = m->nth;
     const std::vector<double> &drbar = m->h_drbar, &dsbar = m->h_dsbar, &hres = m->h_hres;
-    // the profiles back to back in one host vector that grows with every table put into it
+    // the profiles back to back in one host vector that grows with every table put into it;
+    // each table starts on a 16-element boundary, so that a kernel can read a lane's run of a
+    // theta profile as 16-byte vectors
     std::vector<T> h;
-    h.reserve(32 * (size_t)nr + 12 * (size_t)nth + 2);
+    h.reserve(32 * (size_t)nr + 12 * (size_t)nth + 1024);
     size_t o = 0;
     auto put = [&](size_t n, auto gen) {
+        o = (o + 15) & ~(size_t)15;
         const size_t at = o;
         h.resize(o + n);
         for (size_t i = 0; i < n; i++) h[o + i] = (T)gen(i);
@@ -1518,9 +1521,9 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
     const double b_speed = ts * N * (1.0 + (sp.dstype == 1 ? 3.0 + (sp.magnetic ? 4.0 : 0.0) : 0.0));
     CSSX_TIMED(CSSX_PHASE_SLD, stream, b_speed,
                hipLaunchKernelGGL(k_sld_speed<T>, dim3(nb), dim3(256), 0, stream, c, P, sp, st, B, vcc));
-    // tiles of 64 theta x 4 phi columns (one thread each); the r range is cut into segments of
+    // tiles of 64 theta x kSldRows phi columns (one thread each); the r range is cut into segments of
     // at least 16 levels so that the launch holds ~16K wavefronts where the grid allows
-    const int ntt = (m->nth + 63) / 64, ntp = (m->nphi + 3) / 4;
+    const int ntt = (m->nth + 63) / 64, ntp = (m->nphi + kSldRows - 1) / kSldRows;
     const long tiles = (long)ntt * ntp;
     int nseg = (int)std::min<long>((4096 + tiles - 1) / tiles, std::max(1, m->nr / 16));
     nseg = std::max(nseg, 1);
@@ -1535,17 +1538,17 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
     const double na = sp.last >= 5 ? (double)(sp.last - 4) : 0.0;   // diffused A components
     auto two_pass = [&]() {
         CSSX_TIMED(CSSX_PHASE_SLD, stream, b_march - ts * N * 2.0 * na,
-                   hipLaunchKernelGGL((k_sld_col<T, 0, 5>), grid, dim3(256), 0, stream, c, P, sp, st, (const T *)vcc,
+                   hipLaunchKernelGGL((k_sld_col<T, 0, 5>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st, (const T *)vcc,
                                       dif, ntt, ntp, rseg));
         if (na > 0.0)
             CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (2.0 * na + 1.0),
-                       hipLaunchKernelGGL((k_sld_col<T, 5, 3>), grid, dim3(256), 0, stream, c, P, sp, st,
+                       hipLaunchKernelGGL((k_sld_col<T, 5, 3>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st,
                                           (const T *)vcc, dif, ntt, ntp, rseg));
     };
     if constexpr (sizeof(T) == 4) {
         if (sp.last >= 5 && !sld_split())
             CSSX_TIMED(CSSX_PHASE_SLD, stream, b_march,
-                       hipLaunchKernelGGL((k_sld_col<T, 0, 8>), grid, dim3(256), 0, stream, c, P, sp, st,
+                       hipLaunchKernelGGL((k_sld_col<T, 0, 8>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st,
                                           (const T *)vcc, dif, ntt, ntp, rseg));
         else
             two_pass();

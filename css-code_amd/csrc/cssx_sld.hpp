@@ -47,9 +47,15 @@ template <typename T>
 __device__ __forceinline__ T van_albada(T q, T qph, T tiny, T itiny2) {
     const T s = q * qph;
     const T a = fabs(q);
-    const T r = a >= tiny ? (fabs(qph) < a ? qph : q) : q * fmin(T(1), s * itiny2);
+    const T t = s * itiny2;
+    const T r = a >= tiny ? (fabs(qph) < a ? qph : q) : q * (t < T(1) ? t : T(1));   // min(1, t)
     return s > T(0) ? r : T(0);
 }
+
+// the SLD face speed sld_coef sqrt(x): FP32 takes the hardware square root (1 ulp), FP64 the
+// correctly rounded one
+__device__ __forceinline__ float sld_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double sld_sqrt(double x) { return sqrt(x); }
 
 // limited face difference delue = qph - (phiR + phiL)/2 of the face between f0 and fm1 (:766)
 template <typename T>
@@ -142,6 +148,9 @@ __device__ T sld_ghost_t(const Consts<T> &c, const Prof<T> &P, const SldParams<T
     }
 }
 
+// phi rows (wavefronts) per workgroup of k_sld_col
+constexpr int kSldRows = 8;
+
 // logical block index with each XCD (8 per chip, workgroups dealt round-robin) walking a
 // contiguous range of tiles, so that neighbouring tiles share their halo rows in one L2
 __device__ __forceinline__ long xcd_block(long b, long nb) {
@@ -150,8 +159,9 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
 
 // Compute_Diffusion's fluxes (Diffusion_R :476-891, Diffusion_Th :893-1263, Diffusion_Phi
 // :1265-1495) and the viscous-heating / mass-diffusion corrections (:375-428) of every field.
-// One thread per (theta, phi) column: a workgroup is 4 wavefronts, 64 consecutive theta columns
-// each, on 4 consecutive phi rows, and it marches up the r levels [r0, r0 + rseg).  Each thread
+// One thread per (theta, phi) column: a workgroup is kSldRows wavefronts, 64 consecutive theta
+// columns each, on kSldRows consecutive phi rows (the phi neighbours two rows out of the tile
+// come from other workgroups: 8 rows keep that overhang to half the tile), and it marches up the r levels [r0, r0 + rseg).  Each thread
 // keeps a four-level window of its column of every diffused field in registers and carries the
 // upper r face's flux to the next level as that level's lower face, so every r face is
 // limited once.  The theta and phi faces of a cell are evaluated from the neighbouring values
@@ -160,7 +170,7 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
 // The tile index comes from the flat block index, XCD-contiguous, so that phi-neighbouring
 // rows share one L2.
 template <typename T, int F0, int NF>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
+__global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
                                                   const T *__restrict__ st, const T *__restrict__ vcc,
                                                   T *__restrict__ D, int ntt, int ntp, int rseg) {
     // fields [F0, F0 + NF): rho..s (with the point corrections, which need all five) and A
@@ -172,7 +182,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     const int tt = (int)(lb % ntt), tp = (int)((lb / ntt) % ntp), seg = (int)(lb / ((long)ntt * ntp));
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nth = c.nth, nphi = c.nphi, nr = c.nr;
-    const int it = tt * 64 + lane, ip = tp * 4 + wv;
+    const int it = tt * 64 + lane, ip = tp * kSldRows + wv;
     if (it >= nth || ip >= nphi) return;
     const long N = c.N, plane = (long)nth * nphi;
     const int r0 = seg * rseg, r1 = min(nr, r0 + rseg);
@@ -217,7 +227,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
     {
         const T vlow = colvcc(r0 - 1);
         const T rl = P.rfc[r0];
-        const T spd = sp.coef * sqrt(T(0.5) * (vc_cur + vlow));
+        const T spd = sp.coef * sld_sqrt(T(0.5) * (vc_cur + vlow));
         // levels r0-2 .. r0+2 clamped into the grid, then the wall ghosts where a level lies
         // outside it (uniform branches: only the first and the last segment take them)
 #pragma unroll
@@ -265,9 +275,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
         // face speeds sld_coef sqrt(mean of the two cells' speed^2); ghost cells copy the nearest
         // cell (diffusion.f90:290-302)
         const T vtm = ldv(ir, otm1), vtp = ldv(ir, otp1), vpm = ldv(ir, opm1), vpp = ldv(ir, opp1);
-        const T s_tl = sp.coef * sqrt(T(0.5) * (vc_cur + vtm)), s_th = sp.coef * sqrt(T(0.5) * (vtp + vc_cur));
-        const T s_pl = sp.coef * sqrt(T(0.5) * (vpm + vc_cur)), s_ph = sp.coef * sqrt(T(0.5) * (vc_cur + vpp));
-        const T spd = sp.coef * sqrt(T(0.5) * (vc_next + vc_cur));
+        const T s_tl = sp.coef * sld_sqrt(T(0.5) * (vc_cur + vtm)), s_th = sp.coef * sld_sqrt(T(0.5) * (vtp + vc_cur));
+        const T s_pl = sp.coef * sld_sqrt(T(0.5) * (vpm + vc_cur)), s_ph = sp.coef * sld_sqrt(T(0.5) * (vc_cur + vpp));
+        const T spd = sp.coef * sld_sqrt(T(0.5) * (vc_next + vc_cur));
         const T rh = P.rfc[ir + 1];
         const T gr = P.rv[ir], gtt = P.rfac[ir] * P.ct[it], gp = P.gt[it] * (P.rfac[ir] * sp.invdphi);
         T out[5];

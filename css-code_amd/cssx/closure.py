@@ -467,6 +467,64 @@ def compare(fine_cfg: Config, factor: int, net: Network | None, coarse_steps: in
                   "dt_fine": dtf}
 
 
+def calibrate_gain(fine_cfg: Config, factor: int, net: Network, ts: TrainingSet, mask, gains, coarse_steps: int,
+                   every: int, precision: str = "f32", progress=None):
+    """A-posteriori gain of the gated correction: the coarse grid runs with the correction
+    scaled by each gain in `gains` (0 = the plain fallback) from the filtered initial state of
+    `fine_cfg` (a calibration case apart from the evaluated one), beside the fine run filtered
+    every `every` coarse steps.  An a-priori gain on snapshots does not survive the feedback of
+    a run (the corrected field drifts), so the gain is set by the run itself.  Each statistic's
+    error is its largest relative error over the records; a gain is scored by the mean of those
+    over all statistics (the largest alone is often a statistic no gain changes, such as the
+    magnetic energy when A is not corrected).  Returns the chosen gain, each gain's score and
+    each gain's largest error, and leaves the network at the chosen gain."""
+    fine = Solver(Model(fine_cfg))
+    cmodel = Model(coarse_config(fine_cfg, factor))
+    plain = Solver(cmodel)
+    closed = Solver(cmodel)
+    y = fine.model.initial_state(device="cuda")
+    dtf = fine.cfl(y)
+    yc0 = filter_state(y, fine, plain, factor)
+    dtc = min(factor * dtf, plain.cfl(yc0))
+    dtf = dtc / factor
+    fine_recs = []
+    for k0 in range(0, coarse_steps, every):
+        n = min(every, coarse_steps - k0)
+        fine.rk4(y, dtf, n * factor, precision)
+        fine_recs.append(stats_of(filter_state(y, fine, plain, factor), plain))
+    fine.close()
+    del y
+    mask = np.asarray(mask, bool)
+    score, worst = [], []
+    _lib.check(_lib.lib().cssx_mhd_set_closure(closed._h, net.handle), "cssx_mhd_set_closure")
+    for g in gains:
+        net.set_norm(ts.x_mean, ts.x_std, np.where(mask, g * ts.y_mean, 0.0), np.where(mask, g * ts.y_std, 0.0))
+        yq = yc0.clone()
+        err = None   # per statistic, the largest relative error over the records
+        for i, k0 in enumerate(range(0, coarse_steps, every)):
+            n = min(every, coarse_steps - k0)
+            closed.rk4(yq, dtc, n, precision)
+            if not torch.isfinite(yq).all():
+                err = np.full(1, np.inf)
+                break
+            a, b = stats_of(yq, plain), fine_recs[i]
+            keys = ("ke",) + tuple(k for k in SCALARS if k in b)
+            e = np.concatenate([np.abs(np.asarray(a["rms"]) - np.asarray(b["rms"])) /
+                                np.maximum(np.abs(np.asarray(b["rms"])), 1e-300),
+                                [abs(a[k] - b[k]) / max(abs(b[k]), 1e-300) for k in keys]])
+            err = e if err is None else np.maximum(err, e)
+        score.append(float(np.mean(err)))
+        worst.append(float(np.max(err)))
+        if progress is not None:
+            progress(g, score[-1], worst[-1])
+    _lib.check(_lib.lib().cssx_mhd_set_closure(closed._h, None), "cssx_mhd_set_closure")
+    closed.close()
+    plain.close()
+    best = gains[int(np.argmin(score))]
+    net.set_norm(ts.x_mean, ts.x_std, np.where(mask, best * ts.y_mean, 0.0), np.where(mask, best * ts.y_std, 0.0))
+    return best, score, worst
+
+
 def stat_errors(recs, key: str):
     """Max over records of the relative error of each field's RMS and of the kinetic energy
     against the filtered fine run (records after the initial one)."""
