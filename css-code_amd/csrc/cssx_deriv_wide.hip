@@ -32,8 +32,12 @@ namespace cssx {
 // capacity of each interface exchange array: 1024 slots plus one padding slot per row
 constexpr int kWideShared = 1024 + 64;
 
+// FP32: at most 64 VGPRs (8 waves per SIMD), so that two 16-wave workgroups share a CU and one
+// workgroup's loads overlap the other's solve and barriers; at 93 VGPRs a CU held one
+// workgroup and the sweep ran 17% slower.
 template <typename T, int ORDER, int M, int G>
-__global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 8 : 1)))
+void k_deriv_wide(const DerivLaunch<T> L) {
     constexpr int P = 64 / G;    // pencils per workgroup
     constexpr int SM = 16 * G;   // segment slots per pencil
     constexpr int W = M + 8;     // window w[i] = row st-4+i
@@ -86,7 +90,8 @@ __global__ __launch_bounds__(1024) void k_deriv_wide(const DerivLaunch<T> L) {
         w[i] = bload<T>(rin, lofs + (unsigned)q * rowb, 0u);
         if (J.pre) w[i] *= J.pre[q];
     }
-    if (!periodic) {
+    // only the end segments' windows leave the pencil
+    if (!periodic && (st < 4 || st + M + 4 > n)) {
 #pragma unroll
         for (int i = 0; i < W; i++) {
             const int q = st - 4 + i;
