@@ -143,6 +143,31 @@ PlanKey launch_plan_key(const DerivSpec *specs, int nspec, int i, const AxisGeom
     return key;
 }
 
+// Appends spec i of a launch of `nspec` specs along g to L, with its plan (the spec's own or the
+// cache's).
+template <typename T>
+static int add_job(DerivLaunch<T> &L, const DerivSpec *specs, int nspec, int i, const AxisGeom &g) {
+    const bool f64 = sizeof(T) == 8;
+    const DerivSpec &s = specs[i];
+    const PlanKey key = launch_plan_key(specs, nspec, i, g, f64);
+    const void *plan = s.plan;
+    if (!plan) {
+        int rc = get_plan(key, f64 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
+        if (rc) return rc;
+    }
+    DerivJob<T> &J = L.jobs[L.njobs++];
+    J.in = (const T *)s.in; J.out = (T *)s.out; J.rhs_out = (T *)s.rhs_out;
+    J.glo = (const T *)s.glo; J.ghi = (const T *)s.ghi;
+    J.g_k = s.g_k >= 0 ? s.g_k : g.g_k; J.g_a = s.g_k >= 0 ? s.g_a : g.g_a; J.g_b = s.g_k >= 0 ? s.g_b : g.g_b;
+    J.plan = (const T *)plan;
+    J.pre = (const T *)s.pre; J.post = (const T *)s.post;
+    J.mul_a = (const T *)s.mul_a; J.mul_b = (const T *)s.mul_b;
+    J.scale = (T)s.scale; J.neu_mul = (T)s.neu_mul; J.halo_s = (T)s.halo_s;
+    J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
+    J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind; J.lo_r0_val = s.lo_r0_val;
+    return CSSX_OK;
+}
+
 template <typename T>
 static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStream_t stream) {
     DerivLaunch<T> L;
@@ -156,22 +181,8 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
     double passes = 0.0;
     for (int i = 0; i < nspec; i++) {
         const DerivSpec &s = (&sp)[i];
-        const PlanKey key = launch_plan_key(&sp, nspec, i, g, f64);
-        const void *plan = s.plan;
-        if (!plan) {
-            int rc = get_plan(key, f64 ? CSSX_PREC_F64 : CSSX_PREC_F32, &plan);
-            if (rc) return rc;
-        }
-        DerivJob<T> &J = L.jobs[L.njobs++];
-        J.in = (const T *)s.in; J.out = (T *)s.out; J.rhs_out = (T *)s.rhs_out;
-        J.glo = (const T *)s.glo; J.ghi = (const T *)s.ghi;
-        J.g_k = s.g_k >= 0 ? s.g_k : g.g_k; J.g_a = s.g_k >= 0 ? s.g_a : g.g_a; J.g_b = s.g_k >= 0 ? s.g_b : g.g_b;
-        J.plan = (const T *)plan;
-        J.pre = (const T *)s.pre; J.post = (const T *)s.post;
-        J.mul_a = (const T *)s.mul_a; J.mul_b = (const T *)s.mul_b;
-        J.scale = (T)s.scale; J.neu_mul = (T)s.neu_mul; J.halo_s = (T)s.halo_s;
-        J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
-        J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind; J.lo_r0_val = s.lo_r0_val;
+        int rc = add_job(L, &sp, nspec, i, g);
+        if (rc) return rc;
         // algorithmic passes over the field: read f, write f' (and read the old f' when
         // accumulating into it)
         passes += s.beta != 0.0 ? 3.0 : 2.0;

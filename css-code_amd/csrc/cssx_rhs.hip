@@ -88,6 +88,20 @@ enum Slot {
     D_NSLOT
 };
 
+// Where the theta first derivatives of one point are: w v u s bp br at p[k*stride], k = D_TW ..
+// D_TBR (the derivative scratch: p = D + i, stride N).  Each value is read where the terms use
+// it, so that no register holds it longer.
+template <typename T>
+struct ThetaD {
+    const T *p;
+    long stride;
+    __device__ __forceinline__ T operator[](int k) const { return p[k * stride]; }
+};
+template <typename T>
+__device__ __forceinline__ ThetaD<T> theta_from(const T *D, long N, long i) {
+    return ThetaD<T>{D + i, N};
+}
+
 // explicit-diffusion scratch slots (Laplacian / SLaplacian)
 enum LapSlot {
     L_TW2 = 0, L_TV2, L_TU2, L_TS2,          // theta second derivatives: w v u s
@@ -341,8 +355,8 @@ __global__ __launch_bounds__(256) void k_walls_state(Consts<T> c, Prof<T> P, Wal
 // The terms of point i (theta index it, r index ir) in out[0..4] = drho, dw, dv, du, ds.
 template <typename T>
 __device__ __forceinline__ void main_terms(const Consts<T> &c, const Prof<T> &P, const T *__restrict__ st,
-                                           const T *__restrict__ B, const T *__restrict__ D, long i, int it,
-                                           int ir, T (&out)[5]) {
+                                           const T *__restrict__ B, const T *__restrict__ D, const ThetaD<T> &td,
+                                           long i, int it, int ir, T (&out)[5]) {
     const long N = c.N;
     const T ri = P.rinv[ir], rv = P.r[ir], epsr = P.eps[ir];
     const T omsq = rv * c.omega0 * c.omega0, om2 = c.om2;
@@ -363,23 +377,23 @@ __device__ __forceinline__ void main_terms(const Consts<T> &c, const Prof<T> &P,
     xdrho = -Rho * W * cot * ri;
     xdv = -om2 * cs * W;
     xdu = xdu + ri * W * W;
-    wk = D[D_TW * N + i] * ri;
+    wk = td[D_TW] * ri;
     xdrho = xdrho - Rho * wk;
     xdw = xdw - W * wk;
     xdw = xdw + V * (ri * cot * V + om2 * cs);
     xdv = xdv - ri * V * W * cot;
     xdu = xdu + V * (ri * V + om2 * sn);
-    wk = D[D_TV * N + i] * ri;
+    wk = td[D_TV] * ri;
     xdv = xdv - W * wk;
     drho_out = xdrho - T(2) * ri * Rho * U;
     xdv = xdv - U * (ri * V + om2 * sn);
     xdw = xdw - ri * U * W;
     if (!mag) dv_out = xdv;
-    wk = D[D_TU * N + i] * ri;
+    wk = td[D_TU] * ri;
     xdu = xdu - W * wk;
     if (!mag) du_out = xdu;
     xds = (fabs(epsr) > T(0)) ? epsr * invrho / t : T(0);
-    wk = D[D_TS * N + i] * ri;
+    wk = td[D_TS] * ri;
     if (mag) xdw = xdw - c.gamm1 * t * wk;
     else dw_out = xdw - c.gamm1 * t * wk;
     xds = xds - W * wk;
@@ -389,10 +403,10 @@ __device__ __forceinline__ void main_terms(const Consts<T> &c, const Prof<T> &P,
         xdw = xdw - Bp * Bp * cot * oorho;
         xdv = xdv + Bp * Bt * cot * oorho;
         xdu = xdu - Bp * Bp * oorho;
-        wk = D[D_TBP * N + i] * oorho;
+        wk = td[D_TBP] * oorho;
         xdv = xdv + Bt * wk;
         xdw = xdw - Bp * wk;
-        wk = D[D_TBR * N + i] * oorho;
+        wk = td[D_TBR] * oorho;
         dw_out = xdw + Br * (Bt * oorho - wk);
         dv_out = xdv + Br * Bp * oorho;
         du_out = xdu + Bt * wk;
@@ -508,8 +522,8 @@ __global__ __launch_bounds__(256) void k_lap_walls(Consts<T> c, Prof<T> P, const
 // adds the explicit-diffusion terms of point i to out[1..4] (dw, dv, du, ds)
 template <typename T>
 __device__ __forceinline__ void viscous_terms(const Consts<T> &c, const Prof<T> &P, const T *__restrict__ st,
-                                              const T *__restrict__ D, const T *__restrict__ L, long i, int it,
-                                              int ir, T (&out)[5]) {
+                                              const T *__restrict__ D, const ThetaD<T> &td, const T *__restrict__ L,
+                                              long i, int it, int ir, T (&out)[5]) {
     const long N = c.N;
     const T ri = P.rinv[ir], csc = P.csc[it], cot = P.cot[it];
     const T W = st[N + i], V = st[2 * N + i], U = st[3 * N + i];
@@ -519,7 +533,7 @@ __device__ __forceinline__ void viscous_terms(const Consts<T> &c, const Prof<T> 
     T ddw = T(0), ddv = T(0), ddu = T(0), dds = T(0);
     if (c.lap) {
         const T nu = P.mur[ir] * invrho, dlm = P.dlnmu[ir], w5 = T(2) * ri + dlm, csc2 = csc * csc;
-        const T tw = D[D_TW * N + i] * ri, tv = D[D_TV * N + i] * ri, tu = D[D_TU * N + i] * ri;
+        const T tw = td[D_TW] * ri, tv = td[D_TV] * ri, tu = td[D_TU] * ri;
         const T pv = D[D_PV * N + i] * oors, pw = D[D_PW * N + i] * oors, pu = D[D_PU * N + i] * oors;
         const T rw = D[D_RW * N + i], rv = D[D_RV * N + i], ru = D[D_RU * N + i];
         const T c43 = T(4) / T(3), c13 = T(1) / T(3), c73 = T(7) / T(3), c23 = T(2) / T(3);
@@ -574,7 +588,7 @@ __device__ __forceinline__ void viscous_terms(const Consts<T> &c, const Prof<T> 
         // thermal diffusion (:959-975, :1221-1228, :1456-1472); the theta second-derivative term
         // carries 1/r^3 as in the reference (:970-973)
         const T ks = P.ks[ir], ort = invrho * invt;
-        const T ts = D[D_TS * N + i] * ri;
+        const T ts = td[D_TS] * ri;
         T rs = D[D_RS * N + i];
         T d2r = L[L_RS2 * N + i];
         if (c.fluct) {
@@ -597,9 +611,10 @@ __device__ __forceinline__ void viscous_terms(const Consts<T> &c, const Prof<T> 
 template <typename T>
 __device__ __forceinline__ void pre_density_terms(const Consts<T> &c, const Prof<T> &P, const T *__restrict__ st,
                                                   const T *__restrict__ B, const T *__restrict__ D,
-                                                  const T *__restrict__ L, long i, int it, int ir, T (&out)[5]) {
-    main_terms(c, P, st, B, D, i, it, ir, out);
-    if (c.slap) viscous_terms(c, P, st, D, L, i, it, ir, out);
+                                                  const ThetaD<T> &td, const T *__restrict__ L, long i, int it,
+                                                  int ir, T (&out)[5]) {
+    main_terms(c, P, st, B, D, td, i, it, ir, out);
+    if (c.slap) viscous_terms(c, P, st, D, td, L, i, it, ir, out);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -620,8 +635,8 @@ __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, W
         const long q0 = ((long)ir * c.nphi + ip) * c.nth, q1 = q0 + c.nth - 1;
         const Thermo<T> a0 = thermo(st[q0], st[4 * N + q0], ir, c, P), a1 = thermo(st[q1], st[4 * N + q1], ir, c, P);
         const T rho0 = a0.rho, rho1 = a1.rho, t0 = a0.t, t1 = a1.t;
-        pre_density_terms(c, P, st, B, D, L, q0, 0, ir, o0);
-        pre_density_terms(c, P, st, B, D, L, q1, c.nth - 1, ir, o1);
+        pre_density_terms(c, P, st, B, D, theta_from(D, N, q0), L, q0, 0, ir, o0);
+        pre_density_terms(c, P, st, B, D, theta_from(D, N, q1), L, q1, c.nth - 1, ir, o1);
         wall[W.off_trho_lo + i] = rv * o0[1] * rho0 / t0;
         wall[W.off_trho_hi + i] = rv * o1[1] * rho1 / t1;
     }
@@ -634,11 +649,11 @@ __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, W
         const bool nlo = lbcr_rho == 2 || lbcr_rho == 4, nhi = lbcr_rho == 3 || lbcr_rho == 4;
         T du0 = T(0), du1 = T(0);
         if (nlo) {
-            pre_density_terms(c, P, st, B, D, L, q0, it, 0, o0);
+            pre_density_terms(c, P, st, B, D, theta_from(D, N, q0), L, q0, it, 0, o0);
             du0 = o0[3];
         }
         if (nhi) {
-            pre_density_terms(c, P, st, B, D, L, q1, it, c.nr - 1, o1);
+            pre_density_terms(c, P, st, B, D, theta_from(D, N, q1), L, q1, it, c.nr - 1, o1);
             du1 = o1[3];
         }
         T lo = nlo ? du0 * rho0 * c.invpconst / t0 : rho0;
@@ -725,31 +740,30 @@ __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<
     }
 }
 
+// Every term of point i from the theta derivatives td, the density derivatives *dthrho (theta)
+// and *drrho (r) and the phi / r derivative slots of D, then the fused stage (or dv).
 template <typename T>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 7) void k_rhs_final(Consts<T> c, Prof<T> P, const T *st,
-                                                   const T *__restrict__ B, const T *__restrict__ dth_rho,
-                                                   const T *__restrict__ dr_rho, T *dv, StageFuse<T> F,
-                                                   const float *__restrict__ ext, const T *__restrict__ D,
-                                                   const T *__restrict__ L) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.N) return;
+__device__ __forceinline__ void final_point(const Consts<T> &c, const Prof<T> &P, const T *st,
+                                            const T *__restrict__ B, const ThetaD<T> &td, const T *dthrho,
+                                            const T *drrho,
+                                            T *dv, const StageFuse<T> &F, const float *__restrict__ ext,
+                                            const T *__restrict__ D, const T *__restrict__ L, long i, int it,
+                                            int ip, int ir) {
     const long N = c.N;
-    int it, ip, ir;
-    coords(i, c, it, ip, ir);
     const T W = st[N + i], V = st[2 * N + i], U = st[3 * N + i];
     const Thermo<T> th = thermo(st[i], st[4 * N + i], ir, c, P);
     const T Rho = th.rho, t = th.t;
     const T invrho = T(1) / Rho;
     T o[5];
-    pre_density_terms(c, P, st, B, D, L, i, it, ir, o);
+    pre_density_terms(c, P, st, B, D, td, L, i, it, ir, o);
     T drho = o[0], dw = o[1], du = o[3], ds = o[4];
     // Density_Terms_Theta
-    T wk = P.rinv[ir] * dth_rho[i];
+    T wk = P.rinv[ir] * *dthrho;
     drho = drho - W * wk;
     wk = wk * invrho;
     dw = dw - c.pconst * wk * t;
     // Density_Terms_Radial
-    wk = dr_rho[i];
+    wk = *drrho;
     if (c.fluct) {
         // pconst (T/rho) (drho'/dr + drbar) minus the background part already in hres
         du = du - c.pconst * ((t * invrho) * wk + P.tr[ir] * th.a * P.drbar[ir]);
@@ -824,6 +838,19 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 7) void k_rhs_final(Const
             stage_value(F, c, 7, i, it, ip, ir, da2);
         }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 7) void k_rhs_final(Consts<T> c, Prof<T> P, const T *st,
+                                                   const T *__restrict__ B, const T *__restrict__ dth_rho,
+                                                   const T *__restrict__ dr_rho, T *dv, StageFuse<T> F,
+                                                   const float *__restrict__ ext, const T *__restrict__ D,
+                                                   const T *__restrict__ L) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= c.N) return;
+    int it, ip, ir;
+    coords(i, c, it, ip, ir);
+    final_point(c, P, st, B, theta_from(D, c.N, i), dth_rho + i, dr_rho + i, dv, F, ext, D, L, i, it, ip, ir);
 }
 
 template <typename T>
