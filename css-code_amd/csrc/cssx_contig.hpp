@@ -48,24 +48,15 @@ __device__ __forceinline__ void lds_rows(const T *tab, int nlT, int j, T (&v)[MR
     }
 }
 
-// Plan rows of one lane.  LdsRows: the five row tables staged in LDS in 16-byte chunks (see
-// k_deriv_contig); GlobalRows: read from the plan buffer as 16-byte runs (FULL pencils, whose
-// rows split into whole runs; the buffer is 256-byte aligned).
+// Plan rows of one lane: the five row tables staged in LDS in 16-byte chunks (see
+// k_deriv_contig).  (Read straight from the L2-resident plan buffer instead, the 512-row FP32
+// sweep ran 13% slower.)
 template <typename T, int MR>
 struct LdsRows {
     const T *sp;
     int nT, nlT;
     __device__ __forceinline__ void rows(int c, int j, T (&v)[MR]) const { lds_rows<T, MR>(sp + c * nT, nlT, j, v); }
     __device__ __forceinline__ const T *tail() const { return sp + 5 * nT; }
-};
-template <typename T, int MR>
-struct GlobalRows {
-    const T *plan;
-    int n, nl;
-    __device__ __forceinline__ void rows(int c, int j, T (&v)[MR]) const {
-        load_run<T, MR>(plan + (long)c * n + (long)min(j, nl - 1) * MR, v);
-    }
-    __device__ __forceinline__ const T *tail() const { return plan + 5 * n; }
 };
 
 // One lane's share of a contiguous compact derivative: rows [st, st+MR) of pencil a, solved with

@@ -42,14 +42,22 @@ struct SldParams {
 
 // Van Albada slope of diffusion.f90:744-762, q * max(0, min(1, q qph / max(|q|, tiny)^2)).
 // Where |q| >= tiny this is minmod(q, qph) exactly, which needs no reciprocal; below tiny the
-// closed form q min(1, q qph / tiny^2) is kept.
+// closed form q min(1, q qph / tiny^2) is kept.  In FP32 both are medians of three: minmod(q, qph)
+// = med3(q, qph, 0) (the one nearer zero when the signs agree, else 0) and the small-q form
+// q * med3(q qph / tiny^2, 0, 1) (v_med3_f32; the same values, a zero may carry q's sign).
 template <typename T>
 __device__ __forceinline__ T van_albada(T q, T qph, T tiny, T itiny2) {
     const T s = q * qph;
-    const T a = fabs(q);
     const T t = s * itiny2;
-    const T r = a >= tiny ? (fabs(qph) < a ? qph : q) : q * (t < T(1) ? t : T(1));   // min(1, t)
-    return s > T(0) ? r : T(0);
+    if constexpr (sizeof(T) == 4) {
+        const T slow = q * __builtin_amdgcn_fmed3f(t, 0.0f, 1.0f);
+        const T fast = __builtin_amdgcn_fmed3f(q, qph, 0.0f);
+        return fabs(q) >= tiny ? fast : slow;
+    } else {
+        const T a = fabs(q);
+        const T r = a >= tiny ? (fabs(qph) < a ? qph : q) : q * (t < T(1) ? t : T(1));   // min(1, t)
+        return s > T(0) ? r : T(0);
+    }
 }
 
 // the SLD face speed sld_coef sqrt(x): FP32 takes the hardware square root (1 ulp), FP64 the
@@ -186,7 +194,10 @@ __global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(s
     if (it >= nth || ip >= nphi) return;
     const long N = c.N, plane = (long)nth * nphi;
     const int r0 = seg * rseg, r1 = min(nr, r0 + rseg);
-    const bool wall_lo = tt == 0, wall_hi = tt == ntt - 1;   // wavefront holds a theta wall cell
+    // the wavefront holds a cell whose theta stencil reaches a ghost: it < 2 (lower wall) or
+    // it + 2 >= nth (upper wall; the tile's last column is tt*64 + 63, so the tile before the
+    // last holds column nth - 2 when nth = 1 mod 64)
+    const bool wall_lo = tt == 0, wall_hi = tt * 64 + 65 >= nth;
     // Loads are raw buffer loads: a scalar resource per field, the level's byte offset in a
     // scalar register and one of nine 32-bit lane byte offsets within the level plane, shared
     // by every field, so that no per-lane address is live per field across the march (a field

@@ -81,6 +81,26 @@ def test_diffusion_matches_oracle(mhd, shape, variant):
     sol.close()
 
 
+# theta lengths over several 64-column wavefronts: 65 and 129 put column nth-2 (whose stencil
+# reads the upper ghost) on the last lane of the tile before the last, 130 spans three tiles
+@pytest.mark.parametrize("variant", ["default", "flags"])
+@pytest.mark.parametrize("shape", [(10, 65, 12), (8, 129, 10), (9, 130, 8)])
+def test_diffusion_theta_tiles_match_oracle(mhd, shape, variant):
+    nr, nth, nphi = shape
+    print(f"diffusion theta tiles {variant} {shape}", flush=True)
+    m = Model(Config(nr=nr, nth=nth, nphi=nphi, **VARIANTS[variant]))
+    st = _state(m, seed=7)
+    ref = oracle.diffusion(m, st)
+    sol = mhd(m)
+    x = torch.as_tensor(st, device="cuda")
+    got = sol.diffusion(x).cpu().numpy()
+    e = _rel_fields(got, ref)
+    assert e.max() < 1e-11, e
+    got32 = sol.diffusion(sol.to_f32(x)).double().cpu().numpy()
+    assert _rel_fields(got32, ref).max() < 1e-4
+    sol.close()
+
+
 def test_diffusion_given_b_equals_computed(mhd):
     m = Model(Config(nr=20, nth=24, nphi=16, sld_coef=0.3))
     st = _state(m, seed=2)
