@@ -696,6 +696,10 @@ struct StageFuse {
     float *acc32;
     T *y0f;
     const T *dif;
+    // stage 1 with SLD Ohmic heating: the curl of the SLD A tendency (3 fields), whose heating
+    // term (diffusion.f90:430-461) the stage subtracts from the entropy's SLD tendency here
+    // instead of a separate pass over ds
+    const T *ohm;
     T *y1;
     const double *sr1_64, *rbar64, *sbar64;
     const T *sr1_t;   // constant-temperature wall entropy in stage precision/form
@@ -703,7 +707,7 @@ struct StageFuse {
 
 template <typename T>
 __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<T> &c, int iv, long i, int it,
-                                            int ip, int ir, T kk_t) {
+                                            int ip, int ir, T kk_t, bool own_dif = false, T difv = T(0)) {
     const long j = (long)iv * c.N + i;
     const int st = F.stage;
     if constexpr (sizeof(T) == sizeof(double)) {
@@ -711,7 +715,7 @@ __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<
         double s;
         // stage 1: vars_save = D + k1/6 with SLDiffusion, vars_save = k1 without it
         // (step_rk4.f90:369-389: k1 then carries weight 1, as in the reference)
-        if (st == 1) s = F.dif ? (double)F.dif[j] + kk / 6.0 : kk;
+        if (st == 1) s = F.dif ? (double)(own_dif ? difv : F.dif[j]) + kk / 6.0 : kk;
         else s = F.acc64[j] + kk / (st == 4 ? 6.0 : 3.0);
         const double y = st == 4 ? F.y0[j] + F.dt * s : F.y0[j] + (st == 3 ? F.dt : F.dt / 2.0) * kk;
         const double yb = bound_value<double>(iv, it, ip, ir, y, F.f, c.nth, c.nr, F.sr1_64);
@@ -724,7 +728,7 @@ __device__ __forceinline__ void stage_value(const StageFuse<T> &F, const Consts<
     } else {
         const float kk = (float)kk_t;
         if (st < 4) {
-            const float a = st == 1 ? (F.dif ? (float)F.dif[j] + kk / 6.0f : kk) : F.acc32[j] + kk / 3.0f;
+            const float a = st == 1 ? (F.dif ? (float)(own_dif ? difv : F.dif[j]) + kk / 6.0f : kk) : F.acc32[j] + kk / 3.0f;
             const float h = (float)(st == 3 ? F.dt : 0.5 * F.dt);
             const float y = (float)F.y0f[j] + h * kk;
             F.acc32[j] = a;
@@ -831,7 +835,18 @@ __device__ __forceinline__ void final_point(const Consts<T> &c, const Prof<T> &P
         stage_value(F, c, 1, i, it, ip, ir, dw);
         stage_value(F, c, 2, i, it, ip, ir, dvv);
         stage_value(F, c, 3, i, it, ip, ir, du);
-        stage_value(F, c, 4, i, it, ip, ir, ds);
+        if (F.ohm && F.stage == 1 && F.dif) {
+            // SLD Ohmic heating: ds_sld -= B . curl(dA_sld/dt) / (8 pi rho T), the operations of
+            // k_sld_finish in its order
+            const T tt = T(0.5 * kOneOver4Pi) * invrho * (T(1) / t);
+            T difs = F.dif[4 * N + i];
+            difs = difs - B[i] * F.ohm[i] * tt;
+            difs = difs - B[N + i] * F.ohm[N + i] * tt;
+            difs = difs - B[2 * N + i] * F.ohm[2 * N + i] * tt;
+            stage_value(F, c, 4, i, it, ip, ir, ds, true, difs);
+        } else {
+            stage_value(F, c, 4, i, it, ip, ir, ds);
+        }
         if (c.magnetic) {
             stage_value(F, c, 5, i, it, ip, ir, da0);
             stage_value(F, c, 6, i, it, ip, ir, da1);
@@ -1286,7 +1301,7 @@ static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream, bool zer
 }
 
 template <typename T>
-static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream);
+static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream, T *ohm = nullptr);
 
 // RHS(call_num) of state st into dv.  With a stage descriptor F (F.stage > 0) the last kernel
 // applies that RK4 stage instead of writing the full tendency; with sld_dif the SLD diffusion
@@ -1357,7 +1372,7 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     }
     // SLD diffusion of the stage-1 state (it needs only B and the state) while the derivative
     // scratch is still free
-    if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream))) return rc;
+    if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream, F ? const_cast<T *>(F->ohm) : nullptr))) return rc;
     // 3. derivative sweeps
     {
         const int mag = m->p.magnetic;
@@ -1468,7 +1483,8 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         const double fb = !F ? (mag ? 8.0 : 5.0) * sizeof(T)
                           : sizeof(T) == 8 ? (FF.stage == 1 ? 24.0 + (FF.dif ? 8.0 : 0.0) : FF.stage < 4 ? 32.0 : 24.0) * m->nv
                                            : (FF.stage == 1 ? 12.0 + (FF.dif ? 4.0 : 0.0) : FF.stage < 4 ? 16.0 : 24.0) * m->nv;
-        CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, reads * N * sizeof(T) + fb * N + (ext ? 32.0 * N : 0.0),
+        const double fo = (F && FF.ohm && FF.stage == 1) ? 3.0 * sizeof(T) : 0.0;   // the folded Ohmic curl
+        CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, reads * N * sizeof(T) + (fb + fo) * N + (ext ? 32.0 * N : 0.0),
                    hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B,
                                       (const T *)(D + D_THRHO * N), (const T *)(D + D_RRHO * N), dv, FF, ext,
                                       (const T *)D, (const T *)L));
@@ -1533,7 +1549,9 @@ static bool sld_split() {
 }
 
 template <typename T>
-static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream) {
+// ohm: with Ohmic heating, a 3-field buffer that receives the curl of the A tendency and the
+// caller applies the heating term (the fused RK4 stage 1); null: k_sld_finish applies it to dif
+static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream, T *ohm) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
     Scratch<T> &S = work<T>(m);
@@ -1582,7 +1600,9 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
     } else {
         two_pass();
     }
-    if (sp.ohmic) {
+    if (sp.ohmic && ohm) {
+        if ((rc = curl_impl<T>(m, dif + 5 * N, ohm, stream, false))) return rc;
+    } else if (sp.ohmic) {
         T *C = S.D + 6 * N;
         if ((rc = curl_impl<T>(m, dif + 5 * N, C, stream, false))) return rc;
         CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (3.0 + 3.0 + 2.0 + 2.0),
@@ -1601,6 +1621,9 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
     const long NV = m->N * m->nv;
     const unsigned nb = nblocks(NV, 256);
     T *dif = m->p.sld_on ? S.dif : nullptr;
+    // SLD Ohmic heating folded into stage 1: the curl of the A tendency goes to S.k, which the
+    // fused stages do not use (they write no tendency field)
+    const bool ohm = dif && make_sld<T>(m).ohmic;
     constexpr bool mixed = sizeof(T) != sizeof(double);
     StageFuse<T> F{};
     F.dt = dt;
@@ -1624,6 +1647,7 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
         for (int stage = 1; stage <= 4; stage++) {
             F.stage = stage;
             F.dif = stage == 1 ? dif : nullptr;
+            F.ohm = (stage == 1 && ohm) ? S.k : nullptr;
             if ((rc = rhs_impl<T>(m, stage == 1 ? in1 : S.y1, S.k, nullptr, stream, &F, stage == 1 ? dif : nullptr)))
                 return rc;
         }
