@@ -59,9 +59,28 @@ struct LdsRows {
     __device__ __forceinline__ const T *tail() const { return sp + 5 * nT; }
 };
 
-// One lane's share of a contiguous compact derivative: rows [st, st+MR) of pencil a, solved with
-// the rest of the pencil's lanes (halos by wave shifts, interfaces by PCR); x = the solution
-// before the job's scale.  lo_val / hi_val: the prescribed wall derivative at a wall end (used by
+// The raw rows [j*MR, j*MR+MR) of lane j of pencil a (contig_lane's input v): issued by the
+// caller ahead of its plan staging, so that the loads are in flight meanwhile.
+template <typename T, int MR, bool FULL>
+__device__ __forceinline__ void contig_load(const DerivJob<T> &J, int n, int nl, long sa, long sb, long srow, long a,
+                                            int b, int j, bool pen_ok, T (&v)[MR]) {
+    const T *__restrict__ in = J.in;
+    const long pb_ld = (pen_ok ? a : 0) * sa + (long)b * sb;
+    const int st = j * MR;
+    if constexpr (FULL) {
+        if (j < nl) load_run<T, MR>(in + pb_ld + st, v);
+    } else {
+#pragma unroll
+        for (int k = 0; k < MR; k++) {
+            const int q = min(st + k, n - 1);
+            v[k] = in[pb_ld + (long)q * srow];
+        }
+    }
+}
+
+// One lane's share of a contiguous compact derivative: rows [st, st+MR) of pencil a (v: their
+// raw values from contig_load), solved with the rest of the pencil's lanes (halos by wave
+// shifts, interfaces by PCR); x = the solution before the job's scale.  lo_val / hi_val: the prescribed wall derivative at a wall end (used by
 // lane 0 / lane nl-1).
 // FULL: unit-stride pencils whose length is a multiple of MR.  Every lane then holds MR rows,
 // lane 0 alone touches the lower end and lane nl-1 alone the upper one, so the wall, halo and
@@ -69,8 +88,8 @@ struct LdsRows {
 // per-row comparisons.
 template <typename T, int ORDER, int MR, bool FULL, typename PlanRows>
 __device__ __forceinline__ void contig_lane(const DerivJob<T> &J, const PlanRows &PR, int n, int nl, long sa, long sb,
-                                            long srow, long a, int b, int j, int lane, bool pen_ok, T lo_val,
-                                            T hi_val, T (&x)[MR]) {
+                                            long srow, long a, int b, int j, int lane, bool pen_ok, T (&v)[MR],
+                                            T lo_val, T hi_val, T (&x)[MR]) {
     constexpr bool vec = FULL;
     constexpr int W = MR + 8;
     const int nI = nl - 1;                    // interface equations
@@ -82,19 +101,8 @@ __device__ __forceinline__ void contig_lane(const DerivJob<T> &J, const PlanRows
     const T *rl = PR.tail(), *ru = rl + kMaxLanes, *k1 = ru + kMaxLanes, *k2 = k1 + kPcrSteps * kMaxLanes,
             *invb = k2 + kPcrSteps * kMaxLanes;
 
-    // ---- own rows, then the halo rows from the neighbouring lanes ----------------------------
+    // ---- own rows (loaded by contig_load), then the halo rows from the neighbouring lanes ------
     const T *__restrict__ in = J.in;
-    const long pb_ld = (pen_ok ? a : 0) * sa + (long)b * sb;
-    T v[MR];
-    if (vec) {
-        if (j < nl) load_run<T, MR>(in + pb_ld + st, v);
-    } else {
-#pragma unroll
-        for (int k = 0; k < MR; k++) {
-            const int q = min(st + k, n - 1);
-            v[k] = in[pb_ld + (long)q * srow];
-        }
-    }
     if (J.pre) {
         // a row profile: the lane's run as 16-byte vectors where the pencil splits into whole
         // runs and the table is 16-byte aligned (lanes past the pencil read the last run)

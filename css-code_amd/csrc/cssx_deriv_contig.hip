@@ -19,22 +19,6 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     T *sp = reinterpret_cast<T *>(smem_raw);
     const DerivJob<T> &J = L.jobs[blockIdx.z];
     const int n = L.n;
-    // The five row tables are staged in 16-byte chunks of E = 16/sizeof(T) rows: rows
-    // jj = h*E..h*E+E-1 of lane j sit at [(h*nl + j)*E], so a lane reads E coefficients with one
-    // 16-byte LDS load and the 16 lanes of a load phase read consecutive chunks (no bank
-    // conflicts); the interface tables follow them unchanged.
-    constexpr int E = 16 / (int)sizeof(T);
-    const int nlT = (n + MR - 1) / MR, nT = nlT * MR;
-    // (one pass per table: MR and E are powers of two, so no division by n)
-#pragma unroll
-    for (int c = 0; c < 5; c++)
-        for (int r = threadIdx.x; r < n; r += blockDim.x) {
-            const int jl = r / MR, jr = r % MR;
-            sp[c * nT + ((jr / E) * nlT + jl) * E + jr % E] = J.plan[c * n + r];
-        }
-    constexpr int tail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
-    for (int i = threadIdx.x; i < tail; i += blockDim.x) sp[5 * nT + i] = J.plan[5 * n + i];
-    __syncthreads();
     // unit-stride pencils whose length is a multiple of MR move as 16-byte vectors
     constexpr bool vec = FULL;
     const int b = blockIdx.y;
@@ -53,9 +37,46 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
     T lo_val = T(0), hi_val = T(0);
     if (active && j == 0 && J.lo_kind == END_WALL && J.glo) lo_val = J.glo[gbase];
     if (active && j == nl - 1 && J.hi_kind == END_WALL && J.ghi) hi_val = J.ghi[gbase];
+    // own rows first: in flight while the plan is staged
+    T v[MR];
+    contig_load<T, MR, FULL>(J, n, nl, L.sa, L.sb, L.srow, a, b, j, pen_ok, v);
+    // The five row tables are staged in 16-byte chunks of E = 16/sizeof(T) rows: rows
+    // jj = h*E..h*E+E-1 of lane j sit at [(h*nl + j)*E], so a lane reads E coefficients with one
+    // 16-byte LDS load and the 16 lanes of a load phase read consecutive chunks (no bank
+    // conflicts); the interface tables follow them unchanged.
+    constexpr int E = 16 / (int)sizeof(T);
+    const int nlT = (n + MR - 1) / MR, nT = nlT * MR;
+    constexpr int tail = 3 * kMaxLanes + 2 * kPcrSteps * kMaxLanes;
+    if (n <= (int)blockDim.x && tail <= (int)blockDim.x) {
+        // one row of every table and one tail value per thread: all six loads issued before
+        // the first wait
+        const int r = threadIdx.x;
+        T pv[5], tv = T(0);
+        if (r < n) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) pv[c] = J.plan[c * n + r];
+        }
+        if (r < tail) tv = J.plan[5 * n + r];
+        if (r < n) {
+            const int jl = r / MR, jr = r % MR;
+#pragma unroll
+            for (int c = 0; c < 5; c++) sp[c * nT + ((jr / E) * nlT + jl) * E + jr % E] = pv[c];
+        }
+        if (r < tail) sp[5 * nT + r] = tv;
+    } else {
+        // (one pass per table: MR and E are powers of two, so no division by n)
+#pragma unroll
+        for (int c = 0; c < 5; c++)
+            for (int r = threadIdx.x; r < n; r += blockDim.x) {
+                const int jl = r / MR, jr = r % MR;
+                sp[c * nT + ((jr / E) * nlT + jl) * E + jr % E] = J.plan[c * n + r];
+            }
+        for (int i = threadIdx.x; i < tail; i += blockDim.x) sp[5 * nT + i] = J.plan[5 * n + i];
+    }
+    __syncthreads();
     T x[MR];
     contig_lane<T, ORDER, MR, FULL>(J, LdsRows<T, MR>{sp, nT, nlT}, n, nl, L.sa, L.sb, L.srow, a, b, j, lane, pen_ok,
-                                    lo_val, hi_val, x);
+                                    v, lo_val, hi_val, x);
 
     // ---- correct and store ------------------------------------------------------------------------
     if (active) {

@@ -49,9 +49,23 @@ void k_deriv_wide(const DerivLaunch<T> L) {
     const int nI = nl - 1;
     const int pn = n + nl;            // padded row count: row r at r + r/M
     const int tid = threadIdx.x;
-    for (int i = tid; i < 5 * n; i += 1024) {
-        const int c = i / n, r = i - c * n;
-        sp[c * pn + r + r / M] = J.plan[i];
+    {
+        // the five row tables (5n <= 5120 values): every thread's loads issued before the first
+        // wait
+        T pv[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const int i = tid + k * 1024;
+            pv[k] = i < 5 * n ? J.plan[i] : T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const int i = tid + k * 1024;
+            if (i < 5 * n) {
+                const int c = i / n, r = i - c * n;
+                sp[c * pn + r + r / M] = pv[k];
+            }
+        }
     }
     // exchange arrays, one padding slot per row so that the lanes of a wavefront, which walk
     // the other index, hit different banks: [segment][pencil] with stride PS, [pencil][segment]
