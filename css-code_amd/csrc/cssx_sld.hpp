@@ -54,9 +54,10 @@ __device__ __forceinline__ T van_albada(T q, T qph, T tiny, T itiny2) {
         const T fast = __builtin_amdgcn_fmed3f(q, qph, 0.0f);
         return fabs(q) >= tiny ? fast : slow;
     } else {
-        const T a = fabs(q);
-        const T r = a >= tiny ? (fabs(qph) < a ? qph : q) : q * (t < T(1) ? t : T(1));   // min(1, t)
-        return s > T(0) ? r : T(0);
+        // the same medians from min and max (FP64 has no three-operand median)
+        const T slow = q * fmin(fmax(t, T(0)), T(1));
+        const T fast = fmax(fmin(q, qph), fmin(fmax(q, qph), T(0)));
+        return fabs(q) >= tiny ? fast : slow;
     }
 }
 
