@@ -96,13 +96,28 @@ void k_deriv_wide(const DerivLaunch<T> L) {
     // ---- 1. window: own rows and 4 on each side (wrapped on a periodic pencil, clamped
     //         otherwise; rows outside a non-periodic pencil are replaced below) ---------------
     T w[W];
+    // FP64, wave-uniform: every window row of the wavefront's G segments lies inside the pencil
+    // (all wavefronts but the first and the last ones), so the rows are a plain strided run with
+    // no per-row wrap or clamp.  (In FP32 this path and the store path below measured slower:
+    // 0.30 against 0.28 ms per 1024-row field, with 25% fewer VALU instructions.)
+    constexpr bool kFast = sizeof(T) == 8;
+    const int ws = (tid >> 6) * G;                  // the wavefront's first segment
+    if (kFast && ws * M >= 4 && (ws + G) * M + 4 <= n) {
+        const unsigned vb = lofs + (unsigned)(st - 4) * rowb;
 #pragma unroll
-    for (int i = 0; i < W; i++) {
-        int q = st - 4 + i;
-        if (periodic) q = q < 0 ? q + n : (q >= n ? q - n : q);
-        else q = q < 0 ? 0 : (q >= n ? n - 1 : q);
-        w[i] = bload<T>(rin, lofs + (unsigned)q * rowb, 0u);
-        if (J.pre) w[i] *= J.pre[q];
+        for (int i = 0; i < W; i++) {
+            w[i] = bload<T>(rin, vb + (unsigned)i * rowb, 0u);
+            if (J.pre) w[i] *= J.pre[st - 4 + i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            int q = st - 4 + i;
+            if (periodic) q = q < 0 ? q + n : (q >= n ? q - n : q);
+            else q = q < 0 ? 0 : (q >= n ? n - 1 : q);
+            w[i] = bload<T>(rin, lofs + (unsigned)q * rowb, 0u);
+            if (J.pre) w[i] *= J.pre[q];
+        }
     }
     // only the end segments' windows leave the pencil
     if (!periodic && (st < 4 || st + M + 4 > n)) {
@@ -245,6 +260,18 @@ void k_deriv_wide(const DerivLaunch<T> L) {
     // with beta != 0 the old values of four rows are loaded ahead of their stores (a store may
     // alias the next row's load, so a load per store would wait a memory round trip each)
     constexpr int C = 4;
+    if (kFast && ws + G < nl && !post && beta == T(0)) {
+        // wave-uniform: every segment of the wavefront is full and ends in an interface row
+        const unsigned vo = lofs + (unsigned)st * rowb;
+#pragma unroll
+        for (int jj = 0; jj < M; jj++) {
+            const T x = jj < M - 1 ? d[jj] + vL[pst + jj] * xl + vR[pst + jj] * xr : xr;
+            T o = scale * x;
+            if (has_pm) o *= pm;
+            bstore(o, rout, vo + (unsigned)jj * rowb, 0u);
+        }
+        return;
+    }
 #pragma unroll
     for (int jc = 0; jc < M; jc += C) {
         T old[C];
