@@ -326,6 +326,48 @@ def run_c3(args):
     print(json.dumps(_clean(out)))
 
 
+def run_deriv(args):
+    """The derivative sweeps alone: one seeded field of the --nr/--nth/--nphi shape, the compact
+    first and second derivative along each axis (derivatives.f90 Dbydr/Dbydt/Dbydp and
+    D2byd*2, boundary code 1 on the walls, phi periodic), each timed over --steps launches with
+    HIP events after --warmup, in each precision of --paths.  One JSON line: microseconds per
+    sweep and the algorithmic rate (one read and one write of the field)."""
+    import torch
+    from cssx import operators as O
+
+    nr, nth, nphi = args.nr, args.nth, args.nphi
+    ops = O.Operators(nr, nth, nphi, nr - 1.0, nth - 1.0, float(nphi))
+    res = {}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    for prec in args.paths.split(","):
+        dt = torch.float32 if prec == "f32" else torch.float64
+        f = torch.randn((nr, nphi, nth), dtype=dt, device="cuda", generator=g)
+        out = torch.empty_like(f)
+        for ax, name in ((0, "r"), (1, "theta"), (2, "phi")):
+            for order in (1, 2):
+                for _ in range(args.warmup):
+                    ops.deriv(f, ax, order, 1, out=out)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record()
+                for _ in range(args.steps):
+                    ops.deriv(f, ax, order, 1, out=out)
+                e1.record()
+                torch.cuda.synchronize()
+                us = e0.elapsed_time(e1) * 1e3 / args.steps
+                tbs = 2.0 * f.numel() * f.element_size() / (us * 1e-6) / 1e12
+                res[f"{prec}_{name}_d{order}"] = {"us": round(us, 1), "TB_s": round(tbs, 2)}
+                print(f"bench: deriv {nr}x{nth}x{nphi} {prec} {name} order {order}: {us:.1f} us, {tbs:.2f} TB/s",
+                      file=sys.stderr, flush=True)
+        del f, out
+    ops.close()
+    print(json.dumps({"metric": "compact FD6 derivative sweep time", "unit": "us", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+                      "data": "synthetic", "config": {"workload": f"deriv_{nr}x{nth}x{nphi}", "nr": nr,
+                                                      "nth": nth, "nphi": nphi}, "sweeps": res}))
+
+
 def measured_traffic(workload):
     """The rocprofv3 FETCH_SIZE/WRITE_SIZE summary committed for this workload (FP32-bulk path):
     the JSON block of profiles/hbm_traffic_<workload>.md, or None."""
@@ -376,8 +418,9 @@ def main():
     ap.add_argument("--c3-chunk", type=int, default=64)
     ap.add_argument("--c3-members", type=int, default=64)
     ap.add_argument("--c3-ens-steps", type=int, default=5)
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c4",
-                    help="c4: one 256x512x1024 subdomain with the precision study (the headline); c2: 128x256x512")
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5", "deriv"], default="c4",
+                    help="c4: one 256x512x1024 subdomain with the precision study (the headline); c2: 128x256x512; "
+                         "deriv: the derivative sweeps alone (--nr/--nth/--nphi, default 256x512x1024)")
     ap.add_argument("--no-c2", action="store_true", help="c4: leave out the 128x256x512 measurement")
     ap.add_argument("--nr", type=int, default=0)
     ap.add_argument("--nth", type=int, default=0)
@@ -394,7 +437,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
-    shape = {"c2": (128, 256, 512), "c3": (0, 0, 0), "c4": (256, 512, 1024), "c5": (256, 512, 1024)}[args.config]
+    shape = {"c2": (128, 256, 512), "c3": (0, 0, 0), "c4": (256, 512, 1024), "c5": (256, 512, 1024),
+             "deriv": (256, 512, 1024)}[args.config]
     args.nr, args.nth, args.nphi = (args.nr or shape[0], args.nth or shape[1], args.nphi or shape[2])
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -413,6 +457,9 @@ def main():
         return
     if args.config == "c3":
         run_c3(args)
+        return
+    if args.config == "deriv":
+        run_deriv(args)
         return
     if args.copy_calibration:
         n = args.copy_calibration * (1 << 20) // 4
