@@ -37,6 +37,7 @@ struct DerivSpec {
 
 // per-phase event timing (cssx_timing.hip); token < 0 = disabled
 int timing_begin(int phase, hipStream_t stream);
+bool timing_on();
 void timing_end(int token, hipStream_t stream, double bytes);
 
 // learned closure tendency of a whole state (cssx_closure.hip)

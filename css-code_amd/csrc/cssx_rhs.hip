@@ -122,6 +122,7 @@ struct Scratch {
     T *y1 = nullptr;     // RK stage state (nv fields)
     T *k = nullptr;      // RK stage tendency (nv fields)
     T *dif = nullptr;    // SLD diffusion tendency of stage 1 (nv fields, SLDiffusion runs)
+    T *vcc = nullptr;    // SLD speed^2 per cell (1 field, SLDiffusion runs)
     T *acc = nullptr;    // FP32 runs: RK4 tendency accumulator (nv fields)
     T *y0f = nullptr;    // FP32 runs: the FP64 state y0 in stage precision, fluctuation form
     T *L = nullptr;      // explicit diffusion: L_NSLOT fields
@@ -159,6 +160,11 @@ struct cssx_mhd {
     // matrix-core work overlaps the memory-bound derivative sweeps
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // RK4 stage 1's SLD diffusion runs on a stream of its own (non-blocking, default priority),
+    // forked after Compute_B and joined before the final kernel: its VALU-bound column march
+    // overlaps the memory-bound derivative sweeps
+    hipStream_t sld_side = nullptr;
+    hipEvent_t ev_sld_fork = nullptr, ev_sld_join = nullptr;
 };
 
 namespace cssx {
@@ -1192,6 +1198,7 @@ static int ensure_scratch(cssx_mhd *m) {
     if (hipMalloc(&S.y1, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (hipMalloc(&S.k, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (m->p.sld_on && hipMalloc(&S.dif, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
+    if (m->p.sld_on && hipMalloc(&S.vcc, sizeof(T) * N) != hipSuccess) return CSSX_ENOMEM;
     if (sizeof(T) == 4 && (hipMalloc(&S.acc, sizeof(T) * N * m->nv) != hipSuccess ||
                            hipMalloc(&S.y0f, sizeof(T) * N * m->nv) != hipSuccess))
         return CSSX_ENOMEM;
@@ -1206,6 +1213,31 @@ static int ensure_scratch(cssx_mhd *m) {
 }
 
 static inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Whether the SLD diffusion of an RHS may run on its own stream: not while the caller's stream
+// is captured into a graph (the fork is kept out of captured graphs), not while phase timing is
+// on (its phases are timed one after another), and not with CSSX_SLD_FORK=0 (read on every
+// call); creates the stream and its events on first use.
+static bool sld_fork_ok(cssx_mhd *m, hipStream_t stream) {
+    const char *e = std::getenv("CSSX_SLD_FORK");
+    if (e && std::atoi(e) == 0) return false;
+    if (timing_on()) return false;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return false;
+    if (!m->sld_side && hipStreamCreateWithFlags(&m->sld_side, hipStreamNonBlocking) != hipSuccess) {
+        m->sld_side = nullptr;
+        return false;
+    }
+    if (!m->ev_sld_fork && hipEventCreateWithFlags(&m->ev_sld_fork, hipEventDisableTiming) != hipSuccess) {
+        m->ev_sld_fork = nullptr;
+        return false;
+    }
+    if (!m->ev_sld_join && hipEventCreateWithFlags(&m->ev_sld_join, hipEventDisableTiming) != hipSuccess) {
+        m->ev_sld_join = nullptr;
+        return false;
+    }
+    return true;
+}
 
 // one batched derivative launch along an axis
 template <typename T>
@@ -1370,9 +1402,33 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            m->p.bottom_stress_free, m->p.top_stress_free, m->p.bottom_const_flux,
                            m->p.top_const_flux, m->p.theta_normal_field);
     }
-    // SLD diffusion of the stage-1 state (it needs only B and the state) while the derivative
-    // scratch is still free
-    if (sld_dif && (rc = sld_impl<T>(m, st, B, sld_dif, stream, F ? const_cast<T *>(F->ohm) : nullptr))) return rc;
+    // SLD diffusion of the stage-1 state: it needs only B and the state, and writes the SLD
+    // tendency, its speed buffer and (RK4) the Ohmic curl, which nothing before the final kernel
+    // reads; off a captured stream and with phase timing off it runs on its own stream
+    bool fork_sld = false;
+    if (sld_dif) {
+        fork_sld = sld_fork_ok(m, stream);
+        if (fork_sld) {
+            if (hipEventRecord(m->ev_sld_fork, stream) != hipSuccess ||
+                hipStreamWaitEvent(m->sld_side, m->ev_sld_fork, 0) != hipSuccess)
+                return CSSX_EHIP;
+            rc = sld_impl<T>(m, st, B, sld_dif, m->sld_side, F ? const_cast<T *>(F->ohm) : nullptr);
+            // recorded whatever sld_impl returned: every exit below joins it
+            const hipError_t e = hipEventRecord(m->ev_sld_join, m->sld_side);
+            if (rc || e != hipSuccess) {
+                if (e == hipSuccess) (void)hipStreamWaitEvent(stream, m->ev_sld_join, 0);
+                else (void)hipStreamSynchronize(m->sld_side);
+                return rc ? rc : CSSX_EHIP;
+            }
+        } else if ((rc = sld_impl<T>(m, st, B, sld_dif, stream, F ? const_cast<T *>(F->ohm) : nullptr))) {
+            return rc;
+        }
+    }
+    // from here every error return first joins the SLD stream
+    auto fail = [&](int code) {
+        if (fork_sld) (void)hipStreamWaitEvent(stream, m->ev_sld_join, 0);
+        return code;
+    };
     // 3. derivative sweeps
     {
         const int mag = m->p.magnetic;
@@ -1382,7 +1438,7 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            spec<T>(m, CSSX_AXIS_THETA, s, D + D_TS * N, ibct[5], nullptr, nullptr),
                            spec<T>(m, CSSX_AXIS_THETA, bp, D + D_TBP * N, ibct[8], wall + W.off_tbp_lo, wall + W.off_tbp_hi),
                            spec<T>(m, CSSX_AXIS_THETA, br, D + D_TBR * N, ibct[9], nullptr, nullptr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, mag ? 6 : 4, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, mag ? 6 : 4, stream))) return fail(rc);
         DerivSpec ph[7] = {spec<T>(m, CSSX_AXIS_PHI, rho, D + D_PRHO * N, 0, nullptr, nullptr),
                            spec<T>(m, CSSX_AXIS_PHI, v, D + D_PV * N, 0, nullptr, nullptr),
                            spec<T>(m, CSSX_AXIS_PHI, w, D + D_PW * N, 0, nullptr, nullptr),
@@ -1390,14 +1446,14 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            spec<T>(m, CSSX_AXIS_PHI, s, D + D_PS * N, 0, nullptr, nullptr),
                            spec<T>(m, CSSX_AXIS_PHI, bt, D + D_PBT * N, 0, nullptr, nullptr),
                            spec<T>(m, CSSX_AXIS_PHI, br, D + D_PBR * N, 0, nullptr, nullptr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, mag ? 7 : 5, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, mag ? 7 : 5, stream))) return fail(rc);
         DerivSpec rr[6] = {spec<T>(m, CSSX_AXIS_R, w, D + D_RW * N, ibcr[2], wall + W.off_rw_lo, wall + W.off_rw_hi, nullptr, P.dxdr),
                            spec<T>(m, CSSX_AXIS_R, v, D + D_RV * N, ibcr[3], wall + W.off_rv_lo, wall + W.off_rv_hi, nullptr, P.dxdr),
                            spec<T>(m, CSSX_AXIS_R, u, D + D_RU * N, ibcr[4], nullptr, nullptr, nullptr, P.dxdr),
                            spec<T>(m, CSSX_AXIS_R, s, D + D_RS * N, ibcr[5], wall + W.off_rs_lo, wall + W.off_rs_hi, nullptr, P.dxdr),
                            spec<T>(m, CSSX_AXIS_R, bt, D + D_RBT * N, ibcr[7], nullptr, nullptr, nullptr, P.dxdr),
                            spec<T>(m, CSSX_AXIS_R, bp, D + D_RBP * N, ibcr[8], nullptr, nullptr, nullptr, P.dxdr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, mag ? 6 : 4, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, mag ? 6 : 4, stream))) return fail(rc);
     }
     // 3b. explicit diffusion: second derivatives (same wall values as the first), the cross-term
     //     sweeps of first-derivative slots, and curl B
@@ -1408,17 +1464,17 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            spec2<T>(m, CSSX_AXIS_THETA, w, L + L_TW2 * N, ibct[2], nullptr, nullptr),
                            spec2<T>(m, CSSX_AXIS_THETA, v, L + L_TV2 * N, ibct[3], wall + W.off_tv_lo, wall + W.off_tv_hi),
                            spec2<T>(m, CSSX_AXIS_THETA, u, L + L_TU2 * N, ibct[4], nullptr, nullptr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, nd, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, nd, stream))) return fail(rc);
         DerivSpec ph[4] = {spec2<T>(m, CSSX_AXIS_PHI, s, L + L_PS2 * N, 0, nullptr, nullptr),
                            spec2<T>(m, CSSX_AXIS_PHI, v, L + L_PV2 * N, 0, nullptr, nullptr),
                            spec2<T>(m, CSSX_AXIS_PHI, w, L + L_PW2 * N, 0, nullptr, nullptr),
                            spec2<T>(m, CSSX_AXIS_PHI, u, L + L_PU2 * N, 0, nullptr, nullptr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, nd, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, ph, nd, stream))) return fail(rc);
         DerivSpec rr[4] = {spec2<T>(m, CSSX_AXIS_R, s, L + L_RS2 * N, ibcr[5], wall + W.off_rs_lo, wall + W.off_rs_hi),
                            spec2<T>(m, CSSX_AXIS_R, w, L + L_RW2 * N, ibcr[2], wall + W.off_rw_lo, wall + W.off_rw_hi),
                            spec2<T>(m, CSSX_AXIS_R, v, L + L_RV2 * N, ibcr[3], wall + W.off_rv_lo, wall + W.off_rv_hi),
                            spec2<T>(m, CSSX_AXIS_R, u, L + L_RU2 * N, ibcr[4], nullptr, nullptr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, nd, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, rr, nd, stream))) return fail(rc);
     }
     if (c.lap) {
         const long nrr = (long)m->nth * m->nphi, nt = (long)m->nr * m->nphi;
@@ -1429,23 +1485,23 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            spec<T>(m, CSSX_AXIS_R, D + D_PU * N, L + L_XR2 * N, ibcr[4], nullptr, nullptr, nullptr, P.dxdr),
                            spec<T>(m, CSSX_AXIS_R, D + D_TU * N, L + L_XR3 * N, ibcr[4], nullptr, nullptr, nullptr, P.dxdr),
                            spec<T>(m, CSSX_AXIS_R, D + D_PV * N, L + L_XR4 * N, ibcr[3], nullptr, lw + nrr, nullptr, P.dxdr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, xr, 4, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, xr, 4, stream))) return fail(rc);
         DerivSpec xt[2] = {spec<T>(m, CSSX_AXIS_THETA, D + D_PV * N, L + L_XT1 * N, ibct[3], lw + 2 * nrr, lw + 2 * nrr + nt),
                            spec<T>(m, CSSX_AXIS_THETA, D + D_PW * N, L + L_XT2 * N, ibct[2], nullptr, nullptr)};
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, xt, 2, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, xt, 2, stream))) return fail(rc);
         if (m->p.magnetic) {
             // VectorCurl: theta sweeps with code 1 (Theta_Normal_Field) or 0, whose wall values go
             // unused; r sweeps with zero wall values (the radial-field walls zero those halos)
             const int tcode = m->p.theta_normal_field ? 1 : 0;
             DerivSpec jt[2] = {spec<T>(m, CSSX_AXIS_THETA, bp, L + L_JT1 * N, tcode, nullptr, nullptr, P.sn),
                                spec<T>(m, CSSX_AXIS_THETA, br, L + L_JT2 * N, tcode, nullptr, nullptr)};
-            if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, jt, 2, stream))) return rc;
+            if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, jt, 2, stream))) return fail(rc);
             DerivSpec jp[2] = {spec<T>(m, CSSX_AXIS_PHI, br, L + L_JP1 * N, 0, nullptr, nullptr),
                                spec<T>(m, CSSX_AXIS_PHI, bt, L + L_JP2 * N, 0, nullptr, nullptr)};
-            if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, jp, 2, stream))) return rc;
+            if ((rc = derivs_axis<T>(m, CSSX_AXIS_PHI, jp, 2, stream))) return fail(rc);
             DerivSpec jr[2] = {spec<T>(m, CSSX_AXIS_R, bt, L + L_JRT * N, ibcr[7], nullptr, nullptr, P.r, P.dxdr),
                                spec<T>(m, CSSX_AXIS_R, bp, L + L_JRP * N, ibcr[8], nullptr, nullptr, P.r, P.dxdr)};
-            if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, jr, 2, stream))) return rc;
+            if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, jr, 2, stream))) return fail(rc);
         }
     }
     // 4-5. density wall derivatives from the momentum tendency, which k_walls_density evaluates
@@ -1459,20 +1515,20 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     {
         DerivSpec th = spec<T>(m, CSSX_AXIS_THETA, rho, D + D_THRHO * N, ibct[1], wall + W.off_trho_lo,
                                wall + W.off_trho_hi);
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, &th, 1, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, &th, 1, stream))) return fail(rc);
         DerivSpec rr = spec<T>(m, CSSX_AXIS_R, rho, D + D_RRHO * N, ibcr[1], wall + W.off_rrho_lo,
                                wall + W.off_rrho_hi, nullptr, P.dxdr);
-        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, &rr, 1, stream))) return rc;
+        if ((rc = derivs_axis<T>(m, CSSX_AXIS_R, &rr, 1, stream))) return fail(rc);
         StageFuse<T> F0{};
         const StageFuse<T> &FF = F ? *F : F0;
         const float *ext = nullptr;
         if (fork_closure) {   // join the closure's side stream
-            if (hipStreamWaitEvent(stream, m->ev_join, 0) != hipSuccess) return CSSX_EHIP;
+            if (hipStreamWaitEvent(stream, m->ev_join, 0) != hipSuccess) return fail(CSSX_EHIP);
             ext = m->ext;
         } else if (m->closure) {
             rc = closure_tendency(m->closure, sizeof(T) == 8 ? CSSX_PREC_F64 : CSSX_PREC_F32, st, c.fluct, P.rbar,
                                   P.sbar, m->nr, m->nphi, m->nth, m->ext, stream);
-            if (rc) return rc;
+            if (rc) return fail(rc);
             ext = m->ext;
         }
         // reads: 5 state, 3 B, 19 (13) first-derivative and 2 density-derivative fields, 18 (3)
@@ -1484,6 +1540,10 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                           : sizeof(T) == 8 ? (FF.stage == 1 ? 24.0 + (FF.dif ? 8.0 : 0.0) : FF.stage < 4 ? 32.0 : 24.0) * m->nv
                                            : (FF.stage == 1 ? 12.0 + (FF.dif ? 4.0 : 0.0) : FF.stage < 4 ? 16.0 : 24.0) * m->nv;
         const double fo = (F && FF.ohm && FF.stage == 1) ? 3.0 * sizeof(T) : 0.0;   // the folded Ohmic curl
+        if (fork_sld && hipStreamWaitEvent(stream, m->ev_sld_join, 0) != hipSuccess) {
+            (void)hipStreamSynchronize(m->sld_side);
+            return CSSX_EHIP;
+        }
         CSSX_TIMED(CSSX_PHASE_RHS_FINAL, stream, reads * N * sizeof(T) + (fb + fo) * N + (ext ? 32.0 * N : 0.0),
                    hipLaunchKernelGGL(k_rhs_final<T>, dim3(nb), dim3(256), 0, stream, c, P, st, B,
                                       (const T *)(D + D_THRHO * N), (const T *)(D + D_RRHO * N), dv, FF, ext,
@@ -1559,7 +1619,7 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
     const Consts<T> c = make_consts<T>(m);
     const SldParams<T> sp = make_sld<T>(m);
     const long N = m->N;
-    T *vcc = S.D + 18 * N;
+    T *vcc = S.vcc;
     const unsigned nb = nblocks(N, 256);
     const int nv = m->nv;
     const double ts = (double)sizeof(T);
@@ -1804,14 +1864,22 @@ extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
         (void)hipStreamDestroy(m->side);
         m->side = nullptr;
     }
+    if (m->sld_side) {
+        (void)hipStreamSynchronize(m->sld_side);
+        (void)hipStreamDestroy(m->sld_side);
+        m->sld_side = nullptr;
+    }
     for (void *p : {(void *)m->p64.buf, (void *)m->p32.buf, (void *)m->s64.D, (void *)m->s64.B, (void *)m->s64.wall,
                     (void *)m->s64.y1, (void *)m->s64.k, (void *)m->s32.D, (void *)m->s32.B, (void *)m->s32.wall,
                     (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red, (void *)m->s64.dif,
                     (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f, (void *)m->ext, (void *)m->s64.L,
-                    (void *)m->s64.lwall, (void *)m->s32.L, (void *)m->s32.lwall})
+                    (void *)m->s64.lwall, (void *)m->s32.L, (void *)m->s32.lwall, (void *)m->s64.vcc,
+                    (void *)m->s32.vcc})
         if (p) (void)hipFree(p);
     if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
     if (m->ev_join) (void)hipEventDestroy(m->ev_join);
+    if (m->ev_sld_fork) (void)hipEventDestroy(m->ev_sld_fork);
+    if (m->ev_sld_join) (void)hipEventDestroy(m->ev_sld_join);
     delete m;
 }
 

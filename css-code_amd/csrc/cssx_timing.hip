@@ -38,6 +38,11 @@ hipEvent_t take_event() {
 }
 }  // namespace
 
+bool timing_on() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    return g_on;
+}
+
 int timing_begin(int phase, hipStream_t stream) {
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_on) return -1;
