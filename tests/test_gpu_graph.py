@@ -3,7 +3,8 @@
 * Solver.capture records whole Step_RK4 steps (step_rk4.f90:339-493: four RHS evaluations,
   the SLD diffusion, the fused stage updates and Bound) in one graph.  Each replay must equal
   eager stepping of a copy bit for bit, replay after replay, with eager work and
-  synchronizations in between.
+  synchronizations in between.  The graph runs stage 1's SLD on the captured stream and the
+  eager steps on the solver's SLD stream, so this also holds the forked SLD to the same bits.
 * With a closure attached, RK4 must equal the stages composed by hand from ``rhs`` (which
   then includes the closure tendency), ``diffusion`` and ``bound`` with the RK4 weights:
   FP64 to rounding, and the FP32-bulk path (FP32 stages and accumulator, FP64 state) against
