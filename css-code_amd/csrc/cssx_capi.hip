@@ -162,6 +162,7 @@ static int add_job(DerivLaunch<T> &L, const DerivSpec *specs, int nspec, int i, 
     J.plan = (const T *)plan;
     J.pre = (const T *)s.pre; J.post = (const T *)s.post;
     J.mul_a = (const T *)s.mul_a; J.mul_b = (const T *)s.mul_b;
+    J.tap_in = (T *)s.tap_in; J.tap_out = (T *)s.tap_out; J.tap_side = s.tap_side;
     J.scale = (T)s.scale; J.neu_mul = (T)s.neu_mul; J.halo_s = (T)s.halo_s;
     J.rhs_scale = (T)s.rhs_scale; J.beta = (T)s.beta;
     J.bdry = s.bdry; J.lo_kind = s.lo_kind; J.hi_kind = s.hi_kind; J.lo_r0_val = s.lo_r0_val;
@@ -177,6 +178,8 @@ static int deriv_impl(const DerivSpec &sp, const AxisGeom &g, int nspec, hipStre
     launch_family(&sp, nspec, g, f64, variant, seg);
     const bool wide = variant == 2, contig = variant == 1;
     if (contig) { L.M = seg; L.S = lanes_per_pencil(g.n, L.M); }
+    for (int i = 0; i < nspec && !contig; i++)   // only the contiguous kernel writes end-row taps
+        if ((&sp)[i].tap_in || (&sp)[i].tap_out) return CSSX_EINVAL;
     if (wide) { L.M = seg; L.S = (g.n + seg - 1) / seg; }
     double passes = 0.0;
     for (int i = 0; i < nspec; i++) {

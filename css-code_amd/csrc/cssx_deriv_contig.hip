@@ -74,6 +74,21 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
         for (int i = threadIdx.x; i < tail; i += blockDim.x) sp[5 * nT + i] = J.plan[5 * n + i];
     }
     __syncthreads();
+    // end-row taps of the raw input (before contig_lane applies a row profile to v); the lane's
+    // last row is m - 1
+    const long tap_at = (long)b * L.na + a;
+    if (J.tap_in && active) {
+        if (j == 0) J.tap_in[tap_at] = v[0];
+        if (j == nl - 1) {
+            T e = v[MR - 1];
+            if constexpr (!FULL) {
+#pragma unroll
+                for (int k = 0; k < MR - 1; k++)
+                    if (k == m - 1) e = v[k];
+            }
+            J.tap_in[J.tap_side + tap_at] = e;
+        }
+    }
     T x[MR];
     contig_lane<T, ORDER, MR, FULL>(J, LdsRows<T, MR>{sp, nT, nlT}, n, nl, L.sa, L.sb, L.srow, a, b, j, lane, pen_ok,
                                     v, lo_val, hi_val, x);
@@ -104,6 +119,18 @@ __global__ __launch_bounds__(1024) void k_deriv_contig(const DerivLaunch<T> L) {
 #pragma unroll
         for (int jj = 0; jj < MR; jj++)
             if (jj < m) o[jj] += beta * out[pbase + (long)(st + jj) * L.srow];
+    }
+    if (J.tap_out) {
+        if (j == 0) J.tap_out[tap_at] = o[0];
+        if (j == nl - 1) {
+            T e = o[MR - 1];
+            if constexpr (!FULL) {
+#pragma unroll
+                for (int k = 0; k < MR - 1; k++)
+                    if (k == m - 1) e = o[k];
+            }
+            J.tap_out[J.tap_side + tap_at] = e;
+        }
     }
     if (vec) {
         store_run<T, MR>(out + pbase + st, o);

@@ -28,6 +28,8 @@ struct DerivSpec {
     long g_k = -1, g_a = 0, g_b = 0;      // -1: use the axis default ghost layout
     const void *pre = nullptr, *post = nullptr;
     const void *mul_a = nullptr, *mul_b = nullptr;   // per-pencil multipliers (fast / slow index)
+    void *tap_in = nullptr, *tap_out = nullptr;      // contiguous axis only: end-row copies
+    long tap_side = 0;
     double scale = 1.0, neu_mul = 1.0, halo_s = 1.0, rhs_scale = 1.0, beta = 0.0;
     int bdry = 0;
     int lo_kind = END_WALL, hi_kind = END_WALL;

@@ -21,6 +21,9 @@ struct DerivJob {
     const T *post;      // optional per-row multiplier applied to f' (e.g. dx/dr)
     const T *mul_a;     // optional multiplier of f' by the pencil's fast index a
     const T *mul_b;     // optional multiplier of f' by the pencil's slow index b
+    T *tap_in, *tap_out;  // contiguous kernel only, optional: the input and f' of the pencil's end
+                          // rows copied to [side][b][a] (side 0: row 0, 1: row n-1; stride tap_side)
+    long tap_side;
     T scale;            // 1/dx (order 1) or 1/dx^2 (order 2)
     T neu_mul;          // converts a prescribed wall derivative to index units
     T halo_s;           // multiplier of the stencil part of the halo rows (1 = Fortran)

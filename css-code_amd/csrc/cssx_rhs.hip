@@ -123,6 +123,10 @@ struct Scratch {
     T *k = nullptr;      // RK stage tendency (nv fields)
     T *dif = nullptr;    // SLD diffusion tendency of stage 1 (nv fields, SLDiffusion runs)
     T *vcc = nullptr;    // SLD speed^2 per cell (1 field, SLDiffusion runs)
+    // the theta-wall block: the state (5), B (3) and every derivative slot (D_NSLOT) at the
+    // theta-wall points, [field][side][ir][ip] (side 0: theta row 0, 1: row nth-1), so that the
+    // density wall closure reads them contiguously
+    T *tw = nullptr;
     T *acc = nullptr;    // FP32 runs: RK4 tendency accumulator (nv fields)
     T *y0f = nullptr;    // FP32 runs: the FP64 state y0 in stage precision, fluctuation form
     T *L = nullptr;      // explicit diffusion: L_NSLOT fields
@@ -624,18 +628,58 @@ __device__ __forceinline__ void pre_density_terms(const Consts<T> &c, const Prof
 }
 
 // ---------------------------------------------------------------------------------------------
+// 5a. the theta-wall block's fields that no theta sweep reads or writes: rho, B_theta and the
+//     phi / r derivative slots of the momentum tendency's theta component (D_PW, D_PBT, D_RW,
+//     D_RBT), copied from the theta-wall points (one thread per (ir, ip), both walls)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_walls_gather(Consts<T> c, const T *__restrict__ st, const T *__restrict__ B,
+                                                      const T *__restrict__ D, T *__restrict__ tw) {
+    const long nt = (long)c.nr * c.nphi, N = c.N, M = 2 * nt;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nt) return;
+    const long q0 = i * c.nth;   // (ir, ip) row of theta points, theta index 0
+    for (int side = 0; side < 2; side++) {
+        const long q = q0 + (side ? c.nth - 1 : 0), w = side * nt + i;
+        tw[w] = st[q];
+        tw[(8 + D_PW) * M + w] = D[D_PW * N + q];
+        tw[(8 + D_RW) * M + w] = D[D_RW * N + q];
+        if (c.magnetic) {
+            tw[5 * M + w] = B[q];
+            tw[(8 + D_PBT) * M + w] = D[D_PBT * N + q];
+            tw[(8 + D_RBT) * M + w] = D[D_RBT * N + q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // 5. density wall derivatives from the momentum tendency (rhs_mpi.f90:1553-1564, :1750-1773)
 // ---------------------------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_walls_density(Consts<T> c, Prof<T> P, WallPtrs W, T *__restrict__ wall,
                                                        const T *__restrict__ st, const T *__restrict__ B,
                                                        const T *__restrict__ D, const T *__restrict__ L,
-                                                       int lbcr_rho) {
+                                                       int lbcr_rho, const T *__restrict__ tw) {
     const long nt = (long)c.nr * c.nphi, nrr = (long)c.nth * c.nphi, N = c.N;
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     // the momentum tendency before the density terms, evaluated at the wall points themselves
     T o0[5], o1[5];
-    if (i < nt) {
+    if (tw && i < nt) {
+        // the theta-wall block (k_walls_gather and the theta sweeps' end-row taps): the same
+        // values, one field after another with stride 2*nt
+        const int ir = (int)(i / c.nphi);
+        const T rv = P.r[ir] * c.invpconst;
+        Consts<T> cw = c;
+        cw.N = 2 * nt;
+        const T *stw = tw, *Bw = tw + 5 * cw.N, *Dw = tw + 8 * cw.N;
+        const long q0 = i, q1 = nt + i;
+        const Thermo<T> a0 = thermo(stw[q0], stw[4 * cw.N + q0], ir, cw, P);
+        const Thermo<T> a1 = thermo(stw[q1], stw[4 * cw.N + q1], ir, cw, P);
+        pre_density_terms(cw, P, stw, Bw, Dw, theta_from(Dw, cw.N, q0), L, q0, 0, ir, o0);
+        pre_density_terms(cw, P, stw, Bw, Dw, theta_from(Dw, cw.N, q1), L, q1, c.nth - 1, ir, o1);
+        wall[W.off_trho_lo + i] = rv * o0[1] * a0.rho / a0.t;
+        wall[W.off_trho_hi + i] = rv * o1[1] * a1.rho / a1.t;
+    } else if (i < nt) {
         const int ir = (int)(i / c.nphi), ip = (int)(i % c.nphi);
         const T rv = P.r[ir] * c.invpconst;
         const long q0 = ((long)ir * c.nphi + ip) * c.nth, q1 = q0 + c.nth - 1;
@@ -1199,6 +1243,11 @@ static int ensure_scratch(cssx_mhd *m) {
     if (hipMalloc(&S.k, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (m->p.sld_on && hipMalloc(&S.dif, sizeof(T) * N * m->nv) != hipSuccess) return CSSX_ENOMEM;
     if (m->p.sld_on && hipMalloc(&S.vcc, sizeof(T) * N) != hipSuccess) return CSSX_ENOMEM;
+    {
+        const size_t tw = (size_t)(8 + D_NSLOT) * 2 * m->nr * m->nphi;
+        if (hipMalloc(&S.tw, sizeof(T) * tw) != hipSuccess) return CSSX_ENOMEM;
+        if (hipMemset(S.tw, 0, sizeof(T) * tw) != hipSuccess) return CSSX_EHIP;   // slots never written stay 0
+    }
     if (sizeof(T) == 4 && (hipMalloc(&S.acc, sizeof(T) * N * m->nv) != hipSuccess ||
                            hipMalloc(&S.y0f, sizeof(T) * N * m->nv) != hipSuccess))
         return CSSX_ENOMEM;
@@ -1429,6 +1478,10 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
         if (fork_sld) (void)hipStreamWaitEvent(stream, m->ev_sld_join, 0);
         return code;
     };
+    // the theta-wall block for the density wall closure: without explicit diffusion (whose
+    // viscous terms the closure would read too)
+    T *tw = (!c.slap && !c.lap) ? S.tw : nullptr;
+    const long twM = 2L * m->nr * m->nphi;
     // 3. derivative sweeps
     {
         const int mag = m->p.magnetic;
@@ -1438,6 +1491,17 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                            spec<T>(m, CSSX_AXIS_THETA, s, D + D_TS * N, ibct[5], nullptr, nullptr),
                            spec<T>(m, CSSX_AXIS_THETA, bp, D + D_TBP * N, ibct[8], wall + W.off_tbp_lo, wall + W.off_tbp_hi),
                            spec<T>(m, CSSX_AXIS_THETA, br, D + D_TBR * N, ibct[9], nullptr, nullptr)};
+        if (tw) {
+            // the theta sweeps copy their inputs and results at the theta-wall rows into the
+            // theta-wall block (slots: w v u s -> 1..4, bp br -> B 1..2, results -> D slots)
+            const int in_slot[6] = {1, 2, 3, 4, 6, 7};
+            const int out_slot[6] = {D_TW, D_TV, D_TU, D_TS, D_TBP, D_TBR};
+            for (int k = 0; k < 6; k++) {
+                th[k].tap_in = tw + in_slot[k] * twM;
+                th[k].tap_out = tw + (8 + out_slot[k]) * twM;
+                th[k].tap_side = twM / 2;
+            }
+        }
         if ((rc = derivs_axis<T>(m, CSSX_AXIS_THETA, th, mag ? 6 : 4, stream))) return fail(rc);
         DerivSpec ph[7] = {spec<T>(m, CSSX_AXIS_PHI, rho, D + D_PRHO * N, 0, nullptr, nullptr),
                            spec<T>(m, CSSX_AXIS_PHI, v, D + D_PV * N, 0, nullptr, nullptr),
@@ -1508,8 +1572,11 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     //      (every term before the density terms) at the wall points themselves
     {
         long nw = std::max((long)m->nr * m->nphi, (long)m->nth * m->nphi);
+        if (tw)
+            hipLaunchKernelGGL(k_walls_gather<T>, dim3(nblocks((long)m->nr * m->nphi, 256)), dim3(256), 0, stream, c,
+                               st, (const T *)B, (const T *)D, tw);
         hipLaunchKernelGGL(k_walls_density<T>, dim3(nblocks(nw, 256)), dim3(256), 0, stream, c, P, W, wall, st,
-                           (const T *)B, (const T *)D, (const T *)L, ibcr[1]);
+                           (const T *)B, (const T *)D, (const T *)L, ibcr[1], (const T *)tw);
     }
     // 6. density sweeps + every pointwise term, the density terms and the stage in one kernel
     {
@@ -1874,7 +1941,7 @@ extern "C" void cssx_mhd_destroy(cssx_mhd *m) {
                     (void *)m->s32.y1, (void *)m->s32.k, (void *)m->save, (void *)m->red, (void *)m->s64.dif,
                     (void *)m->s32.dif, (void *)m->s32.acc, (void *)m->s32.y0f, (void *)m->ext, (void *)m->s64.L,
                     (void *)m->s64.lwall, (void *)m->s32.L, (void *)m->s32.lwall, (void *)m->s64.vcc,
-                    (void *)m->s32.vcc})
+                    (void *)m->s32.vcc, (void *)m->s64.tw, (void *)m->s32.tw})
         if (p) (void)hipFree(p);
     if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
     if (m->ev_join) (void)hipEventDestroy(m->ev_join);
