@@ -30,9 +30,13 @@ def _rel_fields(a, b):
 
 
 SHAPES = [(24, 40, 32), (40, 36, 72)]      # (nr, nth, nphi)
+# theta pencils over many lanes of the contiguous sweep with a ragged last lane (129 = 16*8 + 1
+# and 300 = 37*8 + 4 rows): the density closure's theta-wall values are copied out of the
+# lanes that hold a pencil's first and last rows
+THETA_SHAPES = [(10, 129, 12), (8, 300, 10)]
 
 
-@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("shape", SHAPES + THETA_SHAPES)
 def test_rhs_matches_oracle(mhd, shape):
     nr, nth, nphi = shape
     c = Config(nr=nr, nth=nth, nphi=nphi, omega0=0.3, damping_time=50.0)
