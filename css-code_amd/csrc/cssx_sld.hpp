@@ -177,9 +177,12 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
 // at the level, which neighbouring lanes and wavefronts load as their own columns (cache
 // hits).  No LDS and no barriers: a thread whose column lies outside the grid returns at once.
 // The tile index comes from the flat block index, XCD-contiguous, so that phi-neighbouring
-// rows share one L2.
+// rows share one L2.  Register targets: the A pass (F0 = 5) at 6 (FP32) / 4 (FP64) wavefronts
+// per SIMD fits three / two 8-wavefront workgroups per CU (80 / 128 VGPRs; its few spills are in
+// the lower theta wall's ghost branch); the rho..s pass does not fit the next step without
+// spilling in its main loop.
 template <typename T, int F0, int NF>
-__global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 2))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
+__global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (F0 == 0 ? 3 : 6) : (F0 == 0 ? 2 : 4)))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
                                                   const T *__restrict__ st, const T *__restrict__ vcc,
                                                   T *__restrict__ D, int ntt, int ntp, int rseg) {
     // fields [F0, F0 + NF): rho..s (with the point corrections, which need all five) and A
