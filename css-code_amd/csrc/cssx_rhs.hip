@@ -750,6 +750,10 @@ struct StageFuse {
     // term (diffusion.f90:430-461) the stage subtracts from the entropy's SLD tendency here
     // instead of a separate pass over ds
     const T *ohm;
+    // stage 1 with SLD: the viscous-heating / mass-diffusion corrections of the SLD tendency
+    // (diffusion.f90:375-428) are applied here, at the point, instead of in the SLD march
+    int corr, visc, mass;
+    T gcv;
     T *y1;
     const double *sr1_64, *rbar64, *sbar64;
     const T *sr1_t;   // constant-temperature wall entropy in stage precision/form
@@ -881,21 +885,60 @@ __device__ __forceinline__ void final_point(const Consts<T> &c, const Prof<T> &P
     } else {
         // st may alias F.y1 (the stage state is updated in place): every read of st at this
         // point happened above
-        stage_value(F, c, 0, i, it, ip, ir, drho);
-        stage_value(F, c, 1, i, it, ip, ir, dw);
-        stage_value(F, c, 2, i, it, ip, ir, dvv);
-        stage_value(F, c, 3, i, it, ip, ir, du);
-        if (F.ohm && F.stage == 1 && F.dif) {
-            // SLD Ohmic heating: ds_sld -= B . curl(dA_sld/dt) / (8 pi rho T), the operations of
-            // k_sld_finish in its order
-            const T tt = T(0.5 * kOneOver4Pi) * invrho * (T(1) / t);
-            T difs = F.dif[4 * N + i];
-            difs = difs - B[i] * F.ohm[i] * tt;
-            difs = difs - B[N + i] * F.ohm[N + i] * tt;
-            difs = difs - B[2 * N + i] * F.ohm[2 * N + i] * tt;
-            stage_value(F, c, 4, i, it, ip, ir, ds, true, difs);
+        const bool sld1 = F.stage == 1 && F.dif;
+        if (sld1 && F.corr) {
+            // the SLD march left rho..s uncorrected: viscous heating, then mass diffusion, the
+            // operations of k_sld_col's point corrections in their order (the same state values
+            // and thermodynamics at the point)
+            // (d4 takes the three viscous terms in field order before the mass term; each velocity
+            // tendency enters d4 before its own mass correction; one velocity field at a time)
+            const T d0 = F.dif[i];
+            const T itemp = T(1) / t;
+            T d4 = F.dif[4 * N + i];
+            stage_value(F, c, 0, i, it, ip, ir, drho, true, d0);
+            {
+                T d1 = F.dif[N + i];
+                if (F.visc) d4 = d4 - W * d1 * itemp;
+                if (F.mass) d1 = d1 - invrho * W * d0;
+                stage_value(F, c, 1, i, it, ip, ir, dw, true, d1);
+            }
+            {
+                T d2 = F.dif[2 * N + i];
+                if (F.visc) d4 = d4 - V * d2 * itemp;
+                if (F.mass) d2 = d2 - invrho * V * d0;
+                stage_value(F, c, 2, i, it, ip, ir, dvv, true, d2);
+            }
+            {
+                T d3 = F.dif[3 * N + i];
+                if (F.visc) d4 = d4 - U * d3 * itemp;
+                if (F.mass) d3 = d3 - invrho * U * d0;
+                stage_value(F, c, 3, i, it, ip, ir, du, true, d3);
+            }
+            if (F.mass) d4 = d4 - F.gcv * invrho * d0;
+            if (F.ohm) {
+                const T tt = T(0.5 * kOneOver4Pi) * invrho * (T(1) / t);
+                d4 = d4 - B[i] * F.ohm[i] * tt;
+                d4 = d4 - B[N + i] * F.ohm[N + i] * tt;
+                d4 = d4 - B[2 * N + i] * F.ohm[2 * N + i] * tt;
+            }
+            stage_value(F, c, 4, i, it, ip, ir, ds, true, d4);
         } else {
-            stage_value(F, c, 4, i, it, ip, ir, ds);
+            stage_value(F, c, 0, i, it, ip, ir, drho);
+            stage_value(F, c, 1, i, it, ip, ir, dw);
+            stage_value(F, c, 2, i, it, ip, ir, dvv);
+            stage_value(F, c, 3, i, it, ip, ir, du);
+            if (F.ohm && sld1) {
+                // SLD Ohmic heating: ds_sld -= B . curl(dA_sld/dt) / (8 pi rho T), the operations
+                // of k_sld_finish in its order
+                const T tt = T(0.5 * kOneOver4Pi) * invrho * (T(1) / t);
+                T difs = F.dif[4 * N + i];
+                difs = difs - B[i] * F.ohm[i] * tt;
+                difs = difs - B[N + i] * F.ohm[N + i] * tt;
+                difs = difs - B[2 * N + i] * F.ohm[2 * N + i] * tt;
+                stage_value(F, c, 4, i, it, ip, ir, ds, true, difs);
+            } else {
+                stage_value(F, c, 4, i, it, ip, ir, ds);
+            }
         }
         if (c.magnetic) {
             stage_value(F, c, 5, i, it, ip, ir, da0);
@@ -1382,7 +1425,8 @@ static int curl_impl(cssx_mhd *m, const T *A, T *B, hipStream_t stream, bool zer
 }
 
 template <typename T>
-static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream, T *ohm = nullptr);
+static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream, T *ohm = nullptr,
+                    bool defer_corr = false);
 
 // RHS(call_num) of state st into dv.  With a stage descriptor F (F.stage > 0) the last kernel
 // applies that RK4 stage instead of writing the full tendency; with sld_dif the SLD diffusion
@@ -1455,13 +1499,15 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
     // tendency, its speed buffer and (RK4) the Ohmic curl, which nothing before the final kernel
     // reads; off a captured stream and with phase timing off it runs on its own stream
     bool fork_sld = false;
+    // the SLD point corrections are left to this RHS's fused stage-1 kernel
+    const bool defer_corr = F && F->stage == 1 && F->corr;
     if (sld_dif) {
         fork_sld = sld_fork_ok(m, stream);
         if (fork_sld) {
             if (hipEventRecord(m->ev_sld_fork, stream) != hipSuccess ||
                 hipStreamWaitEvent(m->sld_side, m->ev_sld_fork, 0) != hipSuccess)
                 return CSSX_EHIP;
-            rc = sld_impl<T>(m, st, B, sld_dif, m->sld_side, F ? const_cast<T *>(F->ohm) : nullptr);
+            rc = sld_impl<T>(m, st, B, sld_dif, m->sld_side, F ? const_cast<T *>(F->ohm) : nullptr, defer_corr);
             // recorded whatever sld_impl returned: every exit below joins it
             const hipError_t e = hipEventRecord(m->ev_sld_join, m->sld_side);
             if (rc || e != hipSuccess) {
@@ -1469,7 +1515,7 @@ static int rhs_body(cssx_mhd *m, const T *st, T *dv, T *Bout, hipStream_t stream
                 else (void)hipStreamSynchronize(m->sld_side);
                 return rc ? rc : CSSX_EHIP;
             }
-        } else if ((rc = sld_impl<T>(m, st, B, sld_dif, stream, F ? const_cast<T *>(F->ohm) : nullptr))) {
+        } else if ((rc = sld_impl<T>(m, st, B, sld_dif, stream, F ? const_cast<T *>(F->ohm) : nullptr, defer_corr))) {
             return rc;
         }
     }
@@ -1678,7 +1724,7 @@ static bool sld_split() {
 template <typename T>
 // ohm: with Ohmic heating, a 3-field buffer that receives the curl of the A tendency and the
 // caller applies the heating term (the fused RK4 stage 1); null: k_sld_finish applies it to dif
-static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream, T *ohm) {
+static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t stream, T *ohm, bool defer_corr) {
     int rc = ensure_scratch<T>(m);
     if (rc) return rc;
     Scratch<T> &S = work<T>(m);
@@ -1717,7 +1763,20 @@ static int sld_impl(cssx_mhd *m, const T *st, const T *B, T *dif, hipStream_t st
                        hipLaunchKernelGGL((k_sld_col<T, 5, 3>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st,
                                           (const T *)vcc, dif, ntt, ntp, rseg));
     };
-    if constexpr (sizeof(T) == 4) {
+    if (defer_corr) {
+        // the point corrections are the caller's: rho..s in two passes of at most three fields,
+        // which fit three (FP32) / two (FP64) workgroups per CU, then A
+        CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * 7.0,
+                   hipLaunchKernelGGL((k_sld_col<T, 0, 3>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st,
+                                      (const T *)vcc, dif, ntt, ntp, rseg));
+        CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * 5.0,
+                   hipLaunchKernelGGL((k_sld_col<T, 3, 2>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st,
+                                      (const T *)vcc, dif, ntt, ntp, rseg));
+        if (na > 0.0)
+            CSSX_TIMED(CSSX_PHASE_SLD, stream, ts * N * (2.0 * na + 1.0),
+                       hipLaunchKernelGGL((k_sld_col<T, 5, 3>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st,
+                                          (const T *)vcc, dif, ntt, ntp, rseg));
+    } else if constexpr (sizeof(T) == 4) {
         if (sp.last >= 5 && !sld_split())
             CSSX_TIMED(CSSX_PHASE_SLD, stream, b_march,
                        hipLaunchKernelGGL((k_sld_col<T, 0, 8>), grid, dim3(64 * kSldRows), 0, stream, c, P, sp, st,
@@ -1750,9 +1809,16 @@ static int rk4_impl(cssx_mhd *m, double *y0, double dt, int nsteps, hipStream_t 
     T *dif = m->p.sld_on ? S.dif : nullptr;
     // SLD Ohmic heating folded into stage 1: the curl of the A tendency goes to S.k, which the
     // fused stages do not use (they write no tendency field)
-    const bool ohm = dif && make_sld<T>(m).ohmic;
+    const SldParams<T> sp = make_sld<T>(m);
+    const bool ohm = dif && sp.ohmic;
     constexpr bool mixed = sizeof(T) != sizeof(double);
     StageFuse<T> F{};
+    // stage 1's final kernel applies the SLD point corrections (the march then runs rho..s in
+    // passes of fewer registers)
+    F.corr = dif && (sp.visc || sp.mass);
+    F.visc = sp.visc;
+    F.mass = sp.mass;
+    F.gcv = sp.gcv;
     F.dt = dt;
     F.f = make_bound_flags(m);
     F.y0 = y0;

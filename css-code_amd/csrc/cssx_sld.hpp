@@ -8,7 +8,8 @@
 //                 r face's flux carried to the next level (every r face limited once); the
 //                 theta and phi faces of each cell from the neighbouring columns at the level
 //                 (cache hits).  Then the viscous heating and mass diffusion corrections
-//                 (:375-428), which need only the point itself.
+//                 (:375-428), which need only the point itself (under RK4 the stage-1 final
+//                 kernel applies them, and rho..s runs as two smaller passes).
 //   (curl of the A tendency, the same batched derivative sweeps as Compute_B without its wall
 //    zeroing, VectorCurlA of :437)
 //   k_sld_finish  Ohmic heating (:430-461)
@@ -177,17 +178,19 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
 // at the level, which neighbouring lanes and wavefronts load as their own columns (cache
 // hits).  No LDS and no barriers: a thread whose column lies outside the grid returns at once.
 // The tile index comes from the flat block index, XCD-contiguous, so that phi-neighbouring
-// rows share one L2.  Register targets: the A pass (F0 = 5) at 6 (FP32) / 4 (FP64) wavefronts
-// per SIMD fits three / two 8-wavefront workgroups per CU (80 / 128 VGPRs; its few spills are in
-// the lower theta wall's ghost branch); the rho..s pass does not fit the next step without
-// spilling in its main loop.
+// rows share one L2.  Register targets: passes of at most three fields at 6 (FP32) / 4 (FP64)
+// wavefronts per SIMD fit three / two 8-wavefront workgroups per CU (80 / 128 VGPRs; their few
+// spills are in the lower theta wall's ghost branch); the rho..s pass with the point corrections
+// does not fit the next step without spilling in its main loop.
 template <typename T, int F0, int NF>
-__global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (F0 == 0 ? 3 : 6) : (F0 == 0 ? 2 : 4)))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
+__global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (NF >= 5 ? 3 : 6) : (NF >= 5 ? 2 : 4)))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
                                                   const T *__restrict__ st, const T *__restrict__ vcc,
                                                   T *__restrict__ D, int ntt, int ntp, int rseg) {
-    // fields [F0, F0 + NF): rho..s (with the point corrections, which need all five) and A
-    static_assert((F0 == 0 && NF >= 5 && NF <= 8) || (F0 >= 5 && F0 + NF <= 8), "field group");
-    constexpr bool CORR = F0 == 0;
+    // fields [F0, F0 + NF): rho..s with the point corrections (which need all five), parts of
+    // rho..s without them (the caller applies them: RK4 stage 1's final kernel), and A
+    static_assert((F0 == 0 && NF >= 5 && NF <= 8) || (F0 + NF <= 5 && NF <= 3) || (F0 >= 5 && F0 + NF <= 8),
+                  "field group");
+    constexpr bool CORR = F0 == 0 && NF >= 5;
     // FP32 states hold rho and s as fluctuations already; FP64 states are absolute
     constexpr bool FLUCT = sizeof(T) == 4;
     const long lb = xcd_block(blockIdx.x, gridDim.x);
@@ -300,7 +303,8 @@ __global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(s
         for (int q = 0; q < NF; q++) {
             const int f = F0 + q;
             if (f < sp.first) {   // rho without mass diffusion
-                out[0] = T(0);
+                if constexpr (CORR) out[0] = T(0);
+                else at_row(dl(0, ir), oc) = T(0);
                 continue;
             }
             const T b = f == 0 ? bg0 : f == 4 ? bg4 : T(0);
@@ -330,8 +334,8 @@ __global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(s
             const T Dt = sp.mt[f] * (gtt * (ft_hi - ft_lo));
             const T Dp = gp * (sp.mt[f] * (fp_hi - fp_lo));
             const T o = (Dr + Dt) + Dp;
-            if (f < 5) out[f] = o;
-            else if (c.magnetic) at_row(dl(f, ir), oc) = o;       // A: no correction
+            if (CORR && f < 5) out[f] = o;
+            else if (f < 5 || c.magnetic) at_row(dl(f, ir), oc) = o;       // no correction here
         }
         if constexpr (CORR) {
             // viscous heating and mass diffusion (:375-428), at the point
