@@ -180,10 +180,11 @@ __device__ __forceinline__ long xcd_block(long b, long nb) {
 // The tile index comes from the flat block index, XCD-contiguous, so that phi-neighbouring
 // rows share one L2.  Register targets: passes of at most three fields at 6 (FP32) / 4 (FP64)
 // wavefronts per SIMD fit three / two 8-wavefront workgroups per CU (80 / 128 VGPRs; their few
-// spills are in the lower theta wall's ghost branch); the rho..s pass with the point corrections
-// does not fit the next step without spilling in its main loop.
+// spills are in the lower theta wall's ghost branch), the FP32 two-field pass at 8 fits four
+// (64 VGPRs; spills outside the main loop); the rho..s pass with the point corrections does not
+// fit the next step without spilling in its main loop.
 template <typename T, int F0, int NF>
-__global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (NF >= 5 ? 3 : 6) : (NF >= 5 ? 2 : 4)))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
+__global__ __launch_bounds__(64 * kSldRows) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (NF >= 5 ? 3 : NF <= 2 ? 8 : 6) : (NF >= 5 ? 2 : 4)))) void k_sld_col(Consts<T> c, Prof<T> P, SldParams<T> sp,
                                                   const T *__restrict__ st, const T *__restrict__ vcc,
                                                   T *__restrict__ D, int ntt, int ntp, int rseg) {
     // fields [F0, F0 + NF): rho..s with the point corrections (which need all five), parts of
