@@ -137,6 +137,33 @@ def test_step_sld_matches_oracle(mhd, variant, scheme):
     sol.close()
 
 
+@pytest.mark.parametrize("variant", ["explicit", "flags"])
+def test_rk4_sld_variants_match_oracle(mhd, variant):
+    """RK4 stage 1 applies the SLD viscous-heating and mass-diffusion corrections in its final
+    kernel: beside the explicit diffusion (Laplacian, whose terms share that kernel), and with
+    both corrections off (the rho..s march then keeps its single pass)."""
+    print(f"rk4 sld parity {variant}", flush=True)
+    m = Model(Config(nr=24, nth=40, nphi=32, omega0=0.3, **VARIANTS[variant]))
+    st = oracle.bound(m, _state(m, seed=5))
+    # the explicit diffusion (kappa 1e-3) limits the stable step at this grid
+    dt, n = (1e-5 if variant == "explicit" else 1e-4), 2
+    ref = oracle.rk4(m, st, dt, n, "rk4")
+    assert np.isfinite(ref).all()
+    sol = mhd(m)
+    y = torch.as_tensor(st, device="cuda").contiguous()
+    sol.rk4(y, dt, n, "f64", "rk4")
+    got = y.cpu().numpy()
+    d0 = np.abs(ref - st).max(axis=(1, 2, 3))
+    chg = np.abs((got - st) - (ref - st)).max(axis=(1, 2, 3))
+    # the explicit terms' FP64 parity is 1e-9 of the change (their own RK4 test's bound)
+    assert np.all(chg <= 1e-9 * d0 + 1e-300), (chg, d0)
+    y32 = torch.as_tensor(st, device="cuda").contiguous()
+    sol.rk4(y32, dt, n, "f32", "rk4")
+    chg32 = np.abs((y32.cpu().numpy() - st) - (ref - st)).max(axis=(1, 2, 3))
+    assert np.all(chg32 <= 1e-3 * d0 + 1e-300), (chg32, d0)
+    sol.close()
+
+
 def test_c2_shape_sld_step_fp32_close(mhd):
     """Benchmark shape 128x256x512 with SLD on: FP32 and FP64 two-step states agree."""
     c = Config(nr=128, nth=256, nphi=512, sld_coef=0.1)
