@@ -137,13 +137,17 @@ def test_step_sld_matches_oracle(mhd, variant, scheme):
     sol.close()
 
 
-@pytest.mark.parametrize("variant", ["explicit", "flags"])
+RK4_VARIANTS = dict(VARIANTS, nomass=dict(sld_coef=0.3, Mass_Diffusion=False))
+
+
+@pytest.mark.parametrize("variant", ["explicit", "flags", "nomass"])
 def test_rk4_sld_variants_match_oracle(mhd, variant):
     """RK4 stage 1 applies the SLD viscous-heating and mass-diffusion corrections in its final
-    kernel: beside the explicit diffusion (Laplacian, whose terms share that kernel), and with
-    both corrections off (the rho..s march then keeps its single pass)."""
+    kernel: beside the explicit diffusion (Laplacian, whose terms share that kernel), with both
+    corrections off (the rho..s march then keeps its single pass), and with viscous heating
+    alone (rho undiffused: the march stores its zero tendency)."""
     print(f"rk4 sld parity {variant}", flush=True)
-    m = Model(Config(nr=24, nth=40, nphi=32, omega0=0.3, **VARIANTS[variant]))
+    m = Model(Config(nr=24, nth=40, nphi=32, omega0=0.3, **RK4_VARIANTS[variant]))
     st = oracle.bound(m, _state(m, seed=5))
     # the explicit diffusion (kappa 1e-3) limits the stable step at this grid
     dt, n = (1e-5 if variant == "explicit" else 1e-4), 2
