@@ -11,8 +11,8 @@ This module adds, beside it, a small network that predicts what a coarse grid's 
   * the network 216 -> 128 -> 128 -> 8 (GELU) is trained with Adam on the GPU by the library's
     own BF16 matrix-core forward/backward kernels (FP32 master weights and accumulation);
   * a coarse run attaches it (cssx_mhd_set_closure): every RHS evaluation adds its tendency; the
-    plain coarse run (SLD only) is the fallback it is judged against, and the filtered fine run
-    is the judge of both.
+    plain coarse run (SLD only) is the fallback it is compared with, and the filtered fine run
+    is the reference for both.
 
 Everything heavy runs in libcssx.so (csrc/cssx_closure.hip); torch holds the device memory.
 """
